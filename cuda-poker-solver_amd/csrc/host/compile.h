@@ -1,0 +1,88 @@
+#pragma once
+// Host tree compiler: spot + bet config -> the flat arrays the MI355X
+// engine consumes (include/ppsg.h). Replaces the reference's Compiler and
+// the layout half of GpuPostflopSolver::init (cuda/gpu_cfr.cu:1019-1827).
+//
+// Node numbering is the reference's pre-order creation order (fold
+// terminals created in action order), so node ids, decide indices and
+// children tables are interchangeable with the reference's. Layout
+// differences (all deliberate):
+//   - combo lists, chance maps, per-card segments and strength tables are
+//     stored once per board class, not per node;
+//   - all row offsets are 64-bit;
+//   - reach rows hold exactly the opponent's list (not max(n0, n1));
+//   - showdown tables carry per-combo prefix bounds so the device does two
+//     scans instead of per-card correction loops.
+#include <array>
+#include <cstdint>
+#include <vector>
+
+#include "ppsg.h"
+#include "pps/gpu_cfr.h"
+
+namespace pps {
+namespace gpu {
+
+struct CompiledTree {
+  PostflopSpot spot;
+  pf::BetConfig cfg;
+  bool empty = false;
+
+  // base lists (positive weight, not blocked by the spot board, ascending)
+  int n[2] = {0, 0};
+  std::vector<uint8_t> cards[2];  // 2 per base slot
+  std::vector<double> w[2];
+  std::vector<int> sameOther[2];  // base slot of the same cards on the other side
+
+  // board classes (class 0 = spot board), in first-seen order
+  std::vector<std::array<uint8_t, 5>> clsBoard;
+  std::vector<int32_t> clsNBoard, clsParent, clsDealt;
+  std::vector<int32_t> clsN[2];
+  std::vector<int64_t> clsListOff[2];
+  std::vector<int32_t> listBase[2];
+  std::vector<uint16_t> listCards[2];
+  std::vector<int16_t> listSame[2];
+  std::vector<uint16_t> toParent[2];
+  std::vector<int64_t> fromParentOff[2];
+  std::vector<int16_t> fromParent[2];
+  std::vector<int32_t> segOff[2];
+  std::vector<uint16_t> segSlot[2];
+  std::vector<uint16_t> sorted[2];
+  std::vector<uint32_t> sdMeta[2];
+
+  // nodes
+  int64_t numNodes = 0;
+  int maxDepth = 0;
+  std::vector<uint8_t> kind;
+  std::vector<int8_t> owner;
+  std::vector<uint8_t> na;
+  std::vector<int32_t> depth, cls, childBase, childFlat, parent;
+  std::vector<int64_t> potBase, sc0, sc1, pc0, pc1;
+  std::vector<int64_t> regOff, cfvOff, reachOff[2];
+  std::vector<float> termVal;
+  int64_t regTotal = 0, cfvTotal = 0, reachTotal[2] = {0, 0};
+  std::vector<float> rootReach[2];
+  std::vector<pf::TreeAction> actFlat;
+  std::vector<int32_t> actBase;
+  std::vector<int32_t> decideNodes;  // decide index -> node id
+  std::vector<int32_t> decideOf;     // node id -> decide index or -1
+
+  int listSize(int p, int node) const { return clsN[p][cls[node]]; }
+  ppsg_tree_desc desc() const;
+  // Algorithmic f32 row bytes of one iteration (both half-steps).
+  double iterationBytes() const;
+};
+
+// Compiles a spot. Throws std::runtime_error on a bet config wider than
+// the engine supports (more than 32 actions at a node).
+CompiledTree compileSpot(const PostflopSpot& spot, const pf::BetConfig& cfg);
+
+// Disjoint weighted pair mass over the base lists (cuda/gpu_cfr.cu:871-888).
+double pairMass(const CompiledTree& t);
+
+// Discount coefficients for schedule index t of an `iters`-long solve
+// (cuda/gpu_cfr.cu:585-614; "dcfr-g2" = paper DCFR, gamma 2, no restart).
+void discountCoefs(int t, int iters, const std::string& algo, double out[3]);
+
+}  // namespace gpu
+}  // namespace pps

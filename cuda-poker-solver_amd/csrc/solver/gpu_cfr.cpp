@@ -1,0 +1,781 @@
+// GpuPostflopSolver (include/pps/gpu_cfr.h) on top of the host compiler
+// and the MI355X engine's C ABI. Mirrors the reference API's semantics
+// (cuda/gpu_cfr.cu:1829-2853); host reductions are f64 over the f32 rows.
+#include "pps/gpu_cfr.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+
+#include "../host/compile.h"
+#include "ppsg.h"
+
+namespace pps {
+namespace gpu {
+
+namespace {
+
+void check(int rc) {
+  if (rc != 0) throw std::runtime_error(std::string("ppsg: ") + ppsg_last_error());
+}
+
+bool knownAlgo(const std::string& a) { return a == "dcfr" || a == "hs30" || a == "dcfr-g2"; }
+
+std::vector<float> scheduleTable(int64_t from, int64_t to, int planned, const std::string& algo) {
+  std::vector<float> tab((size_t)to * 3);
+  double c0[3];
+  discountCoefs((int)from, planned, algo, c0);
+  for (int64_t t = 0; t < to; ++t) {
+    double c[3];
+    if (t < from) {
+      std::copy(c0, c0 + 3, c);  // never read again: the counter only grows
+    } else {
+      discountCoefs((int)t, planned, algo, c);
+    }
+    for (int k = 0; k < 3; ++k) tab[(size_t)t * 3 + k] = (float)c[k];
+  }
+  return tab;
+}
+
+}  // namespace
+
+struct GpuPostflopSolver::Impl {
+  CompiledTree T;
+  ppsg_t* eng = nullptr;
+  std::string lastAlgo;
+  double Z = 0.0;
+  double lastMs = -1.0;
+
+  ~Impl() {
+    if (eng) ppsg_destroy(eng);
+  }
+
+  std::vector<float> read(int buf, int64_t off, int64_t n) const {
+    std::vector<float> v((size_t)n);
+    check(ppsg_read(eng, buf, off, n, v.data()));
+    return v;
+  }
+
+  // Range-weighted root value of traverser tr (EV or BR walk) / pair mass.
+  double rootValue(int tr, int mode) {
+    std::vector<float> cfv(T.n[tr]);
+    check(ppsg_walk(eng, tr, mode, cfv.data()));
+    double v = 0.0;
+    for (int i = 0; i < T.n[tr]; ++i) v += T.w[tr][i] * (double)cfv[i];
+    return Z > 0.0 ? v / Z : 0.0;
+  }
+
+  double explNow() {
+    const double v0 = rootValue(0, PPSG_MODE_BR);
+    const double v1 = rootValue(1, PPSG_MODE_BR);
+    return (v0 + v1 - (double)T.spot.pot) / 2.0;
+  }
+
+  const std::vector<int32_t>& listBase() const { return T.listBase[0]; }
+
+  // node-local slot of base slot `base` in player p's list at node u, or -1
+  int localSlot(int p, int u, int base) const {
+    const int cl = T.cls[u];
+    const int64_t off = T.clsListOff[p][cl];
+    const int n = T.clsN[p][cl];
+    const int32_t* b = T.listBase[p].data() + off;
+    const int32_t* it = std::lower_bound(b, b + n, base);
+    return (it != b + n && *it == base) ? (int)(it - b) : -1;
+  }
+
+  // Opponent mass valid against each of tr's combos at u, from layout tr's
+  // reach row (per-card sums + identical-combo correction, f64).
+  std::vector<double> massAt(int tr, int u, std::vector<float>* reachOut = nullptr) const {
+    const int op = 1 - tr, cl = T.cls[u];
+    const int nO = T.clsN[op][cl], nT = T.clsN[tr][cl];
+    std::vector<float> reach = read(tr == 0 ? PPSG_REACH0 : PPSG_REACH1, T.reachOff[tr][u], nO);
+    const int64_t offO = T.clsListOff[op][cl], offT = T.clsListOff[tr][cl];
+    double perCard[52] = {0.0}, tot = 0.0;
+    for (int j = 0; j < nO; ++j) {
+      const int b = T.listBase[op][offO + j];
+      tot += reach[j];
+      perCard[T.cards[op][2 * b]] += reach[j];
+      perCard[T.cards[op][2 * b + 1]] += reach[j];
+    }
+    std::vector<double> m(nT);
+    for (int i = 0; i < nT; ++i) {
+      const int b = T.listBase[tr][offT + i];
+      double v = tot - perCard[T.cards[tr][2 * b]] - perCard[T.cards[tr][2 * b + 1]];
+      const int so = T.sameOther[tr][b];
+      if (so >= 0) {
+        const int s = localSlot(op, u, so);
+        if (s >= 0) v += reach[s];
+      }
+      m[i] = v;
+    }
+    if (reachOut) *reachOut = std::move(reach);
+    return m;
+  }
+
+  std::vector<ActionLabel> labels(int u) const {
+    std::vector<ActionLabel> out;
+    for (int a = 0; a < T.na[u]; ++a) {
+      const pf::TreeAction& act = T.actFlat[T.actBase[u] + a];
+      out.push_back({act.kind, act.amount});
+    }
+    return out;
+  }
+
+  int decideIndexCheck(int idx) const {
+    if (idx < 0 || idx >= (int)T.decideNodes.size())
+      throw std::runtime_error("decide node index " + std::to_string(idx) + " out of range (" +
+                               std::to_string(T.decideNodes.size()) + " decide nodes)");
+    return T.decideNodes[idx];
+  }
+
+  int64_t childCount(int u) const {
+    const int64_t end = u + 1 < T.numNodes ? T.childBase[u + 1] : (int64_t)T.childFlat.size();
+    return end - T.childBase[u];
+  }
+};
+
+GpuPostflopSolver::GpuPostflopSolver(const PostflopSpot& spot, const pf::BetConfig& cfg)
+    : impl_(new Impl()) {
+  try {
+    init(spot, cfg);
+  } catch (...) {
+    delete impl_;
+    impl_ = nullptr;
+    throw;
+  }
+}
+
+GpuPostflopSolver::~GpuPostflopSolver() { delete impl_; }
+
+void GpuPostflopSolver::init(const PostflopSpot& spot, const pf::BetConfig& cfg) {
+  Impl* I = impl_;
+  I->T = compileSpot(spot, cfg);
+  if (I->T.empty) return;
+  numNodes_ = (int)I->T.numNodes;
+  maxDepth_ = I->T.maxDepth;
+  I->Z = pairMass(I->T);
+  const ppsg_tree_desc d = I->T.desc();
+  check(ppsg_create(&d, &I->eng));
+}
+
+void GpuPostflopSolver::reset() {
+  iterationsRun_ = 0;
+  totalIters_ = 0;
+  if (impl_->T.empty) return;
+  check(ppsg_reset(impl_->eng));
+}
+
+void GpuPostflopSolver::replayChunks(int iterations, double target) {
+  Impl* I = impl_;
+  const int chunk = 128;
+  int done = 0;
+  double ms = 0.0;
+  while (done < iterations) {
+    const int n = std::min(chunk, iterations - done);
+    check(ppsg_iterate(I->eng, n));
+    double m = 0.0;
+    check(ppsg_last_iterate_ms(I->eng, &m));
+    ms += m;
+    done += n;
+    iterationsRun_ = done;
+    totalIters_ += n;
+    if (target > 0.0 && done < iterations && I->explNow() <= target) break;
+  }
+  I->lastMs = ms;
+}
+
+void GpuPostflopSolver::solve(int iterations, const std::string& algo, double target) {
+  Impl* I = impl_;
+  iterationsRun_ = 0;
+  if (I->T.empty) return;
+  if (!knownAlgo(algo))
+    throw std::runtime_error("unknown algo '" + algo + "' (expected dcfr, dcfr-g2 or hs30)");
+  reset();
+  if (iterations <= 0) return;
+  I->lastAlgo = algo;
+  const std::vector<float> tab = scheduleTable(0, iterations, iterations, algo);
+  check(ppsg_set_schedule(I->eng, tab.data(), iterations));
+  replayChunks(iterations, target);
+}
+
+void GpuPostflopSolver::continueSolve(int iterations, double target) {
+  Impl* I = impl_;
+  iterationsRun_ = 0;
+  if (I->T.empty) return;
+  if (totalIters_ == 0) throw std::runtime_error("continueSolve: no prior solve() to continue");
+  if (I->lastAlgo == "hs30" || !knownAlgo(I->lastAlgo))
+    throw std::runtime_error("continueSolve: the last solve used '" + I->lastAlgo +
+                             "', whose schedule depends on the planned iteration count; "
+                             "continuation is defined for DCFR only");
+  if (iterations <= 0) return;
+  const int64_t t0 = totalIters_;
+  if (t0 + iterations > (int64_t)std::numeric_limits<int>::max())
+    throw std::runtime_error("continueSolve: schedule index overflow");
+  const int64_t tEnd = t0 + iterations;
+  const std::vector<float> tab = scheduleTable(t0, tEnd, (int)tEnd, I->lastAlgo);
+  check(ppsg_set_schedule(I->eng, tab.data(), tEnd));
+  replayChunks(iterations, target);
+}
+
+pf::NodeStats GpuPostflopSolver::stats() {
+  Impl* I = impl_;
+  pf::NodeStats st{};
+  if (I->T.empty || !(I->Z > 0.0)) return st;
+  if (const char* sm = std::getenv("GPU_CFR_SEED_STRAT")) {
+    // probe profile of cuda/gpu_cfr.cu:2062-2097
+    const int mode = std::atoi(sm);
+    std::vector<float> rows((size_t)std::max<int64_t>(I->T.regTotal, 0), 0.0f);
+    for (size_t mi = 0; mi < I->T.decideNodes.size(); ++mi) {
+      const int u = I->T.decideNodes[mi];
+      const int dpt = I->T.depth[u];
+      bool pick;
+      switch (mode) {
+        case 1: pick = true; break;
+        case 2: pick = dpt <= 2; break;
+        case 3: pick = dpt > 2; break;
+        case 4: pick = dpt == 0; break;
+        case 5: pick = dpt == 1; break;
+        case 6: pick = dpt == 2; break;
+        case 7: pick = mi == 1; break;
+        case 8: pick = mi == 2; break;
+        case 9: pick = dpt == 1 && mi != 1; break;
+        default: pick = false;
+      }
+      if (!pick || I->T.na[u] < 2) continue;
+      const int nC = I->T.listSize(I->T.owner[u], u);
+      for (int c = 0; c < nC; ++c) {
+        const float p0 = (float)(0.15 + 0.7 * ((c % 8) / 7.0));
+        rows[I->T.regOff[u] + c] = p0;
+        rows[I->T.regOff[u] + nC + c] = 1.0f - p0;
+      }
+    }
+    if (!rows.empty()) check(ppsg_write(I->eng, PPSG_STRAT, 0, (int64_t)rows.size(), rows.data()));
+    std::fprintf(stderr, "seeded strat rows (mode %d)\n", mode);
+  }
+  st.ev0 = I->rootValue(0, PPSG_MODE_EV);
+  st.ev1 = I->rootValue(1, PPSG_MODE_EV);
+  const double v0 = I->rootValue(0, PPSG_MODE_BR);
+  const double v1 = I->rootValue(1, PPSG_MODE_BR);
+  st.expl = (v0 + v1 - (double)I->T.spot.pot) / 2.0;
+  st.pairMass = (int64_t)(I->Z * (1LL << 20));
+  return st;
+}
+
+namespace {
+// Range-weighted average frequencies of a first-street decide node.
+std::vector<double> aggregateFreqs(const std::vector<float>& rows, const std::vector<double>& w,
+                                   int na, int nC) {
+  std::vector<double> freq(na, 0.0), tot(nC, 0.0);
+  double z = 0.0;
+  for (int c = 0; c < nC; ++c) {
+    for (int a = 0; a < na; ++a) tot[c] += rows[(size_t)a * nC + c];
+    z += w[c];
+  }
+  if (z <= 0.0) return freq;
+  for (int a = 0; a < na; ++a) {
+    double acc = 0.0;
+    for (int c = 0; c < nC; ++c)
+      acc += w[c] * (tot[c] > 0.0 ? rows[(size_t)a * nC + c] / tot[c] : 1.0 / na);
+    freq[a] = acc / z;
+  }
+  return freq;
+}
+}  // namespace
+
+std::vector<double> GpuPostflopSolver::nodeStrategy(int decideIdx) {
+  Impl* I = impl_;
+  if (I->T.empty || decideIdx < 0 || decideIdx >= (int)I->T.decideNodes.size()) return {};
+  const int u = I->T.decideNodes[decideIdx];
+  if (I->T.clsNBoard[I->T.cls[u]] != I->T.spot.nBoard) return {};
+  const int p = I->T.owner[u], na = I->T.na[u], nC = I->T.listSize(p, u);
+  return aggregateFreqs(I->read(PPSG_STRAT, I->T.regOff[u], (int64_t)na * nC), I->T.w[p], na, nC);
+}
+
+std::vector<double> GpuPostflopSolver::rootStrategy() {
+  Impl* I = impl_;
+  if (I->T.empty) return {};
+  const int na = I->T.na[0], nC = I->T.listSize(0, 0);
+  return aggregateFreqs(I->read(PPSG_STRAT, I->T.regOff[0], (int64_t)na * nC), I->T.w[0], na, nC);
+}
+
+int GpuPostflopSolver::numDecideNodes() const { return (int)impl_->T.decideNodes.size(); }
+
+DecideNodeInfo GpuPostflopSolver::decideNode(int decideIdx) const {
+  const Impl* I = impl_;
+  const int u = I->decideIndexCheck(decideIdx);
+  DecideNodeInfo out;
+  out.nodeId = u;
+  out.player = I->T.owner[u];
+  out.depth = I->T.depth[u];
+  out.nBoard = I->T.clsNBoard[I->T.cls[u]];
+  out.nCombos = I->T.listSize(out.player, u);
+  out.actions = I->labels(u);
+  for (int a = 0; a < I->T.na[u]; ++a) out.children.push_back(I->T.childFlat[I->T.childBase[u] + a]);
+  return out;
+}
+
+ComboStrategy GpuPostflopSolver::comboStrategy(int decideIdx) {
+  Impl* I = impl_;
+  if (I->T.empty) throw std::runtime_error("solver compiled to empty ranges");
+  const int u = I->decideIndexCheck(decideIdx);
+  const int p = I->T.owner[u], na = I->T.na[u], nC = I->T.listSize(p, u);
+  ComboStrategy out;
+  out.actions = I->labels(u);
+  const int64_t off = I->T.clsListOff[p][I->T.cls[u]];
+  out.combos.assign(I->T.listBase[p].begin() + off, I->T.listBase[p].begin() + off + nC);
+  const std::vector<float> rows = I->read(PPSG_STRAT, I->T.regOff[u], (int64_t)na * nC);
+  out.freqs.assign((size_t)na * nC, 0.0);
+  for (int c = 0; c < nC; ++c) {
+    double s = 0.0;
+    for (int a = 0; a < na; ++a) s += rows[(size_t)a * nC + c];
+    for (int a = 0; a < na; ++a)
+      out.freqs[(size_t)a * nC + c] = s > 0.0 ? (double)rows[(size_t)a * nC + c] / s : 1.0 / na;
+  }
+  return out;
+}
+
+ComboEv GpuPostflopSolver::rootEvPerCombo() {
+  Impl* I = impl_;
+  ComboEv out;
+  if (I->T.empty) return out;
+  for (int tr = 0; tr < 2; ++tr) {
+    std::vector<float> cfv(I->T.n[tr]);
+    check(ppsg_walk(I->eng, tr, PPSG_MODE_EV, cfv.data()));
+    const int op = 1 - tr;
+    double perCard[52] = {0.0}, tot = 0.0;
+    for (int j = 0; j < I->T.n[op]; ++j) {
+      tot += I->T.w[op][j];
+      perCard[I->T.cards[op][2 * j]] += I->T.w[op][j];
+      perCard[I->T.cards[op][2 * j + 1]] += I->T.w[op][j];
+    }
+    for (int i = 0; i < I->T.n[tr]; ++i) {
+      double m = tot - perCard[I->T.cards[tr][2 * i]] - perCard[I->T.cards[tr][2 * i + 1]];
+      if (I->T.sameOther[tr][i] >= 0) m += I->T.w[op][I->T.sameOther[tr][i]];
+      out.combos[tr].push_back(i);
+      out.mass[tr].push_back(m);
+      out.ev[tr].push_back(m > 0.0 ? (double)cfv[i] / m : 0.0);
+    }
+  }
+  return out;
+}
+
+NodeEv GpuPostflopSolver::nodeEv(int decideIdx) {
+  Impl* I = impl_;
+  if (I->T.empty) throw std::runtime_error("solver compiled to empty ranges");
+  const int u = I->decideIndexCheck(decideIdx);
+  const int na = I->T.na[u], dec = I->T.owner[u];
+  NodeEv out;
+  out.decider = dec;
+  out.actions = I->labels(u);
+  out.actionEv.assign(na, {});
+  for (int tr = 0; tr < 2; ++tr) {
+    check(ppsg_walk(I->eng, tr, PPSG_MODE_EV, nullptr));
+    NodeEv::Side& S = out.side[tr];
+    const int cl = I->T.cls[u];
+    const int nC = I->T.clsN[tr][cl];
+    const int64_t off = I->T.clsListOff[tr][cl];
+    S.combos.assign(I->T.listBase[tr].begin() + off, I->T.listBase[tr].begin() + off + nC);
+    S.mass = I->massAt(tr, u);
+    const std::vector<float> row = I->read(PPSG_CFV, I->T.cfvOff[u], nC);
+    S.ev.assign(nC, 0.0);
+    for (int i = 0; i < nC; ++i) S.ev[i] = S.mass[i] > 0.0 ? (double)row[i] / S.mass[i] : 0.0;
+    const std::vector<double>& wSelf = I->T.w[tr];
+    if (tr == dec) {
+      const std::vector<float> srows = I->read(PPSG_STRAT, I->T.regOff[u], (int64_t)na * nC);
+      for (int a = 0; a < na; ++a) {
+        const int ch = I->T.childFlat[I->T.childBase[u] + a];
+        const int nChild = I->T.listSize(tr, ch);
+        const std::vector<float> crow = I->read(PPSG_CFV, I->T.cfvOff[ch], nChild);
+        std::vector<double>& pc = out.actionEv[a].perCombo;
+        pc.assign(nC, 0.0);
+        for (int i = 0; i < nC; ++i) {
+          if (S.mass[i] <= 0.0) continue;
+          const int slot = I->T.kind[ch] == PPSG_CHANCE ? I->localSlot(tr, ch, S.combos[i]) : i;
+          if (slot >= 0) pc[i] = (double)crow[slot] / S.mass[i];
+        }
+        double num = 0.0, den = 0.0, num0 = 0.0, den0 = 0.0;
+        for (int i = 0; i < nC; ++i) {
+          if (S.mass[i] <= 0.0) continue;
+          double s = 0.0;
+          for (int b = 0; b < na; ++b) s += srows[(size_t)b * nC + i];
+          const double wb = wSelf[S.combos[i]] * S.mass[i];
+          num0 += wb * pc[i];
+          den0 += wb;
+          if (s <= 0.0) continue;
+          const double wt = wb * (double)srows[(size_t)a * nC + i] / s;
+          num += wt * pc[i];
+          den += wt;
+        }
+        out.actionEv[a].aggEv = den > 0.0 ? num / den : (den0 > 0.0 ? num0 / den0 : 0.0);
+      }
+      for (int i = 0; i < nC; ++i) {
+        double s = 0.0;
+        for (int a = 0; a < na; ++a) s += srows[(size_t)a * nC + i];
+        if (s <= 0.0) continue;
+        double ev = 0.0;
+        for (int a = 0; a < na; ++a)
+          ev += (double)srows[(size_t)a * nC + i] / s * out.actionEv[a].perCombo[i];
+        S.ev[i] = ev;
+      }
+    }
+    double num = 0.0, den = 0.0;
+    for (int i = 0; i < nC; ++i) {
+      if (S.mass[i] <= 0.0) continue;
+      const double wt = wSelf[S.combos[i]] * S.mass[i];
+      num += wt * S.ev[i];
+      den += wt;
+    }
+    S.aggEv = den > 0.0 ? num / den : 0.0;
+  }
+  return out;
+}
+
+RunoutSummary GpuPostflopSolver::runoutSummary(int decideIdx, int actionIdx) {
+  Impl* I = impl_;
+  if (I->T.empty) throw std::runtime_error("solver compiled to empty ranges");
+  const int u = I->decideIndexCheck(decideIdx);
+  if (actionIdx < 0 || actionIdx >= I->T.na[u])
+    throw std::runtime_error("action index " + std::to_string(actionIdx) + " out of range");
+  const int ch = I->T.childFlat[I->T.childBase[u] + actionIdx];
+  if (I->T.kind[ch] != PPSG_CHANCE) throw std::runtime_error("action does not lead to a deal");
+  RunoutSummary out;
+  const int nb = (int)I->childCount(ch);
+  // first decide node under a branch (breadth-first through further deals)
+  auto firstDecide = [&](int start) -> int {
+    if (I->T.kind[start] == PPSG_DECIDE) return start;
+    std::vector<int> q{start};
+    for (size_t qi = 0; qi < q.size(); ++qi) {
+      const int x = q[qi];
+      for (int64_t k = 0; k < I->childCount(x); ++k) {
+        const int c = I->T.childFlat[I->T.childBase[x] + k];
+        if (I->T.kind[c] == PPSG_DECIDE) return c;
+        if (I->T.kind[c] == PPSG_CHANCE) q.push_back(c);
+      }
+    }
+    return -1;
+  };
+  out.branches.resize(nb);
+  int firstIdx = -1;
+  for (int b = 0; b < nb; ++b) {
+    const int bc = I->T.childFlat[I->T.childBase[ch] + b];
+    out.branches[b].card = I->T.clsDealt[I->T.cls[bc]];
+    const int fd = firstDecide(bc);
+    if (fd >= 0) {
+      out.branches[b].nextDecide = I->T.decideOf[fd];
+      if (firstIdx < 0) firstIdx = I->T.decideOf[fd];
+    }
+  }
+  if (firstIdx < 0) return out;
+  const int fu = I->T.decideNodes[firstIdx];
+  out.decider = I->T.owner[fu];
+  out.actions = I->labels(fu);
+  for (int tr = 0; tr < 2; ++tr) {
+    check(ppsg_walk(I->eng, tr, PPSG_MODE_EV, nullptr));
+    const std::vector<double>& wSelf = I->T.w[tr];
+    for (int b = 0; b < nb; ++b) {
+      const int bi = out.branches[b].nextDecide;
+      if (bi < 0) continue;
+      const int f = I->T.decideNodes[bi];
+      const int cl = I->T.cls[f];
+      const int nC = I->T.clsN[tr][cl], nOpp = I->T.clsN[1 - tr][cl];
+      const int64_t offC = I->T.clsListOff[tr][cl];
+      std::vector<float> reach;
+      const std::vector<double> mass = I->massAt(tr, f, &reach);
+      const std::vector<float> row = I->read(PPSG_CFV, I->T.cfvOff[f], nC);
+      double num = 0.0, den = 0.0;
+      for (int i = 0; i < nC; ++i) {
+        if (mass[i] <= 0.0) continue;
+        const double wt = wSelf[I->T.listBase[tr][offC + i]] * mass[i];
+        num += wt * (double)row[i] / mass[i];
+        den += wt;
+      }
+      out.branches[b].aggEv[tr] = den > 0.0 ? num / den : 0.0;
+      const int bna = I->T.na[f];
+      if (I->T.owner[f] != tr && bna == (int)out.actions.size()) {
+        const std::vector<float> srows = I->read(PPSG_STRAT, I->T.regOff[f], (int64_t)bna * nOpp);
+        std::vector<double>& freq = out.branches[b].freq;
+        freq.assign(bna, 0.0);
+        double z = 0.0;
+        for (int c = 0; c < nOpp; ++c) {
+          double s = 0.0;
+          for (int a = 0; a < bna; ++a) s += srows[(size_t)a * nOpp + c];
+          if (s <= 0.0) continue;
+          z += reach[c];
+          for (int a = 0; a < bna; ++a) freq[a] += reach[c] * (double)srows[(size_t)a * nOpp + c] / s;
+        }
+        if (z > 0.0)
+          for (double& f2 : freq) f2 /= z;
+      }
+    }
+  }
+  return out;
+}
+
+std::vector<uint8_t> GpuPostflopSolver::playerCards(int player) const {
+  if (player < 0 || player > 1) throw std::runtime_error("player must be 0 (OOP) or 1 (IP)");
+  return impl_->T.cards[player];
+}
+
+int GpuPostflopSolver::numBaseCombos(int player) const {
+  if (player < 0 || player > 1) throw std::runtime_error("player must be 0 (OOP) or 1 (IP)");
+  return impl_->T.n[player];
+}
+
+std::vector<double> GpuPostflopSolver::playerWeights(int player) const {
+  if (player < 0 || player > 1) throw std::runtime_error("player must be 0 (OOP) or 1 (IP)");
+  return impl_->T.w[player];
+}
+
+TreeStructure GpuPostflopSolver::treeStructure() const {
+  const CompiledTree& T = impl_->T;
+  TreeStructure t;
+  t.kinds = T.kind;
+  t.childBase = T.childBase;
+  t.children = T.childFlat;
+  t.boards.reserve((size_t)T.numNodes * 5);
+  for (int64_t u = 0; u < T.numNodes; ++u) {
+    const int cl = T.cls[u];
+    for (int i = 0; i < 5; ++i) t.boards.push_back(i < T.clsNBoard[cl] ? T.clsBoard[cl][i] : 0);
+  }
+  return t;
+}
+
+double GpuPostflopSolver::iterationBytes() const { return impl_->T.iterationBytes(); }
+
+std::vector<float> GpuPostflopSolver::rows(int which) const {
+  if (impl_->T.empty) return {};
+  return impl_->read(which == 0 ? PPSG_REGRET : PPSG_STRAT, 0, impl_->T.regTotal);
+}
+
+void GpuPostflopSolver::setRows(int which, const std::vector<float>& v) {
+  if (impl_->T.empty) return;
+  if ((int64_t)v.size() != impl_->T.regTotal) throw std::runtime_error("row size mismatch");
+  check(ppsg_write(impl_->eng, which == 0 ? PPSG_REGRET : PPSG_STRAT, 0, (int64_t)v.size(), v.data()));
+}
+
+void GpuPostflopSolver::halfStep(int tr, int t, const std::string& algo, int planned) {
+  Impl* I = impl_;
+  if (I->T.empty) return;
+  if (!knownAlgo(algo)) throw std::runtime_error("unknown algo '" + algo + "'");
+  if (tr != 0 && tr != 1) throw std::runtime_error("traverser must be 0 or 1");
+  int64_t saved = 0;
+  check(ppsg_get_counter(I->eng, &saved));
+  const std::vector<float> tab = scheduleTable(t, (int64_t)t + 1, planned, algo);
+  check(ppsg_set_schedule(I->eng, tab.data(), (int64_t)t + 1));
+  check(ppsg_set_counter(I->eng, t));
+  check(ppsg_half_step(I->eng, tr));
+  check(ppsg_set_counter(I->eng, saved));
+}
+
+std::vector<double> GpuPostflopSolver::profileIterations(int iters) {
+  Impl* I = impl_;
+  if (I->T.empty || iters <= 0) return {};
+  if (totalIters_ == 0 || I->lastAlgo == "hs30")
+    throw std::runtime_error("profileIterations: needs a prior dcfr / dcfr-g2 solve");
+  const int64_t t0 = totalIters_, tEnd = t0 + iters;
+  const std::vector<float> tab = scheduleTable(t0, tEnd, (int)tEnd, I->lastAlgo);
+  check(ppsg_set_schedule(I->eng, tab.data(), tEnd));
+  std::vector<double> out(5, 0.0);
+  check(ppsg_profile(I->eng, iters, out.data(), 5));
+  totalIters_ = tEnd;
+  return out;
+}
+
+std::vector<int64_t> GpuPostflopSolver::engineInfo() const {
+  std::vector<int64_t> out(6, 0);
+  if (impl_->T.empty) return out;
+  check(ppsg_info(impl_->eng, out.data(), 6));
+  return out;
+}
+
+double GpuPostflopSolver::rootValue(int tr, int mode, std::vector<float>* rootCfv) {
+  Impl* I = impl_;
+  if (I->T.empty) return 0.0;
+  if (tr != 0 && tr != 1) throw std::runtime_error("traverser must be 0 or 1");
+  std::vector<float> cfv(I->T.n[tr]);
+  check(ppsg_walk(I->eng, tr, mode, cfv.data()));
+  double v = 0.0;
+  for (int i = 0; i < I->T.n[tr]; ++i) v += I->T.w[tr][i] * (double)cfv[i];
+  if (rootCfv) *rootCfv = cfv;
+  return I->Z > 0.0 ? v / I->Z : 0.0;
+}
+
+double GpuPostflopSolver::lastSolveDeviceMs() const { return impl_->lastMs; }
+
+// ---------------------------------------------------------------------------
+// Solution files: the reference's self-describing "PPS1" v1 layout
+// (cuda/gpu_cfr.cu:2686-2853), little-endian, byte-compatible. The tree is
+// recompiled from the stored spot + config and validated by the row count.
+// ---------------------------------------------------------------------------
+
+namespace {
+constexpr uint32_t kMagic = 0x31535050;  // "PPS1"
+constexpr uint32_t kVersion = 1;
+
+struct Writer {
+  FILE* f;
+  const std::string& path;
+  void raw(const void* p, size_t n) {
+    if (n && std::fwrite(p, 1, n, f) != n) throw std::runtime_error("write failed: " + path);
+  }
+  template <typename T>
+  void val(T v) {
+    raw(&v, sizeof(T));
+  }
+  void doubles(const std::vector<double>& v) {
+    val<int32_t>((int32_t)v.size());
+    raw(v.data(), v.size() * sizeof(double));
+  }
+};
+
+struct Reader {
+  FILE* f = nullptr;
+  std::string path;
+  explicit Reader(const std::string& p) : path(p) {
+    f = std::fopen(p.c_str(), "rb");
+    if (!f) throw std::runtime_error("cannot open solution file: " + p);
+  }
+  ~Reader() {
+    if (f) std::fclose(f);
+  }
+  void raw(void* p, size_t n) {
+    if (n && std::fread(p, 1, n, f) != n) throw std::runtime_error("solution file truncated: " + path);
+  }
+  template <typename T>
+  T val() {
+    T v;
+    raw(&v, sizeof(T));
+    return v;
+  }
+  std::vector<double> doubles() {
+    const int32_t n = val<int32_t>();
+    if (n < 0 || n > (1 << 24)) throw std::runtime_error("solution file corrupt: bad length: " + path);
+    std::vector<double> v((size_t)n);
+    raw(v.data(), v.size() * sizeof(double));
+    return v;
+  }
+};
+}  // namespace
+
+void GpuPostflopSolver::save(const std::string& path) const {
+  const Impl* I = impl_;
+  if (I->T.empty) throw std::runtime_error("cannot save an empty solver");
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) throw std::runtime_error("cannot open for writing: " + path);
+  try {
+    Writer w{f, path};
+    const PostflopSpot& s = I->T.spot;
+    const pf::BetConfig& c = I->T.cfg;
+    w.val<uint32_t>(kMagic);
+    w.val<uint32_t>(kVersion);
+    w.val<int32_t>(s.nBoard);
+    w.raw(s.board, sizeof(s.board));
+    w.val<int64_t>(s.pot);
+    w.val<int64_t>(s.stack);
+    w.raw(s.oop.w, sizeof(s.oop.w));
+    w.raw(s.ip.w, sizeof(s.ip.w));
+    w.doubles(c.betFracs);
+    w.doubles(c.raiseMults);
+    w.val<int8_t>((int8_t)c.betAllIn);
+    w.val<int8_t>((int8_t)c.betGeometric);
+    w.val<int32_t>(c.maxRaises);
+    w.val<double>(c.addAllinThreshold);
+    w.val<double>(c.forceAllinThreshold);
+    w.val<double>(c.mergingThreshold);
+    w.val<int64_t>(I->T.numNodes);
+    w.val<int64_t>(totalIters_);
+    w.val<int32_t>((int32_t)I->lastAlgo.size());
+    w.raw(I->lastAlgo.data(), I->lastAlgo.size());
+    w.val<int64_t>(I->T.regTotal);
+    const std::vector<float> reg = I->read(PPSG_REGRET, 0, I->T.regTotal);
+    const std::vector<float> str = I->read(PPSG_STRAT, 0, I->T.regTotal);
+    w.raw(reg.data(), reg.size() * 4);
+    w.raw(str.data(), str.size() * 4);
+    if (std::fflush(f) != 0) throw std::runtime_error("write failed: " + path);
+  } catch (...) {
+    std::fclose(f);
+    throw;
+  }
+  std::fclose(f);
+}
+
+GpuPostflopSolver::GpuPostflopSolver(const std::string& solutionPath) : impl_(new Impl()) {
+  try {
+    Reader r(solutionPath);
+    if (r.val<uint32_t>() != kMagic) throw std::runtime_error("not a solution file: " + solutionPath);
+    const uint32_t ver = r.val<uint32_t>();
+    if (ver != kVersion)
+      throw std::runtime_error("solution file version " + std::to_string(ver) +
+                               " unsupported (expected " + std::to_string(kVersion) +
+                               "): " + solutionPath);
+    PostflopSpot spot;
+    spot.nBoard = r.val<int32_t>();
+    if (spot.nBoard < 3 || spot.nBoard > 5)
+      throw std::runtime_error("solution file corrupt: bad board size");
+    r.raw(spot.board, sizeof(spot.board));
+    spot.pot = r.val<int64_t>();
+    spot.stack = r.val<int64_t>();
+    r.raw(spot.oop.w, sizeof(spot.oop.w));
+    r.raw(spot.ip.w, sizeof(spot.ip.w));
+    pf::BetConfig cfg;
+    cfg.betFracs = r.doubles();
+    cfg.raiseMults = r.doubles();
+    cfg.betAllIn = r.val<int8_t>() != 0;
+    cfg.betGeometric = r.val<int8_t>() != 0;
+    cfg.maxRaises = r.val<int32_t>();
+    cfg.addAllinThreshold = r.val<double>();
+    cfg.forceAllinThreshold = r.val<double>();
+    cfg.mergingThreshold = r.val<double>();
+    (void)r.val<int64_t>();  // node count (the row count validates the tree)
+    const int64_t savedIters = r.val<int64_t>();
+    const int32_t algoLen = r.val<int32_t>();
+    if (algoLen < 0 || algoLen > 15) throw std::runtime_error("solution file corrupt: bad algo length");
+    std::string algo((size_t)algoLen, '\0');
+    r.raw(algo.data(), (size_t)algoLen);
+    if (!knownAlgo(algo)) throw std::runtime_error("solution file corrupt: unknown algo '" + algo + "'");
+    init(spot, cfg);
+    Impl* I = impl_;
+    if (I->T.empty)
+      throw std::runtime_error(
+          "solution spot compiled to an empty solver (empty or fully board-blocked range)");
+    const int64_t regTotal = r.val<int64_t>();
+    if (regTotal != I->T.regTotal)
+      throw std::runtime_error("solution file does not match this engine's compiled tree (row table " +
+                               std::to_string(regTotal) + " vs " + std::to_string(I->T.regTotal) +
+                               " — different engine version or spot semantics)");
+    if (savedIters < 0 || savedIters > std::numeric_limits<int>::max())
+      throw std::runtime_error("solution file corrupt: bad iteration count");
+    std::vector<float> reg((size_t)regTotal), str((size_t)regTotal);
+    r.raw(reg.data(), reg.size() * 4);
+    r.raw(str.data(), str.size() * 4);
+    check(ppsg_write(I->eng, PPSG_REGRET, 0, regTotal, reg.data()));
+    check(ppsg_write(I->eng, PPSG_STRAT, 0, regTotal, str.data()));
+    check(ppsg_set_counter(I->eng, savedIters));
+    I->lastAlgo = algo;
+    totalIters_ = savedIters;
+  } catch (...) {
+    delete impl_;
+    impl_ = nullptr;
+    throw;
+  }
+}
+
+void GpuPostflopSolver::debugDump() {
+  Impl* I = impl_;
+  if (I->T.empty) return;
+  for (int which = 0; which < 2; ++which) {
+    const std::vector<float> v = I->read(which == 0 ? PPSG_STRAT : PPSG_REGRET, 0, I->T.regTotal);
+    long long bad = 0;
+    for (float x : v) bad += !std::isfinite(x);
+    std::fprintf(stderr, "%s nonfinite=%lld of %lld\n", which == 0 ? "strat" : "regret", bad,
+                 (long long)I->T.regTotal);
+  }
+}
+
+}  // namespace gpu
+}  // namespace pps

@@ -1,0 +1,298 @@
+"""ctypes view of the solver's C ABI (include/pps_capi.h, lib/libppsg.so).
+
+This is the binding a non-C++ host would write against the drop-in C ABI;
+tests and the benchmark use it to drive the MI355X engine directly.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libppsg.so")
+
+
+class Spot(ctypes.Structure):
+    _fields_ = [("n_board", ctypes.c_int32), ("board", ctypes.c_uint8 * 5),
+                ("oop", ctypes.c_double * 1326), ("ip", ctypes.c_double * 1326),
+                ("pot", ctypes.c_int64), ("stack", ctypes.c_int64)]
+
+
+class BetConfig(ctypes.Structure):
+    _fields_ = [("n_bets", ctypes.c_int32), ("bets", ctypes.c_double * 16),
+                ("n_raises", ctypes.c_int32), ("raises", ctypes.c_double * 16),
+                ("bet_allin", ctypes.c_int32), ("bet_geo", ctypes.c_int32),
+                ("max_raises", ctypes.c_int32), ("add_allin", ctypes.c_double),
+                ("force_allin", ctypes.c_double), ("merging", ctypes.c_double)]
+
+
+# every function include/pps_capi.h and include/ppsg.h declare
+CAPI_SYMBOLS = [
+    "pps_last_error", "pps_hand_key", "pps_parse_range", "pps_bet_actions", "pps_schedule",
+    "pps_compile_info", "pps_solver_create", "pps_solver_load", "pps_solver_destroy",
+    "pps_solver_solve", "pps_solver_continue", "pps_solver_reset", "pps_solver_iterations_run",
+    "pps_solver_total_iterations", "pps_solver_stats", "pps_solver_root_strategy",
+    "pps_solver_node_strategy", "pps_solver_num_decide_nodes", "pps_solver_decide_node",
+    "pps_solver_combo_strategy", "pps_solver_num_nodes", "pps_solver_max_depth",
+    "pps_solver_num_base_combos", "pps_solver_save", "pps_solver_read_rows",
+    "pps_solver_write_rows", "pps_solver_half_step", "pps_solver_root_value",
+    "pps_solver_iteration_bytes", "pps_solver_last_solve_ms", "pps_solver_profile",
+    "pps_solver_engine_info",
+]
+PPSG_SYMBOLS = [
+    "ppsg_create", "ppsg_destroy", "ppsg_set_schedule", "ppsg_set_counter", "ppsg_get_counter",
+    "ppsg_reset", "ppsg_iterate", "ppsg_half_step", "ppsg_walk", "ppsg_read", "ppsg_write",
+    "ppsg_last_iterate_ms", "ppsg_info", "ppsg_profile", "ppsg_last_error",
+]
+
+_lib = None
+P = ctypes.POINTER
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"solver library not built: {LIB_PATH}")
+        L = ctypes.CDLL(LIB_PATH)
+        L.pps_last_error.restype = ctypes.c_char_p
+        L.pps_hand_key.restype = ctypes.c_uint64
+        L.pps_hand_key.argtypes = [P(ctypes.c_uint8), ctypes.c_int]
+        L.pps_parse_range.argtypes = [ctypes.c_char_p, P(ctypes.c_double)]
+        L.pps_bet_actions.argtypes = [ctypes.c_int64, ctypes.c_int64, P(BetConfig), ctypes.c_int64,
+                                      ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, P(ctypes.c_int64), ctypes.c_int]
+        L.pps_schedule.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, P(ctypes.c_double)]
+        L.pps_compile_info.argtypes = [P(Spot), P(BetConfig), P(ctypes.c_int64), P(ctypes.c_double)]
+        L.pps_solver_create.argtypes = [P(Spot), P(BetConfig), P(ctypes.c_void_p)]
+        L.pps_solver_load.argtypes = [ctypes.c_char_p, P(ctypes.c_void_p)]
+        L.pps_solver_destroy.argtypes = [ctypes.c_void_p]
+        L.pps_solver_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p,
+                                       ctypes.c_double]
+        L.pps_solver_continue.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        L.pps_solver_reset.argtypes = [ctypes.c_void_p]
+        L.pps_solver_iterations_run.argtypes = [ctypes.c_void_p]
+        L.pps_solver_total_iterations.restype = ctypes.c_int64
+        L.pps_solver_total_iterations.argtypes = [ctypes.c_void_p]
+        L.pps_solver_stats.argtypes = [ctypes.c_void_p, P(ctypes.c_double)]
+        L.pps_solver_root_strategy.argtypes = [ctypes.c_void_p, P(ctypes.c_double), ctypes.c_int]
+        L.pps_solver_node_strategy.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_double),
+                                               ctypes.c_int]
+        L.pps_solver_num_decide_nodes.argtypes = [ctypes.c_void_p]
+        L.pps_solver_decide_node.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_int32),
+                                             P(ctypes.c_int64), ctypes.c_int]
+        L.pps_solver_num_nodes.argtypes = [ctypes.c_void_p]
+        L.pps_solver_max_depth.argtypes = [ctypes.c_void_p]
+        L.pps_solver_save.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        L.pps_solver_read_rows.restype = ctypes.c_int64
+        L.pps_solver_read_rows.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_float),
+                                           ctypes.c_int64]
+        L.pps_solver_write_rows.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_float),
+                                            ctypes.c_int64]
+        L.pps_solver_half_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_char_p, ctypes.c_int]
+        L.pps_solver_root_value.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                            P(ctypes.c_float), P(ctypes.c_double)]
+        L.pps_solver_iteration_bytes.restype = ctypes.c_double
+        L.pps_solver_iteration_bytes.argtypes = [ctypes.c_void_p]
+        L.pps_solver_profile.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_double)]
+        L.pps_solver_engine_info.argtypes = [ctypes.c_void_p, P(ctypes.c_int64)]
+        L.pps_solver_last_solve_ms.restype = ctypes.c_double
+        L.pps_solver_last_solve_ms.argtypes = [ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+class CapiError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc < 0:
+        raise CapiError(lib().pps_last_error().decode())
+    return rc
+
+
+def make_spot(board, w0, w1, pot, stack):
+    s = Spot()
+    s.n_board = len(board)
+    for i, c in enumerate(board):
+        s.board[i] = int(c)
+    for i in range(1326):
+        s.oop[i] = float(w0[i])
+        s.ip[i] = float(w1[i])
+    s.pot = int(pot)
+    s.stack = int(stack)
+    return s
+
+
+def make_config(bets=(), raises=(), bet_allin=False, bet_geo=False, max_raises=0,
+                add_allin=1.5, force_allin=0.15, merging=0.1):
+    c = BetConfig()
+    c.n_bets = len(bets)
+    for i, b in enumerate(bets):
+        c.bets[i] = float(b)
+    c.n_raises = len(raises)
+    for i, r in enumerate(raises):
+        c.raises[i] = float(r)
+    c.bet_allin = int(bool(bet_allin))
+    c.bet_geo = int(bool(bet_geo))
+    c.max_raises = int(max_raises)
+    c.add_allin = add_allin
+    c.force_allin = force_allin
+    c.merging = merging
+    return c
+
+
+def hand_key(cards):
+    arr = (ctypes.c_uint8 * len(cards))(*[int(c) for c in cards])
+    return int(lib().pps_hand_key(arr, len(cards)))
+
+
+def parse_range(text):
+    w = (ctypes.c_double * 1326)()
+    _check(lib().pps_parse_range(text.encode(), w))
+    return np.frombuffer(w, dtype=np.float64).copy()
+
+
+def bet_actions(pot_base, stack, cfg, sc0, sc1, actor, after_allin, num_bets, num_streets):
+    out = (ctypes.c_int64 * 96)()
+    n = _check(lib().pps_bet_actions(pot_base, stack, ctypes.byref(cfg), sc0, sc1, actor,
+                                     int(after_allin), num_bets, num_streets, out, 96))
+    return [(int(out[2 * i]), int(out[2 * i + 1])) for i in range(n)]
+
+
+def schedule(algo, t, iters):
+    out = (ctypes.c_double * 3)()
+    _check(lib().pps_schedule(algo.encode(), t, iters, out))
+    return tuple(out)
+
+
+def compile_info(spot, cfg):
+    out = (ctypes.c_int64 * 8)()
+    b = ctypes.c_double()
+    _check(lib().pps_compile_info(ctypes.byref(spot), ctypes.byref(cfg), out, ctypes.byref(b)))
+    keys = ["nodes", "max_depth", "decide", "reg_total", "showdown", "fold", "chance", "classes"]
+    d = dict(zip(keys, [int(x) for x in out]))
+    d["alg_bytes"] = b.value
+    return d
+
+
+class Solver:
+    """GpuPostflopSolver through the C ABI."""
+
+    def __init__(self, spot=None, cfg=None, path=None):
+        h = ctypes.c_void_p()
+        if path is not None:
+            _check(lib().pps_solver_load(path.encode(), ctypes.byref(h)))
+        else:
+            _check(lib().pps_solver_create(ctypes.byref(spot), ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pps_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def solve(self, iters, algo="dcfr", target=-1.0):
+        _check(lib().pps_solver_solve(self.h, int(iters), algo.encode(), float(target)))
+
+    def continue_solve(self, iters, target=-1.0):
+        _check(lib().pps_solver_continue(self.h, int(iters), float(target)))
+
+    def reset(self):
+        _check(lib().pps_solver_reset(self.h))
+
+    @property
+    def iterations_run(self):
+        return _check(lib().pps_solver_iterations_run(self.h))
+
+    @property
+    def total_iterations(self):
+        return int(lib().pps_solver_total_iterations(self.h))
+
+    @property
+    def num_nodes(self):
+        return _check(lib().pps_solver_num_nodes(self.h))
+
+    @property
+    def num_decide_nodes(self):
+        return _check(lib().pps_solver_num_decide_nodes(self.h))
+
+    def stats(self):
+        out = (ctypes.c_double * 4)()
+        _check(lib().pps_solver_stats(self.h, out))
+        return {"ev_oop": out[0], "ev_ip": out[1], "exploitability": out[2],
+                "pair_mass": int(out[3])}
+
+    def root_strategy(self):
+        out = (ctypes.c_double * 64)()
+        n = _check(lib().pps_solver_root_strategy(self.h, out, 64))
+        return [out[i] for i in range(n)]
+
+    def node_strategy(self, idx):
+        out = (ctypes.c_double * 64)()
+        n = _check(lib().pps_solver_node_strategy(self.h, int(idx), out, 64))
+        return [out[i] for i in range(n)]
+
+    def decide_node(self, idx):
+        info = (ctypes.c_int32 * 6)()
+        acts = (ctypes.c_int64 * 96)()
+        _check(lib().pps_solver_decide_node(self.h, int(idx), info, acts, 96))
+        na = info[5]
+        return {"node_id": info[0], "player": info[1], "depth": info[2], "n_board": info[3],
+                "n_combos": info[4],
+                "actions": [(int(acts[3 * a]), int(acts[3 * a + 1])) for a in range(na)],
+                "children": [int(acts[3 * a + 2]) for a in range(na)]}
+
+    def rows(self, which):
+        total = int(lib().pps_solver_read_rows(self.h, which, None, 0))
+        if total < 0:
+            raise CapiError(lib().pps_last_error().decode())
+        buf = np.zeros(total, dtype=np.float32)
+        got = lib().pps_solver_read_rows(self.h, which,
+                                         buf.ctypes.data_as(P(ctypes.c_float)), total)
+        if got < 0:
+            raise CapiError(lib().pps_last_error().decode())
+        return buf
+
+    def set_rows(self, which, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float32)
+        _check(lib().pps_solver_write_rows(self.h, which, arr.ctypes.data_as(P(ctypes.c_float)),
+                                           arr.size))
+
+    def half_step(self, tr, t, algo="dcfr", planned=None):
+        _check(lib().pps_solver_half_step(self.h, tr, t, algo.encode(), planned or t + 1))
+
+    def root_value(self, tr, mode):
+        buf = (ctypes.c_float * 1326)()
+        v = ctypes.c_double()
+        _check(lib().pps_solver_root_value(self.h, tr, mode, buf, ctypes.byref(v)))
+        return v.value, np.frombuffer(buf, dtype=np.float32).copy()
+
+    def save(self, path):
+        _check(lib().pps_solver_save(self.h, path.encode()))
+
+    @property
+    def iteration_bytes(self):
+        return float(lib().pps_solver_iteration_bytes(self.h))
+
+    def profile(self, iters):
+        out = (ctypes.c_double * 5)()
+        _check(lib().pps_solver_profile(self.h, int(iters), out))
+        return {"fwd_ms": out[0], "fwd_launches": int(out[1]), "bwd_ms": out[2],
+                "bwd_launches": int(out[3]), "iters": int(out[4])}
+
+    def engine_info(self):
+        out = (ctypes.c_int64 * 6)()
+        _check(lib().pps_solver_engine_info(self.h, out))
+        keys = ["kernels_per_iter", "device_bytes", "items_per_iter", "graph", "fwd_bytes",
+                "bwd_bytes"]
+        return dict(zip(keys, [int(x) for x in out]))
+
+    @property
+    def last_solve_ms(self):
+        return float(lib().pps_solver_last_solve_ms(self.h))

@@ -1,0 +1,127 @@
+"""Seeded synthetic spots for the measured workloads.
+
+All randomness comes from one xorshift64* stream per spot (the generator of
+the reference's src/common.h:10-23: shifts 12/25/27, multiplier
+0x2545F4914F6CDD1D, doubles from the top 53 bits). Order of draws: the board
+(partial Fisher-Yates over the 52-card deck), then for each player in turn
+(OOP, IP) the subset (Fisher-Yates over that player's unblocked combos,
+keep round(fraction * N)) and one weight in (0, 1] per kept combo in combo
+order.
+"""
+
+MASK = (1 << 64) - 1
+
+
+class XorShift64Star:
+    def __init__(self, seed):
+        self.s = seed & MASK or 0x9E3779B97F4A7C15
+
+    def next_u64(self):
+        s = self.s
+        s ^= s >> 12
+        s ^= (s << 25) & MASK
+        s ^= s >> 27
+        self.s = s
+        return (s * 0x2545F4914F6CDD1D) & MASK
+
+    def next_double(self):
+        return (self.next_u64() >> 11) * (1.0 / (1 << 53))
+
+    def next_below(self, n):
+        return int(self.next_double() * n)
+
+
+COMBOS = [(a, b) for a in range(52) for b in range(a + 1, 52)]
+
+
+def random_spot(seed, n_board, fraction, pot, stack, bets, raises, allin=True, max_raises=0):
+    """Returns a dict: board (card ids), w0/w1 (1326 weights), pot, stack, bets,
+    raises, allin, max_raises."""
+    rng = XorShift64Star(seed)
+    deck = list(range(52))
+    for i in range(n_board):
+        j = i + rng.next_below(52 - i)
+        deck[i], deck[j] = deck[j], deck[i]
+    board = deck[:n_board]
+    weights = []
+    for _ in range(2):
+        live = [k for k, (a, b) in enumerate(COMBOS) if a not in board and b not in board]
+        order = list(live)
+        for i in range(len(order) - 1, 0, -1):
+            j = rng.next_below(i + 1)
+            order[i], order[j] = order[j], order[i]
+        keep = sorted(order[:int(round(fraction * len(order)))])
+        w = [0.0] * 1326
+        for k in keep:
+            w[k] = 1.0 - rng.next_double()  # (0, 1]
+        weights.append(w)
+    return {"board": board, "w0": weights[0], "w1": weights[1], "pot": pot, "stack": stack,
+            "bets": list(bets), "raises": list(raises), "allin": allin,
+            "max_raises": max_raises}
+
+
+def plain_turn():
+    """Plain turn spot (CPU-runnable): seeded 4-card board, full ranges,
+    pot 100, stack 100, bets 50%/100% + all-in, 2.5x raise, paper DCFR."""
+    s = random_spot(1, 4, 1.0, 100, 100, [0.5, 1.0], [2.5], True, 1)
+    s.update(name="plain-turn", iters=500, algo="dcfr-g2")
+    return s
+
+
+def large_turn():
+    """Large turn spot: 60% ranges, pot 60, stack 240, bets 33/75/150% +
+    all-in, raises 2.5x/4x, capped at 3 bets, reference DCFR."""
+    s = random_spot(2, 4, 0.6, 60, 240, [0.33, 0.75, 1.5], [2.5, 4.0], True, 3)
+    s.update(name="large-turn", iters=2000, algo="dcfr")
+    return s
+
+
+SIZES = [0.33, 0.5, 0.75, 1.0, 1.5]
+
+
+def batch_turn(seed):
+    """One spot of the batched-turn stream: pot in [20, 200], SPR in [0.5, 8],
+    1-3 sizes from {33, 50, 75, 100, 150}% + all-in, raise cap 1-3,
+    ranges 30-100% subsets."""
+    rng = XorShift64Star(seed * 7919 + 17)
+    pot = 20 + rng.next_below(181)
+    spr = 0.5 + 7.5 * rng.next_double()
+    stack = max(1, int(round(pot * spr)))
+    nsz = 1 + rng.next_below(3)
+    pool = list(SIZES)
+    sizes = []
+    for _ in range(nsz):
+        sizes.append(pool.pop(rng.next_below(len(pool))))
+    cap = 1 + rng.next_below(3)
+    frac = 0.3 + 0.7 * rng.next_double()
+    s = random_spot(seed, 4, frac, pot, stack, sorted(sizes), [2.5], True, cap)
+    s.update(name=f"batch-turn-{seed}", algo="dcfr")
+    return s
+
+
+def river_spot(seed):
+    """Serving-stream river spot: 5-card board, 1-4 sizes + all-in, cap 1-4."""
+    rng = XorShift64Star(seed * 104729 + 3)
+    pot = 20 + rng.next_below(181)
+    spr = 0.5 + 7.5 * rng.next_double()
+    stack = max(1, int(round(pot * spr)))
+    nsz = 1 + rng.next_below(4)
+    pool = list(SIZES)
+    sizes = sorted(pool.pop(rng.next_below(len(pool))) for _ in range(nsz))
+    cap = 1 + rng.next_below(4)
+    frac = 0.3 + 0.7 * rng.next_double()
+    s = random_spot(seed, 5, frac, pot, stack, sizes, [2.5], True, cap)
+    s.update(name=f"river-{seed}", algo="dcfr")
+    return s
+
+
+def large_flop():
+    """Large flop spot: 50% ranges, pot 60, stack 300, bets 33/75% + all-in,
+    one 2.5x raise per street."""
+    s = random_spot(5, 3, 0.5, 60, 300, [0.33, 0.75], [2.5], True, 2)
+    s.update(name="large-flop", iters=1000, algo="dcfr")
+    return s
+
+
+def card_text(c):
+    return "23456789TJQKA"[c % 13] + "cdhs"[c // 13]

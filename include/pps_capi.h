@@ -1,0 +1,112 @@
+/*
+ * pps_capi — C ABI of the MI355X postflop solver (exported by libppsg.so).
+ *
+ * The reference exposes its solver through pybind11 (python/pps_native.cpp)
+ * over the C++ class pps::gpu::GpuPostflopSolver (cuda/gpu_cfr.h:141-264).
+ * These are the entry points a C FFI (ctypes, cgo, JNI, N-API) binds to
+ * reach the same solver: plain pointers and sizes, status codes, and
+ * pps_last_error() for the message. Each entry cites the reference member
+ * it stands for. Entries marked "no reference counterpart" are additions
+ * (compile-only introspection and row access for parity checks).
+ *
+ * Status: 0 = ok, -1 = error (pps_last_error, thread-local), >= 0 counts
+ * where documented. A solver handle is not thread-safe.
+ */
+#ifndef PPS_CAPI_H
+#define PPS_CAPI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pps_spot {
+  int32_t n_board;          /* 3, 4 or 5 */
+  uint8_t board[5];         /* suit*13 + rank */
+  double oop[1326];         /* combo weights, combo index (a<b) ascending */
+  double ip[1326];
+  int64_t pot, stack;
+} pps_spot;                 /* PostflopSpot, cuda/gpu_cfr.h:35-42 */
+
+typedef struct pps_bet_config {
+  int32_t n_bets;
+  double bets[16];          /* pot fractions */
+  int32_t n_raises;
+  double raises[16];        /* previous-bet multiples */
+  int32_t bet_allin, bet_geo, max_raises;
+  double add_allin, force_allin, merging;   /* 1.5, 0.15, 0.1 by default */
+} pps_bet_config;           /* pf::BetConfig, src/postflop_cfr.h:24-42 */
+
+typedef struct pps_solver pps_solver;
+
+const char* pps_last_error(void);
+
+/* ---- integer layer ---- */
+/* handKey(evaluateN(cards, n)): src/cards.cpp:30-165 + cuda/gpu_cfr.cu:616-621 */
+uint64_t pps_hand_key(const uint8_t* cards, int n);
+/* parseRange: src/range.cpp:176-254; writes 1326 weights */
+int pps_parse_range(const char* text, double* w1326);
+/* pf::betActions: src/postflop_cfr.cpp:18-152; out = (kind, amount) pairs,
+   returns the action count */
+int pps_bet_actions(int64_t pot_base, int64_t stack, const pps_bet_config* cfg, int64_t sc0,
+                    int64_t sc1, int actor, int after_allin, int num_bets, int num_streets,
+                    int64_t* out, int cap);
+/* discount coefficients of schedule index t: cuda/gpu_cfr.cu:585-614 */
+int pps_schedule(const char* algo, int t, int iters, double out3[3]);
+
+/* ---- compile only (no device; no reference counterpart) ----
+   out: [0] nodes [1] max depth [2] decide nodes [3] row total [4] showdown
+   [5] fold [6] chance [7] board classes; *bytes = algorithmic bytes/iter */
+int pps_compile_info(const pps_spot* spot, const pps_bet_config* cfg, int64_t out[8],
+                     double* bytes);
+
+/* ---- solver: GpuPostflopSolver ---- */
+int pps_solver_create(const pps_spot* spot, const pps_bet_config* cfg,
+                      pps_solver** out);                      /* cuda/gpu_cfr.h:145 */
+int pps_solver_load(const char* path, pps_solver** out);     /* cuda/gpu_cfr.h:150 */
+void pps_solver_destroy(pps_solver* s);                       /* cuda/gpu_cfr.h:151 */
+int pps_solver_solve(pps_solver* s, int iters, const char* algo,
+                     double target);                          /* cuda/gpu_cfr.h:161 */
+int pps_solver_continue(pps_solver* s, int iters, double target); /* cuda/gpu_cfr.h:169 */
+int pps_solver_reset(pps_solver* s);                          /* cuda/gpu_cfr.h:173 */
+int pps_solver_iterations_run(pps_solver* s);                 /* cuda/gpu_cfr.h:177 */
+int64_t pps_solver_total_iterations(pps_solver* s);           /* cuda/gpu_cfr.h:181 */
+/* ev0, ev1, expl, pair mass (x2^20 as a double) */
+int pps_solver_stats(pps_solver* s, double out4[4]);          /* cuda/gpu_cfr.h:185 */
+int pps_solver_root_strategy(pps_solver* s, double* out, int cap);   /* cuda/gpu_cfr.h:188 */
+int pps_solver_node_strategy(pps_solver* s, int idx, double* out,
+                             int cap);                        /* cuda/gpu_cfr.h:197 */
+int pps_solver_num_decide_nodes(pps_solver* s);               /* cuda/gpu_cfr.h:201 */
+/* info6: node id, player, depth, n_board, n_combos, n_actions;
+   actions: (kind, amount, child) triples */
+int pps_solver_decide_node(pps_solver* s, int idx, int32_t info6[6], int64_t* actions,
+                           int cap);                          /* cuda/gpu_cfr.h:202 */
+/* combos[n_combos], freqs[n_actions * n_combos]; returns n_combos */
+int pps_solver_combo_strategy(pps_solver* s, int idx, int32_t* combos, double* freqs,
+                              int cap);                       /* cuda/gpu_cfr.h:209 */
+int pps_solver_num_nodes(pps_solver* s);                      /* cuda/gpu_cfr.h:239 */
+int pps_solver_max_depth(pps_solver* s);                      /* cuda/gpu_cfr.h:240 */
+int pps_solver_num_base_combos(pps_solver* s, int player);    /* cuda/gpu_cfr.h:229 */
+int pps_solver_save(pps_solver* s, const char* path);         /* cuda/gpu_cfr.h:245 */
+
+/* ---- parity / measurement hooks (no reference counterpart) ---- */
+/* which 0 = regret, 1 = strategy sum; rows concatenated over decide nodes
+   in creation order, action-major; returns the row total (out == NULL:
+   size query) */
+int64_t pps_solver_read_rows(pps_solver* s, int which, float* out, int64_t cap);
+int pps_solver_write_rows(pps_solver* s, int which, const float* in, int64_t n);
+/* one CFR half-step for traverser tr with schedule index t of `algo` */
+int pps_solver_half_step(pps_solver* s, int tr, int t, const char* algo, int planned);
+/* root value walk: mode 1 EV, 2 BR; fills the root cfv row; returns value */
+int pps_solver_root_value(pps_solver* s, int tr, int mode, float* root_cfv, double* value);
+double pps_solver_iteration_bytes(pps_solver* s);
+/* per-launch event timing of `iters` more iterations (profileIterations) */
+int pps_solver_profile(pps_solver* s, int iters, double out5[5]);
+/* ppsg_info of the solver's engine: 6 values */
+int pps_solver_engine_info(pps_solver* s, int64_t out6[6]);
+double pps_solver_last_solve_ms(pps_solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

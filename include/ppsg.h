@@ -1,0 +1,120 @@
+/*
+ * ppsg — C ABI of the MI355X DCFR engine (libppsg.so).
+ *
+ * Plain pointers and sizes; no C++ or GPU-runtime types. Every call returns
+ * 0 on success or a negative code; ppsg_last_error() gives the message for
+ * the calling thread. A handle is not thread-safe: one thread drives it.
+ *
+ * This is the seam under the reference's engine API: the host compiler
+ * (pps::gpu::compileSpot) produces a ppsg_tree_desc, and the C++
+ * GpuPostflopSolver (include/pps/gpu_cfr.h), the gpu_pfflop CLI and the pps
+ * Python module drive the engine through these calls. Each entry point
+ * names the reference internals it replaces (cuda/gpu_cfr.cu).
+ *
+ * Tree description. Combo lists are per "board class" (the board after a
+ * given sequence of dealt cards): every node of a class shares its lists,
+ * which are the player's base list minus combos holding a dealt card, in
+ * base order. Per-player arrays below are indexed [player][...]; a class's
+ * n entries start at cls_list_off[p][cls].
+ */
+#ifndef PPSG_H
+#define PPSG_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { PPSG_DECIDE = 0, PPSG_SHOWDOWN = 1, PPSG_FOLD = 2, PPSG_CHANCE = 3 };
+enum { PPSG_MODE_CFR = 0, PPSG_MODE_EV = 1, PPSG_MODE_BR = 2 };
+enum { PPSG_REGRET = 0, PPSG_STRAT = 1, PPSG_REACH0 = 2, PPSG_REACH1 = 3, PPSG_CFV = 4 };
+
+typedef struct ppsg_tree_desc {
+  int32_t n_base[2];               /* base list sizes (root class lists) */
+  /* board classes */
+  int32_t n_class;
+  const int32_t* cls_n[2];         /* list size per class */
+  const int64_t* cls_list_off[2];  /* first entry of the class's list */
+  const int32_t* cls_board_n;      /* cards on the board per class (3..5) */
+  int64_t list_total[2];
+  const uint16_t* list_cards[2];   /* c1 | c2 << 8 per list entry */
+  const int16_t* list_same[2];     /* slot of the same two cards in the other
+                                      player's list of the class, or -1 */
+  const uint16_t* to_parent[2];    /* slot in the parent class (class > 0) */
+  const int64_t* from_parent_off[2]; /* per class: start in from_parent */
+  int64_t from_parent_total[2];
+  const int16_t* from_parent[2];   /* per parent slot: slot in this class or -1 */
+  const int32_t* seg_off[2];       /* 53 per class: per-card segment bounds */
+  const uint16_t* seg_slot[2];     /* 2n per class (at 2*cls_list_off): slots
+                                      holding each card; strength order on
+                                      5-card classes, slot order elsewhere */
+  const uint16_t* sorted[2];       /* 5-card classes: slots by strength */
+  const uint32_t* sd_meta[2];      /* 5-card classes, 4 words per traverser
+                                      slot (at 4*cls_list_off): le | te<<16,
+                                      A1 | B1<<16, A2 | B2<<16, c1 | c2<<8 |
+                                      (same+1)<<16 */
+  /* nodes, in the reference's creation order */
+  int64_t n_nodes;
+  int32_t max_depth;
+  const uint8_t* kind;
+  const int8_t* owner;             /* decide: player; fold: folder */
+  const uint8_t* na;
+  const int32_t* depth;
+  const int32_t* cls;
+  const int32_t* child_base;       /* into child_flat */
+  int64_t n_child;
+  const int32_t* child_flat;
+  const int64_t* reg_off;          /* decide: regret/strategy row start */
+  const int64_t* cfv_off;
+  const int64_t* reach_off[2];     /* per traverser layout (aliased rows) */
+  const float* term_val;           /* 8 per node: [tr][win, tie, lose, -] */
+  int64_t reg_total, cfv_total, reach_total[2];
+  const float* root_reach[2];      /* layout tr root row: opponent weights */
+} ppsg_tree_desc;
+
+typedef struct ppsg_engine ppsg_t;
+
+/* Uploads the compiled tree and allocates the f32 rows.
+   Replaces GpuPostflopSolver::init's uploads (cuda/gpu_cfr.cu:1515-1826). */
+int ppsg_create(const ppsg_tree_desc* desc, ppsg_t** out);
+void ppsg_destroy(ppsg_t* h);
+
+/* Discount table: 3 floats (pos, neg, avg) per schedule index [0, n).
+   Replaces the coefTab upload (cuda/gpu_cfr.cu:1884-1893,2018-2031). */
+int ppsg_set_schedule(ppsg_t* h, const float* coef, int64_t n);
+/* Device-resident schedule index (cuda/gpu_cfr.cu:218-220,2837-2839). */
+int ppsg_set_counter(ppsg_t* h, int64_t t);
+int ppsg_get_counter(ppsg_t* h, int64_t* t);
+/* Zero regret + strategy rows and the counter (cuda/gpu_cfr.cu:1842-1855). */
+int ppsg_reset(ppsg_t* h);
+/* Enqueues n DCFR iterations (one captured graph per iteration) and waits.
+   Replaces captureGraph + replayChunks (cuda/gpu_cfr.cu:844-868,1858-1872). */
+int ppsg_iterate(ppsg_t* h, int n);
+/* One CFR half-step for traverser tr at the current counter, outside the
+   graph (rows only; the counter is not advanced). */
+int ppsg_half_step(ppsg_t* h, int tr);
+/* One EV (mode 1) or best-response (mode 2) half-step for traverser tr
+   against the average strategy; copies the root cfv row (n_base[tr]
+   floats) to host. Replaces rootValue's device half (cuda/gpu_cfr.cu:894-916). */
+int ppsg_walk(ppsg_t* h, int tr, int mode, float* root_cfv);
+/* Row transfer: buf is PPSG_REGRET .. PPSG_CFV; offsets in floats. */
+int ppsg_read(ppsg_t* h, int buf, int64_t off, int64_t n, float* host);
+int ppsg_write(ppsg_t* h, int buf, int64_t off, int64_t n, const float* host);
+/* Device time (ms) of the last ppsg_iterate, from events on its stream. */
+int ppsg_last_iterate_ms(ppsg_t* h, double* ms);
+/* Engine facts: [0] kernels per iteration, [1] device bytes allocated,
+   [2] work items per iteration, [3] graph captured (0/1), [4] forward and
+   [5] backward algorithmic f32 row bytes per iteration. */
+int ppsg_info(ppsg_t* h, int64_t* out, int cap);
+/* Runs `iters` CFR iterations launch by launch (no graph), timing every
+   launch with events on the engine stream: out = [forward ms, forward
+   launches, backward ms, backward launches, iterations]. Advances the
+   counter like ppsg_iterate. (No reference counterpart; the reference's
+   GPU_CFR_PROFILE mode plays this role, cuda/gpu_cfr.cu:1895-1987.) */
+int ppsg_profile(ppsg_t* h, int iters, double* out, int cap);
+const char* ppsg_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

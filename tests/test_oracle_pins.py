@@ -16,7 +16,8 @@ import pytest
 
 import orc
 from bruteforce import BR, EV, BruteForce
-from conftest import board_of, card
+from conftest import board_of
+import spots
 
 
 def load(golden_dir, name):
@@ -181,35 +182,8 @@ def test_reference_root_labels():
     assert o.decide_meta(1)["player"] == 1
 
 
-def _weights(combos, seed):
-    rng = np.random.default_rng(seed)
-    w = np.zeros(1326)
-    idx = {}
-    k = 0
-    for x in range(52):
-        for y in range(x + 1, 52):
-            idx[(x, y)] = k
-            k += 1
-    for s in combos:
-        a, b = card(s[:2]), card(s[2:])
-        w[idx[(min(a, b), max(a, b))]] = rng.uniform(0.2, 1.0)
-    return w
-
-
-TINY_SPOTS = {
-    "river": dict(board="Qs9h2d7c3s", oop=["AhKh", "QhQd", "9c9d", "JhTc", "7h7d"],
-                  ip=["AhKh", "KcQc", "8h8d", "AsAd", "7h6h"], pot=100, stack=200,
-                  bets=[0.5, 1.0], raises=[2.5], allin=True),
-    "turn": dict(board="Qs9h2d7c", oop=["AhKh", "QhQd", "JhTc"], ip=["AhKh", "KcQc", "8h8d"],
-                 pot=60, stack=100, bets=[0.75], raises=[], allin=True),
-}
-
-
 def _tiny(name):
-    s = TINY_SPOTS[name]
-    return orc.make_spot(board_of(s["board"]), _weights(s["oop"], 1), _weights(s["ip"], 2),
-                         s["pot"], s["stack"], bets=s["bets"], raises=s["raises"],
-                         bet_allin=s["allin"])
+    return spots.tiny(name)
 
 
 @pytest.mark.parametrize("name,steps", [("river", [0, 1, 2, 7, 30]), ("turn", [0, 1, 4, 15])])
