@@ -174,12 +174,16 @@ def test_runout_summary():
     # OOP checks, IP checks behind -> deal
     ip_node = next(n for n in s.decide_nodes() if n["node_id"] == root["children"][0])
     rs = s.runout_summary(ip_node["index"], 0)
-    assert len(rs["branches"]) == 44
-    assert all(len(b["card"]) == 2 for b in rs["branches"])
+    # one branch per card off the 4-card board
+    assert len(rs["branches"]) == 48
+    cards = [b["card"] for b in rs["branches"]]
+    assert len(set(cards)) == 48 and not set(cards) & {"Qs", "9h", "2d", "7c"}
+    dn = s.decide_nodes()
     for b in rs["branches"]:
-        if len(b["freq"]):
-            close(np.sum(b["freq"]), 1.0, 1e-6, "branch freq sums")
-            close(b["agg_ev"]["oop"] + b["agg_ev"]["ip"], POT, 1e-2, "branch zero-sum")
+        assert 0 <= b["next_decide"] < s.num_decide_nodes
+        assert dn[b["next_decide"]]["n_board"] == 5 and dn[b["next_decide"]]["player"] == 0
+        close(np.sum(b["freq"]), 1.0, 1e-6, "branch freq sums")
+        assert np.isfinite(b["agg_ev"]["oop"]) and np.isfinite(b["agg_ev"]["ip"])
     with pytest.raises(RuntimeError, match="does not lead to a deal"):
         s.runout_summary(0, 1)
 
