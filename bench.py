@@ -64,7 +64,17 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="large-turn", choices=["large-turn", "plain-turn"])
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the plain copy kernel used to scale byte counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        from pps import capi
+        nbytes = 256 << 20
+        ms = capi.copy_probe(nbytes, 5)
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                          "launches": 5, "ms_per_launch": ms,
+                          "gbs": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (spots shard "
               f"across GPUs with no exchange, so N GPUs run N independent benches)",
@@ -89,6 +99,8 @@ def main():
     value = args.steps / wall
     log(f"bench: {args.steps} iterations in {wall * 1e3:.1f} ms wall, {dev_ms:.1f} ms device")
 
+    st = solver.stats()
+    log(f"bench: stats after {solver.total_iterations} iterations: {st}")
     info = solver.engine_info()
     alg_bytes = solver.iteration_bytes
     prof = solver.profile(args.profile_iters)
@@ -101,8 +113,6 @@ def main():
     else:
         kname, nbytes, lms = "fwdKernel", info["fwd_bytes"] / per_iter_fwd, fwd_launch_ms
     achieved = nbytes / (lms * 1e-3) / 1e9
-    st = solver.stats()
-    log(f"bench: stats after {solver.total_iterations} iterations: {st}")
 
     line = {
         "metric": METRIC,

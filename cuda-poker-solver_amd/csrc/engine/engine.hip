@@ -3,26 +3,35 @@
 // One DCFR iteration = two alternating half-steps (traverser 0, then 1);
 // each half-step is a forward pass over depths 0..D (opponent reach) and a
 // backward pass over depths D..0 (traverser counterfactual values with the
-// fused DCFR update). Every (half-step, depth) is one launch over a flat
-// list of work items; the whole iteration (plus the schedule-counter bump)
-// is captured once in a hipGraph and replayed.
+// fused DCFR update). Every (half-step, depth, pass) is one launch over a
+// flat list of work items; the whole iteration (plus the schedule-counter
+// bump) is captured once in a hipGraph and replayed.
 //
-// Work items (int2 {node, arg}):
-//   forward  decide  (opponent acts)   arg = tile        one lane per opponent combo
-//   forward  chance                    arg = branch<<16 | tile
-//   backward decide                    arg = tile        one lane per traverser combo
-//   backward chance                    arg = tile        gather over branches
-//   backward fold / showdown           arg = 0           one workgroup per node
-// Traverser-acting decide nodes have no forward item: their children's reach
-// rows alias the node's row (cuda/gpu_cfr.cu:1485-1514).
+// Work items are 64-byte descriptors built on the host with every row offset
+// already resolved, so a workgroup issues its data loads after a single
+// (scalar, broadcast) descriptor load instead of walking node -> class ->
+// children -> offsets. Sibling rows are contiguous (child k of a node at
+// base + k * stride), so one base offset reaches every child.
 //
-// Every reduction has a fixed order (no float atomics): results do not
-// depend on launch shape, run or batch composition.
+//   forward  decide (opponent acts)  one lane per opponent combo, a tile of 256
+//   forward  chance (branch, tile)   child reach = parent reach / (52-board-4)
+//   backward decide (tile)           RM+ value + fused DCFR update, or the
+//                                    opponent-side sum of child values
+//   backward chance (tile)           sum of child values over surviving branches
+//   backward fold / showdown         one workgroup per node: one blocked f64
+//                                    LDS scan over gathered opponent reach
+// Traverser-acting decide nodes have no forward item: their children's
+// reach rows alias the node's row (cuda/gpu_cfr.cu:1485-1514).
+//
+// No float atomics anywhere: every sum has a fixed order, so results are
+// bitwise reproducible and independent of launch shape.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -32,7 +41,8 @@ namespace {
 
 constexpr int kBlock = 256;  // 4 wave64s
 constexpr int kTile = 256;   // combos per tile item
-constexpr int kMaxBranch = 64;
+constexpr int kMaxList = 1326;
+constexpr int kRowRegs = (kMaxList + kBlock - 1) / kBlock;  // 6
 
 thread_local std::string g_err;
 
@@ -54,41 +64,54 @@ struct Ctl {
   const float* coef;  // 3 floats per schedule index
 };
 
-// Kernel view of one compiled tree (passed by value).
+enum ItemKind : uint8_t {
+  kFwdDecide = 0,
+  kFwdChance = 1,
+  kBwdDecide = 2,
+  kBwdChance = 3,
+  kBwdFold = 4,
+  kBwdShowdown = 5,
+};
+
+// One work item. Field meaning per kind:
+//   kFwdDecide   a regret/strategy row, b node reach row, c children's reach
+//                block (stride n), n opponent combos, na actions
+//   kFwdChance   a child class's to-parent map, b parent reach row, c child
+//                reach row, n child opponent combos, v0 = 1/(52-board-4)
+//   kBwdDecide   a regret/strategy row, b node cfv row, c children's cfv
+//                block, m child stride, n traverser combos, e = actor is
+//                the traverser
+//   kBwdChance   a branch x parent-slot map (child slot or -1), b node cfv
+//                row, c children's cfv block, m child stride, n traverser
+//                combos (parent), na branches
+//   kBwdFold /   a node reach row, b node cfv row, c gather-index list of
+//   kBwdShowdown the class (opponent), d traverser list offset, e gather
+//                entries per lane (8 or 16), f per-card segment table
+//                index (class * 53), n traverser combos, m opponent combos,
+//                v0..v2 payoffs (fold: v0)
+struct alignas(16) Item {
+  int64_t a, b, c, d;
+  int32_t n, m, e, f;
+  uint8_t kind, na;
+  uint16_t tile;
+  float v0, v1, v2;
+};
+static_assert(sizeof(Item) == 64, "work item must be one 64-byte line");
+
 struct Dev {
-  const int32_t* clsN[2];
-  const int64_t* clsOff[2];
-  const int32_t* clsBoardN;
-  const uint16_t* listCards[2];
-  const int16_t* listSame[2];
-  const uint16_t* toParent[2];
-  const int64_t* fromParentOff[2];
-  const int16_t* fromParent[2];
-  const int32_t* segOff[2];
-  const uint16_t* segSlot[2];
-  const uint16_t* sorted[2];
-  const uint32_t* sdMeta[2];
-  const uint8_t* kind;
-  const int8_t* owner;
-  const uint8_t* na;
-  const int32_t* cls;
-  const int32_t* childBase;
-  const int32_t* childFlat;
-  const int64_t* regOff;
-  const int64_t* cfvOff;
-  const int64_t* reachOff[2];
-  const float* termVal;
   float* regret;
   float* strat;
   float* reach[2];
   float* cfv;
+  const uint16_t* listCards[2];
+  const int16_t* listSame[2];
+  const uint16_t* toParent[2];
+  const int16_t* branchMap[2];
+  const int32_t* segOff[2];
+  const uint16_t* gather[2];
+  const uint32_t* sdMeta[2];
   Ctl* ctl;
 };
-
-// ---------------------------------------------------------------------------
-// Strategy from a row set: regret matching+ (CFR) or the normalized
-// strategy sum (EV/BR walks). Uniform when nothing is positive.
-// ---------------------------------------------------------------------------
 
 __device__ __forceinline__ float sigmaOf(float x, float acc, float uniform, bool plus) {
   if (plus) return acc > 0.0f ? (x > 0.0f ? x / acc : 0.0f) : uniform;
@@ -96,163 +119,201 @@ __device__ __forceinline__ float sigmaOf(float x, float acc, float uniform, bool
 }
 
 // ---------------------------------------------------------------------------
-// Forward: opponent reach.
+// Forward.
 // ---------------------------------------------------------------------------
 
 template <int TR>
-__global__ __launch_bounds__(kBlock) void fwdKernel(Dev d, const int2* __restrict__ items,
+__global__ __launch_bounds__(kBlock) void fwdKernel(Dev d, const Item* __restrict__ items,
                                                     int fromStrat) {
-  constexpr int OPP = 1 - TR;
-  const int2 it = items[blockIdx.x];
-  const int node = it.x;
+  const Item it = items[blockIdx.x];
   float* __restrict__ R = d.reach[TR];
-  if (d.kind[node] == PPSG_DECIDE) {
-    const int n = d.clsN[OPP][d.cls[node]];
-    const int c = it.y * kTile + (int)threadIdx.x;
-    if (c >= n) return;
-    const int na = d.na[node];
-    const float r = R[d.reachOff[TR][node] + c];
-    const float* __restrict__ src = (fromStrat ? d.strat : d.regret) + d.regOff[node] + c;
-    const bool plus = !fromStrat;
-    float acc = 0.0f;
-    for (int a = 0; a < na; ++a) {
-      const float x = src[(int64_t)a * n];
-      acc += plus ? (x > 0.0f ? x : 0.0f) : x;
-    }
-    const float uniform = 1.0f / (float)na;
-    const int32_t* kids = d.childFlat + d.childBase[node];
-    for (int a = 0; a < na; ++a) {
-      const float s = sigmaOf(src[(int64_t)a * n], acc, uniform, plus);
-      R[d.reachOff[TR][kids[a]] + c] = r * s;
-    }
+  const int lane = it.tile * kTile + (int)threadIdx.x;
+  if (lane >= it.n) return;
+  if (it.kind == kFwdChance) {
+    const int tp = d.toParent[1 - TR][it.a + lane];
+    R[it.c + lane] = R[it.b + tp] * it.v0;
     return;
   }
-  // chance branch: child reach = parent reach / (52 - board - 4), compacted
-  const int br = it.y >> 16;
-  const int j = (it.y & 0xffff) * kTile + (int)threadIdx.x;
-  const int child = d.childFlat[d.childBase[node] + br];
-  const int ccl = d.cls[child];
-  if (j >= d.clsN[OPP][ccl]) return;
-  const float inv = 1.0f / (float)(52 - d.clsBoardN[d.cls[node]] - 4);
-  const int tp = d.toParent[OPP][d.clsOff[OPP][ccl] + j];
-  R[d.reachOff[TR][child] + j] = R[d.reachOff[TR][node] + tp] * inv;
+  const int n = it.n, na = it.na;
+  const float r = R[it.b + lane];
+  const float* __restrict__ src = (fromStrat ? d.strat : d.regret) + it.a + lane;
+  const bool plus = !fromStrat;
+  float acc = 0.0f;
+  for (int a = 0; a < na; ++a) {
+    const float x = src[(int64_t)a * n];
+    acc += plus ? (x > 0.0f ? x : 0.0f) : x;
+  }
+  const float uniform = 1.0f / (float)na;
+  float* __restrict__ out = R + it.c + lane;
+  for (int a = 0; a < na; ++a) out[(int64_t)a * n] = r * sigmaOf(src[(int64_t)a * n], acc, uniform, plus);
 }
 
 // ---------------------------------------------------------------------------
-// Backward pieces.
+// Backward.
 // ---------------------------------------------------------------------------
 
-// Decide node, one lane per traverser combo. NA > 0: action count known at
+// Decide tile, one lane per traverser combo. NA > 0: action count fixed at
 // compile time (values stay in registers); NA == 0: generic loop.
-template <int TR, int MODE, int NA>
-__device__ __forceinline__ void decideTile(const Dev& d, int node, int tile) {
-  const int n = d.clsN[TR][d.cls[node]];
-  const int c = tile * kTile + (int)threadIdx.x;
+template <int MODE, int NA>
+__device__ __forceinline__ void decideTile(const Dev& d, const Item& it) {
+  const int c = it.tile * kTile + (int)threadIdx.x;
+  const int n = it.n;
   if (c >= n) return;
-  const int na = NA > 0 ? NA : d.na[node];
-  const int32_t* kids = d.childFlat + d.childBase[node];
-  float* __restrict__ out = d.cfv + d.cfvOff[node];
-  const float* __restrict__ cf = d.cfv;
-  if (d.owner[node] != TR) {  // opponent acts: values add up
+  const int na = NA > 0 ? NA : it.na;
+  const int64_t stride = it.m;
+  const float* __restrict__ kid = d.cfv + it.c + c;
+  float* __restrict__ out = d.cfv + it.b + c;
+  if (!it.e) {  // opponent acts: values add up
     float v = 0.0f;
-    for (int a = 0; a < na; ++a) v += cf[d.cfvOff[kids[a]] + c];
-    out[c] = v;
+    for (int a = 0; a < na; ++a) v += kid[a * stride];
+    *out = v;
     return;
   }
   if (MODE == PPSG_MODE_BR) {
     float best = -1e30f;
-    for (int a = 0; a < na; ++a) best = fmaxf(best, cf[d.cfvOff[kids[a]] + c]);
-    out[c] = best;
+    for (int a = 0; a < na; ++a) best = fmaxf(best, kid[a * stride]);
+    *out = best;
     return;
   }
-  const int64_t base = d.regOff[node] + c;
-  float* __restrict__ src = MODE == PPSG_MODE_CFR ? d.regret : d.strat;
   constexpr bool plus = MODE == PPSG_MODE_CFR;
+  float* __restrict__ src = (plus ? d.regret : d.strat) + it.a + c;
   const float uniform = 1.0f / (float)na;
   if (NA > 0) {
-    float x[NA > 0 ? NA : 1], v[NA > 0 ? NA : 1];
+    float x[NA > 0 ? NA : 1], v[NA > 0 ? NA : 1], s[NA > 0 ? NA : 1];
     float acc = 0.0f;
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
-      x[a] = src[base + (int64_t)a * n];
-      v[a] = cf[d.cfvOff[kids[a]] + c];
+      x[a] = src[(int64_t)a * n];
+      v[a] = kid[a * stride];
       acc += plus ? (x[a] > 0.0f ? x[a] : 0.0f) : x[a];
     }
-    float s[NA > 0 ? NA : 1];
     float val = 0.0f;
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
       s[a] = sigmaOf(x[a], acc, uniform, plus);
       val += s[a] * v[a];
     }
-    out[c] = val;
+    *out = val;
     if (MODE != PPSG_MODE_CFR) return;
     const float* co = d.ctl->coef + 3 * d.ctl->counter;
     const float pos = co[0], neg = co[1], avg = co[2];
+    float* __restrict__ st = d.strat + it.a + c;
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
-      const int64_t e = base + (int64_t)a * n;
-      d.strat[e] = d.strat[e] * avg + s[a];
-      d.regret[e] = x[a] * (x[a] >= 0.0f ? pos : neg) + (v[a] - val);
+      st[(int64_t)a * n] = st[(int64_t)a * n] * avg + s[a];
+      src[(int64_t)a * n] = x[a] * (x[a] >= 0.0f ? pos : neg) + (v[a] - val);
     }
     return;
   }
   float acc = 0.0f;
   for (int a = 0; a < na; ++a) {
-    const float x = src[base + (int64_t)a * n];
+    const float x = src[(int64_t)a * n];
     acc += plus ? (x > 0.0f ? x : 0.0f) : x;
   }
   float val = 0.0f;
   for (int a = 0; a < na; ++a)
-    val += sigmaOf(src[base + (int64_t)a * n], acc, uniform, plus) * cf[d.cfvOff[kids[a]] + c];
-  out[c] = val;
+    val += sigmaOf(src[(int64_t)a * n], acc, uniform, plus) * kid[a * stride];
+  *out = val;
   if (MODE != PPSG_MODE_CFR) return;
   const float* co = d.ctl->coef + 3 * d.ctl->counter;
   const float pos = co[0], neg = co[1], avg = co[2];
+  float* __restrict__ st = d.strat + it.a + c;
   for (int a = 0; a < na; ++a) {
-    const int64_t e = base + (int64_t)a * n;
-    const float x = d.regret[e];
-    const float s = sigmaOf(x, acc, uniform, plus);
-    d.strat[e] = d.strat[e] * avg + s;
-    d.regret[e] = x * (x >= 0.0f ? pos : neg) + (cf[d.cfvOff[kids[a]] + c] - val);
+    const float x = src[(int64_t)a * n];
+    const float sg = sigmaOf(x, acc, uniform, plus);
+    st[(int64_t)a * n] = st[(int64_t)a * n] * avg + sg;
+    src[(int64_t)a * n] = x * (x >= 0.0f ? pos : neg) + (kid[a * stride] - val);
   }
 }
 
-// Chance node, one lane per traverser (parent) combo: sum of the child
+// Chance tile, one lane per traverser (parent) combo: sum of the child
 // values over the branches the combo survives, in branch order.
 template <int TR>
-__device__ __forceinline__ void chanceTile(const Dev& d, int node, int tile) {
-  __shared__ int64_t brCfv[kMaxBranch];
-  __shared__ int64_t brMap[kMaxBranch];
-  const int nb = d.na[node];
-  const int32_t* kids = d.childFlat + d.childBase[node];
-  if ((int)threadIdx.x < nb) {
-    const int ch = kids[threadIdx.x];
-    brCfv[threadIdx.x] = d.cfvOff[ch];
-    brMap[threadIdx.x] = d.fromParentOff[TR][d.cls[ch]];
-  }
-  __syncthreads();
-  const int n = d.clsN[TR][d.cls[node]];
-  const int i = tile * kTile + (int)threadIdx.x;
+__device__ __forceinline__ void chanceTile(const Dev& d, const Item& it) {
+  const int i = it.tile * kTile + (int)threadIdx.x;
+  const int n = it.n;
   if (i >= n) return;
-  const int16_t* __restrict__ fp = d.fromParent[TR];
-  const float* __restrict__ cf = d.cfv;
+  const int16_t* __restrict__ map = d.branchMap[TR] + it.a + i;
+  const float* __restrict__ kid = d.cfv + it.c;
+  const int64_t stride = it.m;
   float v = 0.0f;
-  for (int b = 0; b < nb; ++b) {
-    const int slot = fp[brMap[b] + i];
-    if (slot >= 0) v += cf[brCfv[b] + slot];
+  for (int b = 0; b < it.na; ++b) {
+    const int slot = map[(int64_t)b * n];
+    if (slot >= 0) v += kid[b * stride + slot];
   }
-  d.cfv[d.cfvOff[node] + i] = v;
+  d.cfv[it.b + i] = v;
 }
 
-// Exclusive prefix sum over X[0..m) in place (f64, fixed order); X[m]
-// receives the total. All lanes of the workgroup must call it.
-__device__ void blockScan(double* X, int m, double* waveTot) {
+// Fold and showdown terminals. Opponent reach is staged in LDS, then each
+// lane gathers its PER consecutive entries of the class's gather list
+// (showdown: slots in strength order followed by the per-card segments in
+// strength order; fold: the per-card segments) and one blocked f64 scan
+// turns them into prefix sums X. Per traverser combo:
+//   fold:     mass = total - (its two cards' segment sums) + identical combo
+//   showdown: weaker / weaker-or-equal / all masses from X at le, te, n and
+//             at the compile-time cut points of its two cards' segments.
+// All global reads (reach, gather indices, per-combo metadata) are issued
+// before the first barrier.
+template <int TR, bool SD, int PER>
+__device__ void terminalNode(const Dev& d, const Item& it, char* lds) {
+  constexpr int OPP = 1 - TR;
+  __shared__ double waveTot[kBlock / 64];
+  __shared__ int so[53];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int per = (m + kBlock - 1) / kBlock;
-  const int lo = min(tid * per, m), hi = min(lo + per, m);
+  const int nO = it.m, nT = it.n;
+  float* raw = reinterpret_cast<float*>(lds);
+  double* X = reinterpret_cast<double*>(lds + (((nO + 1) * 4 + 15) & ~15));
+
+  // 1. issue every global read
+  const float* __restrict__ r = d.reach[TR] + it.a;
+  float rv[kRowRegs];
+#pragma unroll
+  for (int q = 0; q < kRowRegs; ++q) {
+    const int j = tid + q * kBlock;
+    rv[q] = j < nO ? r[j] : 0.0f;
+  }
+  uint4 gi[PER / 8];
+  const uint4* gsrc = reinterpret_cast<const uint4*>(d.gather[OPP] + it.c) + tid * (PER / 8);
+#pragma unroll
+  for (int k = 0; k < PER / 8; ++k) gi[k] = gsrc[k];
+  uint4 meta[kRowRegs];
+  uint32_t cards[kRowRegs];
+#pragma unroll
+  for (int q = 0; q < kRowRegs; ++q) {
+    const int i = tid + q * kBlock;
+    if (SD) {
+      meta[q] = i < nT ? reinterpret_cast<const uint4*>(d.sdMeta[TR] + 4 * it.d)[i]
+                       : make_uint4(0, 0, 0, 0);
+    } else {
+      cards[q] = i < nT ? ((uint32_t)d.listCards[TR][it.d + i] |
+                           ((uint32_t)(d.listSame[TR][it.d + i] + 1) << 16))
+                        : 0u;
+    }
+  }
+  if (tid < 53) so[tid] = d.segOff[OPP][it.f + tid];
+
+  // 2. stage reach (raw[nO] = 0 is the padding sentinel)
+#pragma unroll
+  for (int q = 0; q < kRowRegs; ++q) {
+    const int j = tid + q * kBlock;
+    if (j < nO) raw[j] = rv[q];
+  }
+  if (tid == 0) raw[nO] = 0.0f;
+  __syncthreads();
+
+  // 3. gather + blocked exclusive scan (f64)
+  double loc[PER];
   double own = 0.0;
-  for (int k = lo; k < hi; ++k) own += X[k];
+#pragma unroll
+  for (int k = 0; k < PER / 8; ++k) {
+    const uint32_t w[4] = {gi[k].x, gi[k].y, gi[k].z, gi[k].w};
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+      const int idx = (int)((w[h >> 1] >> ((h & 1) * 16)) & 0xffffu);
+      loc[k * 8 + h] = own;
+      own += (double)raw[idx];
+    }
+  }
   double inc = own;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
@@ -261,131 +322,96 @@ __device__ void blockScan(double* X, int m, double* waveTot) {
   }
   if (lane == 63) waveTot[wave] = inc;
   __syncthreads();
-  double run = inc - own;
-  for (int w = 0; w < wave; ++w) run += waveTot[w];
-  for (int k = lo; k < hi; ++k) {
-    const double t = X[k];
-    X[k] = run;
-    run += t;
-  }
+  double base = inc - own;
+  for (int w = 0; w < wave; ++w) base += waveTot[w];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) X[tid * PER + k] = base + loc[k];
+  if (tid == kBlock - 1) X[kBlock * PER] = base + own;
   __syncthreads();
-  if (tid == kBlock - 1) X[m] = run;
-  __syncthreads();
-}
 
-// Fold terminal: per-card opponent sums by a scan over the per-card
-// segments; mass = total - both cards' sums + the identical combo.
-template <int TR>
-__device__ void foldNode(const Dev& d, int node, char* lds) {
-  constexpr int OPP = 1 - TR;
-  __shared__ double waveTot[4];
-  const int cl = d.cls[node];
-  const int nO = d.clsN[OPP][cl], nT = d.clsN[TR][cl];
-  float* raw = reinterpret_cast<float*>(lds);
-  double* X = reinterpret_cast<double*>(lds + ((nO * 4 + 15) & ~15));
-  const float* __restrict__ r = d.reach[TR] + d.reachOff[TR][node];
-  for (int j = threadIdx.x; j < nO; j += kBlock) raw[j] = r[j];
-  __syncthreads();
-  const uint16_t* seg = d.segSlot[OPP] + 2 * d.clsOff[OPP][cl];
-  for (int q = threadIdx.x; q < 2 * nO; q += kBlock) X[q] = (double)raw[seg[q]];
-  __syncthreads();
-  blockScan(X, 2 * nO, waveTot);
-  const int32_t* so = d.segOff[OPP] + (int64_t)cl * 53;
-  const double tot = 0.5 * X[2 * nO];
-  const double v = (double)d.termVal[(int64_t)node * 8 + TR * 4];
-  const int64_t off = d.clsOff[TR][cl];
-  float* out = d.cfv + d.cfvOff[node];
-  for (int i = threadIdx.x; i < nT; i += kBlock) {
-    const int cc = d.listCards[TR][off + i];
-    const int c1 = cc & 0xff, c2 = cc >> 8;
-    const int same = d.listSame[TR][off + i];
-    double m = tot - (X[so[c1 + 1]] - X[so[c1]]) - (X[so[c2 + 1]] - X[so[c2]]);
-    if (same >= 0) m += (double)raw[same];
-    if (m < 0.0) m = 0.0;
-    out[i] = (float)(m * v);
-  }
-}
-
-// Showdown terminal. Opponent reach is gathered in strength order and
-// scanned (S), and the per-card segments — strength-ordered — are scanned
-// too (Y). For each traverser combo the compile-time bounds (le, te and
-// the segment cut points) turn the weaker / weaker-or-equal / all masses
-// into prefix differences with the shared-card entries removed.
-template <int TR>
-__device__ void showdownNode(const Dev& d, int node, char* lds) {
-  constexpr int OPP = 1 - TR;
-  __shared__ double waveTot[4];
-  __shared__ int so[53];
-  const int cl = d.cls[node];
-  const int nO = d.clsN[OPP][cl], nT = d.clsN[TR][cl];
-  float* raw = reinterpret_cast<float*>(lds);
-  double* X = reinterpret_cast<double*>(lds + ((nO * 4 + 15) & ~15));
-  const float* __restrict__ r = d.reach[TR] + d.reachOff[TR][node];
-  for (int j = threadIdx.x; j < nO; j += kBlock) raw[j] = r[j];
-  if (threadIdx.x < 53) so[threadIdx.x] = d.segOff[OPP][(int64_t)cl * 53 + threadIdx.x];
-  __syncthreads();
-  const int64_t offO = d.clsOff[OPP][cl];
-  const uint16_t* srt = d.sorted[OPP] + offO;
-  const uint16_t* seg = d.segSlot[OPP] + 2 * offO;
-  for (int k = threadIdx.x; k < nO; k += kBlock) X[k] = (double)raw[srt[k]];
-  for (int q = threadIdx.x; q < 2 * nO; q += kBlock) X[nO + q] = (double)raw[seg[q]];
-  __syncthreads();
-  blockScan(X, 3 * nO, waveTot);
-  // X[k] = sum of the k weakest; Y(q) = X[nO + q] (offset cancels in differences)
-  const double* Y = X + nO;
-  const float* tv = d.termVal + (int64_t)node * 8 + TR * 4;
-  const double vWin = tv[0], vTie = tv[1], vLose = tv[2];
-  const int64_t off = d.clsOff[TR][cl];
-  const uint4* meta = reinterpret_cast<const uint4*>(d.sdMeta[TR] + 4 * off);
-  float* out = d.cfv + d.cfvOff[node];
-  for (int i = threadIdx.x; i < nT; i += kBlock) {
-    const uint4 m = meta[i];
-    const int le = m.x & 0xffff, te = m.x >> 16;
-    const int a1 = m.y & 0xffff, b1 = m.y >> 16;
-    const int a2 = m.z & 0xffff, b2 = m.z >> 16;
-    const int c1 = m.w & 0xff, c2 = (m.w >> 8) & 0xff;
-    const int same = (int)(m.w >> 16) - 1;
-    const double z1 = Y[so[c1]], z2 = Y[so[c2]];
-    const double rs = same >= 0 ? (double)raw[same] : 0.0;
-    double win = X[le] - (Y[a1] - z1) - (Y[a2] - z2);
-    const double leq = X[te] - (Y[b1] - z1) - (Y[b2] - z2) + rs;
-    const double all = X[nO] - (Y[so[c1 + 1]] - z1) - (Y[so[c2 + 1]] - z2) + rs;
-    if (win < 0.0) win = 0.0;
-    double tie = leq - win;
-    if (tie < 0.0) tie = 0.0;
-    double lose = all - leq;
-    if (lose < 0.0) lose = 0.0;
-    out[i] = (float)(win * vWin + tie * vTie + lose * vLose);
+  // 4. per traverser combo
+  float* __restrict__ out = d.cfv + it.b;
+  if (SD) {
+    const double* Y = X + nO;  // per-card segment prefix (offset cancels)
+    const double vWin = it.v0, vTie = it.v1, vLose = it.v2;
+#pragma unroll
+    for (int q = 0; q < kRowRegs; ++q) {
+      const int i = tid + q * kBlock;
+      if (i >= nT) continue;
+      const uint4 m = meta[q];
+      const int le = m.x & 0xffff, te = m.x >> 16;
+      const int a1 = m.y & 0xffff, b1 = m.y >> 16;
+      const int a2 = m.z & 0xffff, b2 = m.z >> 16;
+      const int c1 = m.w & 0xff, c2 = (m.w >> 8) & 0xff;
+      const int same = (int)(m.w >> 16) - 1;
+      const double z1 = Y[so[c1]], z2 = Y[so[c2]];
+      const double rs = same >= 0 ? (double)raw[same] : 0.0;
+      double win = X[le] - (Y[a1] - z1) - (Y[a2] - z2);
+      const double leq = X[te] - (Y[b1] - z1) - (Y[b2] - z2) + rs;
+      const double all = X[nO] - (Y[so[c1 + 1]] - z1) - (Y[so[c2 + 1]] - z2) + rs;
+      if (win < 0.0) win = 0.0;
+      double tie = leq - win;
+      if (tie < 0.0) tie = 0.0;
+      double lose = all - leq;
+      if (lose < 0.0) lose = 0.0;
+      out[i] = (float)(win * vWin + tie * vTie + lose * vLose);
+    }
+  } else {
+    const double tot = 0.5 * X[2 * nO];
+    const double v = it.v0;
+#pragma unroll
+    for (int q = 0; q < kRowRegs; ++q) {
+      const int i = tid + q * kBlock;
+      if (i >= nT) continue;
+      const int c1 = cards[q] & 0xff, c2 = (cards[q] >> 8) & 0xff;
+      const int same = (int)(cards[q] >> 16) - 1;
+      double m = tot - (X[so[c1 + 1]] - X[so[c1]]) - (X[so[c2 + 1]] - X[so[c2]]);
+      if (same >= 0) m += (double)raw[same];
+      if (m < 0.0) m = 0.0;
+      out[i] = (float)(m * v);
+    }
   }
 }
 
 template <int TR, int MODE>
-__global__ __launch_bounds__(kBlock) void bwdKernel(Dev d, const int2* __restrict__ items) {
+__global__ __launch_bounds__(kBlock) void bwdKernel(Dev d, const Item* __restrict__ items,
+                                                    int kindMask) {
   extern __shared__ __align__(16) char lds[];
-  const int2 it = items[blockIdx.x];
-  const int node = it.x;
-  switch (d.kind[node]) {
-    case PPSG_DECIDE:
-      switch (d.na[node]) {
-        case 1: decideTile<TR, MODE, 1>(d, node, it.y); break;
-        case 2: decideTile<TR, MODE, 2>(d, node, it.y); break;
-        case 3: decideTile<TR, MODE, 3>(d, node, it.y); break;
-        case 4: decideTile<TR, MODE, 4>(d, node, it.y); break;
-        case 5: decideTile<TR, MODE, 5>(d, node, it.y); break;
-        case 6: decideTile<TR, MODE, 6>(d, node, it.y); break;
-        default: decideTile<TR, MODE, 0>(d, node, it.y); break;
+  const Item it = items[blockIdx.x];
+  // kindMask: profiling attribution only (-1 = every kind in real launches)
+  if (!((kindMask >> (it.kind - kBwdDecide)) & 1)) return;
+  switch (it.kind) {
+    case kBwdDecide:
+      switch (it.na) {
+        case 1: decideTile<MODE, 1>(d, it); break;
+        case 2: decideTile<MODE, 2>(d, it); break;
+        case 3: decideTile<MODE, 3>(d, it); break;
+        case 4: decideTile<MODE, 4>(d, it); break;
+        case 5: decideTile<MODE, 5>(d, it); break;
+        case 6: decideTile<MODE, 6>(d, it); break;
+        default: decideTile<MODE, 0>(d, it); break;
       }
       break;
-    case PPSG_CHANCE:
-      chanceTile<TR>(d, node, it.y);
+    case kBwdChance:
+      chanceTile<TR>(d, it);
       break;
-    case PPSG_FOLD:
-      foldNode<TR>(d, node, lds);
+    case kBwdFold:
+      if (it.e == 8) terminalNode<TR, false, 8>(d, it, lds);
+      else terminalNode<TR, false, 16>(d, it, lds);
       break;
     default:
-      showdownNode<TR>(d, node, lds);
+      if (it.e == 8) terminalNode<TR, true, 8>(d, it, lds);
+      else terminalNode<TR, true, 16>(d, it, lds);
       break;
   }
+}
+
+// Plain streaming copy (16 B per lane per step): the calibration kernel for
+// hardware byte counters.
+__global__ __launch_bounds__(kBlock) void copyProbe(const float4* __restrict__ a,
+                                                    float4* __restrict__ b, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
 }
 
 __global__ void advanceCounter(Ctl* ctl) {
@@ -407,14 +433,14 @@ struct ppsg_engine {
   int64_t regTotal = 0, cfvTotal = 0, reachTotal[2] = {0, 0};
   size_t terminalLds = 0;
   // per (tr, depth): device item list + count, and whether it has terminals
-  std::vector<int2*> fwdItems[2], bwdItems[2];
+  std::vector<Item*> fwdItems[2], bwdItems[2];
   std::vector<int> fwdCount[2], bwdCount[2];
   std::vector<char> bwdTerminal[2];
   Ctl* ctl = nullptr;
   float* coef = nullptr;
   int64_t coefCap = 0;
-  int64_t coefLen = 0;   // valid schedule entries
-  int64_t counter = 0;   // host mirror of the device schedule index
+  int64_t coefLen = 0;  // valid schedule entries
+  int64_t counter = 0;  // host mirror of the device schedule index
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -443,6 +469,8 @@ struct ppsg_engine {
     return rc;
   }
 
+  int launchFwd(int tr, int dpt, int fromStrat, hipStream_t s);
+  int launchBwd(int tr, int dpt, int mode, int kindMask, hipStream_t s);
   int launchHalf(int tr, int mode, hipStream_t s);
   int capture();
   ~ppsg_engine();
@@ -457,35 +485,41 @@ ppsg_engine::~ppsg_engine() {
   for (void* p : allocs) (void)hipFree(p);
 }
 
+int ppsg_engine::launchFwd(int tr, int dpt, int fromStrat, hipStream_t s) {
+  const int cnt = fwdCount[tr][dpt];
+  if (cnt == 0) return 0;
+  if (tr == 0) fwdKernel<0><<<cnt, kBlock, 0, s>>>(dev, fwdItems[0][dpt], fromStrat);
+  else fwdKernel<1><<<cnt, kBlock, 0, s>>>(dev, fwdItems[1][dpt], fromStrat);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int ppsg_engine::launchBwd(int tr, int dpt, int mode, int kindMask, hipStream_t s) {
+  const int cnt = bwdCount[tr][dpt];
+  if (cnt == 0) return 0;
+  const size_t lds = bwdTerminal[tr][dpt] ? terminalLds : 0;
+  const Item* it = bwdItems[tr][dpt];
+#define PPSG_BWD(T, M) bwdKernel<T, M><<<cnt, kBlock, lds, s>>>(dev, it, kindMask)
+  if (tr == 0) {
+    if (mode == PPSG_MODE_CFR) PPSG_BWD(0, PPSG_MODE_CFR);
+    else if (mode == PPSG_MODE_EV) PPSG_BWD(0, PPSG_MODE_EV);
+    else PPSG_BWD(0, PPSG_MODE_BR);
+  } else {
+    if (mode == PPSG_MODE_CFR) PPSG_BWD(1, PPSG_MODE_CFR);
+    else if (mode == PPSG_MODE_EV) PPSG_BWD(1, PPSG_MODE_EV);
+    else PPSG_BWD(1, PPSG_MODE_BR);
+  }
+#undef PPSG_BWD
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 int ppsg_engine::launchHalf(int tr, int mode, hipStream_t s) {
   const int fromStrat = mode == PPSG_MODE_CFR ? 0 : 1;
-  for (int dpt = 0; dpt <= maxDepth; ++dpt) {
-    const int cnt = fwdCount[tr][dpt];
-    if (cnt == 0) continue;
-    if (tr == 0)
-      fwdKernel<0><<<cnt, kBlock, 0, s>>>(dev, fwdItems[0][dpt], fromStrat);
-    else
-      fwdKernel<1><<<cnt, kBlock, 0, s>>>(dev, fwdItems[1][dpt], fromStrat);
-    HIP_TRY(hipGetLastError());
-  }
-  for (int dpt = maxDepth; dpt >= 0; --dpt) {
-    const int cnt = bwdCount[tr][dpt];
-    if (cnt == 0) continue;
-    const size_t lds = bwdTerminal[tr][dpt] ? terminalLds : 0;
-    const int2* it = bwdItems[tr][dpt];
-#define PPSG_BWD(T, M) bwdKernel<T, M><<<cnt, kBlock, lds, s>>>(dev, it)
-    if (tr == 0) {
-      if (mode == PPSG_MODE_CFR) PPSG_BWD(0, PPSG_MODE_CFR);
-      else if (mode == PPSG_MODE_EV) PPSG_BWD(0, PPSG_MODE_EV);
-      else PPSG_BWD(0, PPSG_MODE_BR);
-    } else {
-      if (mode == PPSG_MODE_CFR) PPSG_BWD(1, PPSG_MODE_CFR);
-      else if (mode == PPSG_MODE_EV) PPSG_BWD(1, PPSG_MODE_EV);
-      else PPSG_BWD(1, PPSG_MODE_BR);
-    }
-#undef PPSG_BWD
-    HIP_TRY(hipGetLastError());
-  }
+  for (int dpt = 0; dpt <= maxDepth; ++dpt)
+    if (launchFwd(tr, dpt, fromStrat, s) != 0) return -1;
+  for (int dpt = maxDepth; dpt >= 0; --dpt)
+    if (launchBwd(tr, dpt, mode, -1, s) != 0) return -1;
   return 0;
 }
 
@@ -507,9 +541,72 @@ int ppsg_engine::capture() {
   return 0;
 }
 
+namespace {
+
+// Gather entries per lane for a list of `len` entries: 8 or 16 (one or two
+// 16-byte index loads).
+int perLane(int len) { return len <= kBlock * 8 ? 8 : 16; }
+
+// Per-class gather lists (fold: per-card segments; showdown: strength order
+// followed by the per-card segments), padded to kBlock * per entries with the
+// sentinel slot n (a zero in LDS).
+struct GatherTables {
+  std::vector<uint16_t> data[2];
+  std::map<int, int64_t> foldOff[2], sdOff[2];
+  std::map<int, int> foldPer[2], sdPer[2];
+
+  void add(const ppsg_tree_desc* t, int p, int cl, bool sd) {
+    auto& offs = sd ? sdOff[p] : foldOff[p];
+    if (offs.count(cl)) return;
+    const int n = t->cls_n[p][cl];
+    const int64_t lo = t->cls_list_off[p][cl];
+    std::vector<uint16_t> list;
+    if (sd) list.insert(list.end(), t->sorted[p] + lo, t->sorted[p] + lo + n);
+    list.insert(list.end(), t->seg_slot[p] + 2 * lo, t->seg_slot[p] + 2 * lo + 2 * n);
+    const int per = perLane((int)list.size());
+    list.resize((size_t)kBlock * per, (uint16_t)n);
+    while (data[p].size() % 8) data[p].push_back(0);  // 16-byte aligned lanes
+    offs[cl] = (int64_t)data[p].size();
+    (sd ? sdPer[p] : foldPer[p])[cl] = per;
+    data[p].insert(data[p].end(), list.begin(), list.end());
+  }
+};
+
+}  // namespace
+
 extern "C" {
 
 const char* ppsg_last_error(void) { return g_err.c_str(); }
+
+int ppsg_copy_probe(int64_t bytes, int reps, double* ms_per_launch) {
+  if (bytes <= 0 || bytes % 16 != 0 || reps <= 0) return fail("copy probe: bad size");
+  void *a = nullptr, *b = nullptr;
+  HIP_TRY(hipMalloc(&a, (size_t)bytes));
+  if (hipMalloc(&b, (size_t)bytes) != hipSuccess) {
+    (void)hipFree(a);
+    return fail("copy probe: alloc failed");
+  }
+  hipEvent_t e0, e1;
+  int rc = 0;
+  if (hipMemset(a, 1, (size_t)bytes) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    rc = fail("copy probe: setup failed");
+  } else {
+    const int64_t n = bytes / 16;
+    (void)hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps; ++r) copyProbe<<<4096, kBlock>>>((const float4*)a, (float4*)b, n);
+    (void)hipEventRecord(e1, nullptr);
+    if (hipEventSynchronize(e1) != hipSuccess) rc = fail("copy probe: launch failed");
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms_per_launch) *ms_per_launch = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return rc;
+}
 
 int ppsg_create(const ppsg_tree_desc* t, ppsg_t** out) {
   if (!t || !out) return fail("null argument");
@@ -522,31 +619,49 @@ int ppsg_create(const ppsg_tree_desc* t, ppsg_t** out) {
   };
   Dev& d = E->dev;
   const int nc = t->n_class;
+  const int64_t N = t->n_nodes;
   int maxList = 0;
-  for (int p = 0; p < 2; ++p) {
+  for (int p = 0; p < 2; ++p)
     for (int c = 0; c < nc; ++c) maxList = std::max(maxList, t->cls_n[p][c]);
-    if (E->uploadC(d.clsN[p], t->cls_n[p], nc) || E->uploadC(d.clsOff[p], t->cls_list_off[p], nc) ||
-        E->uploadC(d.listCards[p], t->list_cards[p], t->list_total[p]) ||
+  if (maxList > kMaxList) return bail(fail("combo list longer than 1326"));
+  auto kidsOf = [&](int64_t u, int& b, int& e) {
+    b = t->child_base[u];
+    e = u + 1 < N ? t->child_base[u + 1] : (int)t->n_child;
+  };
+
+  // branch maps per parent class: [branch][parent slot] -> child slot or -1
+  std::vector<int16_t> bmap[2];
+  std::map<int, int64_t> bmapOff[2];
+  GatherTables G;
+  for (int64_t u = 0; u < N; ++u) {
+    const int cl = t->cls[u];
+    if (t->kind[u] == PPSG_CHANCE) {
+      int b, e;
+      kidsOf(u, b, e);
+      for (int p = 0; p < 2; ++p) {
+        if (bmapOff[p].count(cl)) continue;
+        bmapOff[p][cl] = (int64_t)bmap[p].size();
+        const int np = t->cls_n[p][cl];
+        for (int k = b; k < e; ++k) {
+          const int ccl = t->cls[t->child_flat[k]];
+          const int16_t* fp = t->from_parent[p] + t->from_parent_off[p][ccl];
+          bmap[p].insert(bmap[p].end(), fp, fp + np);
+        }
+      }
+    } else if (t->kind[u] == PPSG_FOLD || t->kind[u] == PPSG_SHOWDOWN) {
+      for (int p = 0; p < 2; ++p) G.add(t, p, cl, t->kind[u] == PPSG_SHOWDOWN);
+    }
+  }
+  for (int p = 0; p < 2; ++p) {
+    if (E->uploadC(d.listCards[p], t->list_cards[p], t->list_total[p]) ||
         E->uploadC(d.listSame[p], t->list_same[p], t->list_total[p]) ||
         E->uploadC(d.toParent[p], t->to_parent[p], t->list_total[p]) ||
-        E->uploadC(d.fromParentOff[p], t->from_parent_off[p], nc) ||
-        E->uploadC(d.fromParent[p], t->from_parent[p], std::max<int64_t>(t->from_parent_total[p], 1)) ||
+        E->uploadC(d.branchMap[p], bmap[p].data(), std::max<int64_t>((int64_t)bmap[p].size(), 1)) ||
         E->uploadC(d.segOff[p], t->seg_off[p], (int64_t)nc * 53) ||
-        E->uploadC(d.segSlot[p], t->seg_slot[p], 2 * t->list_total[p]) ||
-        E->uploadC(d.sorted[p], t->sorted[p], t->list_total[p]) ||
-        E->uploadC(d.sdMeta[p], t->sd_meta[p], 4 * t->list_total[p]) ||
-        E->uploadC(d.reachOff[p], t->reach_off[p], t->n_nodes))
+        E->uploadC(d.gather[p], G.data[p].data(), std::max<int64_t>((int64_t)G.data[p].size(), 8)) ||
+        E->uploadC(d.sdMeta[p], t->sd_meta[p], 4 * t->list_total[p]))
       return bail(-1);
   }
-  if (maxList > 65535) return bail(fail("combo list too long"));
-  const int64_t N = t->n_nodes;
-  if (E->uploadC(d.clsBoardN, t->cls_board_n, nc) || E->uploadC(d.kind, t->kind, N) ||
-      E->uploadC(d.owner, t->owner, N) || E->uploadC(d.na, t->na, N) ||
-      E->uploadC(d.cls, t->cls, N) || E->uploadC(d.childBase, t->child_base, N) ||
-      E->uploadC(d.childFlat, t->child_flat, std::max<int64_t>(t->n_child, 1)) ||
-      E->uploadC(d.regOff, t->reg_off, N) || E->uploadC(d.cfvOff, t->cfv_off, N) ||
-      E->uploadC(d.termVal, t->term_val, N * 8))
-    return bail(-1);
   E->n[0] = t->n_base[0];
   E->n[1] = t->n_base[1];
   E->maxDepth = t->max_depth;
@@ -565,8 +680,8 @@ int ppsg_create(const ppsg_tree_desc* t, ppsg_t** out) {
       hipMemset(d.strat, 0, sizeof(float) * std::max<int64_t>(t->reg_total, 1)) != hipSuccess)
     return bail(fail("row init failed"));
   for (int tr = 0; tr < 2; ++tr)
-    if (hipMemcpy(d.reach[tr], t->root_reach[tr], sizeof(float) * t->n_base[1 - tr],
-                  hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpy(d.reach[tr] + t->reach_off[tr][0], t->root_reach[tr],
+                  sizeof(float) * t->n_base[1 - tr], hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail("root reach upload failed"));
   {
     void* p = nullptr;
@@ -578,13 +693,12 @@ int ppsg_create(const ppsg_tree_desc* t, ppsg_t** out) {
     if (hipMemcpy(E->ctl, &zero, sizeof(Ctl), hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail("ctl init failed"));
   }
-  // LDS for one terminal workgroup: raw reach (f32) + 3n+1 f64 prefix slots
-  E->terminalLds = (size_t)((maxList * 4 + 15) & ~15) + (size_t)(3 * maxList + 1) * 8;
-  if (E->terminalLds > 160 * 1024) return bail(fail("terminal LDS too large"));
+  // LDS of one terminal workgroup: reach (+ sentinel) and the f64 prefix array
+  E->terminalLds = (size_t)(((maxList + 1) * 4 + 15) & ~15) + (size_t)(kBlock * 16 + 1) * 8;
 
-  // work lists
+  // work items
   const int D = t->max_depth + 1;
-  std::vector<std::vector<int2>> fwd[2], bwd[2];
+  std::vector<std::vector<Item>> fwd[2], bwd[2];
   for (int tr = 0; tr < 2; ++tr) {
     fwd[tr].assign(D, {});
     bwd[tr].assign(D, {});
@@ -592,27 +706,117 @@ int ppsg_create(const ppsg_tree_desc* t, ppsg_t** out) {
   }
   auto tiles = [](int n) { return (n + kTile - 1) / kTile; };
   for (int64_t u = 0; u < N; ++u) {
-    const int dep = t->depth[u], cl = t->cls[u];
-    const int k = t->kind[u];
+    const int dep = t->depth[u], cl = t->cls[u], k = t->kind[u];
+    int kb, ke;
+    kidsOf(u, kb, ke);
     for (int tr = 0; tr < 2; ++tr) {
       const int op = 1 - tr;
+      const int nT = t->cls_n[tr][cl], nO = t->cls_n[op][cl];
       if (k == PPSG_DECIDE) {
-        if (t->owner[u] == op)
-          for (int q = 0; q < tiles(t->cls_n[op][cl]); ++q) fwd[tr][dep].push_back({(int)u, q});
-        for (int q = 0; q < tiles(t->cls_n[tr][cl]); ++q) bwd[tr][dep].push_back({(int)u, q});
-      } else if (k == PPSG_CHANCE) {
-        const int nb = t->na[u];
-        if (nb > kMaxBranch) return bail(fail("too many chance branches"));
-        for (int b = 0; b < nb; ++b) {
-          const int ch = t->child_flat[t->child_base[u] + b];
-          for (int q = 0; q < tiles(t->cls_n[op][t->cls[ch]]); ++q)
-            fwd[tr][dep].push_back({(int)u, (b << 16) | q});
+        const int na = t->na[u];
+        const int first = t->child_flat[kb];
+        if (t->owner[u] == op) {
+          for (int q = 0; q < tiles(nO); ++q) {
+            Item it{};
+            it.kind = kFwdDecide;
+            it.a = t->reg_off[u];
+            it.b = t->reach_off[tr][u];
+            it.c = t->reach_off[tr][first];
+            it.n = nO;
+            it.na = (uint8_t)na;
+            it.tile = (uint16_t)q;
+            fwd[tr][dep].push_back(it);
+          }
         }
-        for (int q = 0; q < tiles(t->cls_n[tr][cl]); ++q) bwd[tr][dep].push_back({(int)u, q});
+        const int64_t stride = na > 1 ? t->cfv_off[t->child_flat[kb + 1]] - t->cfv_off[first] : 0;
+        for (int q = 0; q < tiles(nT); ++q) {
+          Item it{};
+          it.kind = kBwdDecide;
+          it.a = t->reg_off[u];
+          it.b = t->cfv_off[u];
+          it.c = t->cfv_off[first];
+          it.m = (int32_t)stride;
+          it.n = nT;
+          it.na = (uint8_t)na;
+          it.e = t->owner[u] == tr;
+          it.tile = (uint16_t)q;
+          bwd[tr][dep].push_back(it);
+        }
+      } else if (k == PPSG_CHANCE) {
+        const int nb = ke - kb;
+        const float inv = 1.0f / (float)(52 - t->cls_board_n[cl] - 4);
+        for (int b = kb; b < ke; ++b) {
+          const int ch = t->child_flat[b];
+          const int ccl = t->cls[ch];
+          for (int q = 0; q < tiles(t->cls_n[op][ccl]); ++q) {
+            Item it{};
+            it.kind = kFwdChance;
+            it.a = t->cls_list_off[op][ccl];
+            it.b = t->reach_off[tr][u];
+            it.c = t->reach_off[tr][ch];
+            it.n = t->cls_n[op][ccl];
+            it.v0 = inv;
+            it.tile = (uint16_t)q;
+            fwd[tr][dep].push_back(it);
+          }
+        }
+        const int first = t->child_flat[kb];
+        const int64_t stride = nb > 1 ? t->cfv_off[t->child_flat[kb + 1]] - t->cfv_off[first] : 0;
+        for (int q = 0; q < tiles(nT); ++q) {
+          Item it{};
+          it.kind = kBwdChance;
+          it.a = bmapOff[tr][cl];
+          it.b = t->cfv_off[u];
+          it.c = t->cfv_off[first];
+          it.m = (int32_t)stride;
+          it.n = nT;
+          it.na = (uint8_t)nb;
+          it.tile = (uint16_t)q;
+          bwd[tr][dep].push_back(it);
+        }
       } else {
-        bwd[tr][dep].push_back({(int)u, 0});
+        const bool sd = k == PPSG_SHOWDOWN;
+        Item it{};
+        it.kind = sd ? kBwdShowdown : kBwdFold;
+        it.a = t->reach_off[tr][u];
+        it.b = t->cfv_off[u];
+        it.c = sd ? G.sdOff[op][cl] : G.foldOff[op][cl];
+        it.d = t->cls_list_off[tr][cl];
+        it.e = sd ? G.sdPer[op][cl] : G.foldPer[op][cl];
+        it.f = cl * 53;
+        it.n = nT;
+        it.m = nO;
+        const float* tv = t->term_val + u * 8 + tr * 4;
+        it.v0 = tv[0];
+        it.v1 = tv[1];
+        it.v2 = tv[2];
+        bwd[tr][dep].push_back(it);
         E->bwdTerminal[tr][dep] = 1;
       }
+    }
+  }
+  // Algorithmic bytes of one iteration, split by pass: every f32 row element
+  // read or written once per use (index tables excluded).
+  for (int64_t u = 0; u < N; ++u) {
+    const int cl = t->cls[u];
+    const double n0 = t->cls_n[0][cl], n1 = t->cls_n[1][cl];
+    if (t->kind[u] == PPSG_DECIDE) {
+      const double a = t->na[u];
+      const double np = t->owner[u] == 0 ? n0 : n1, nq = t->owner[u] == 0 ? n1 : n0;
+      E->fwdBytes += 4 * np * (2 * a + 1);
+      E->bwdBytes += 4 * np * (5 * a + 1) + 4 * nq * (a + 1);
+    } else if (t->kind[u] == PPSG_CHANCE) {
+      int b, e;
+      kidsOf(u, b, e);
+      for (int k = b; k < e; ++k) {
+        const int ccl = t->cls[t->child_flat[k]];
+        const double nck = (double)t->cls_n[0][ccl] + t->cls_n[1][ccl];
+        E->fwdBytes += 8 * nck;
+        E->bwdBytes += 4 * nck;
+      }
+      E->bwdBytes += 4 * (n0 + n1);
+    } else {
+      E->bwdBytes += 8 * (n0 + n1);
     }
   }
   for (int tr = 0; tr < 2; ++tr) {
@@ -622,10 +826,8 @@ int ppsg_create(const ppsg_tree_desc* t, ppsg_t** out) {
     E->bwdCount[tr].assign(D, 0);
     for (int dep = 0; dep < D; ++dep) {
       // terminal workgroups first: they are the long ones
-      std::stable_sort(bwd[tr][dep].begin(), bwd[tr][dep].end(), [&](int2 a, int2 b) {
-        const bool ta = t->kind[a.x] == PPSG_FOLD || t->kind[a.x] == PPSG_SHOWDOWN;
-        const bool tb = t->kind[b.x] == PPSG_FOLD || t->kind[b.x] == PPSG_SHOWDOWN;
-        return ta && !tb;
+      std::stable_sort(bwd[tr][dep].begin(), bwd[tr][dep].end(), [](const Item& a, const Item& b) {
+        return a.kind >= kBwdFold && b.kind < kBwdFold;
       });
       E->fwdCount[tr][dep] = (int)fwd[tr][dep].size();
       E->bwdCount[tr][dep] = (int)bwd[tr][dep].size();
@@ -637,32 +839,9 @@ int ppsg_create(const ppsg_tree_desc* t, ppsg_t** out) {
     }
   }
   E->kernelsPerIter += 1;
-  // Algorithmic bytes of one iteration, split by pass: every f32 row
-  // element read or written once per use (index tables excluded).
-  for (int64_t u = 0; u < N; ++u) {
-    const int cl = t->cls[u];
-    const double n0 = t->cls_n[0][cl], n1 = t->cls_n[1][cl];
-    if (t->kind[u] == PPSG_DECIDE) {
-      const double a = t->na[u];
-      const double np = t->owner[u] == 0 ? n0 : n1, nq = t->owner[u] == 0 ? n1 : n0;
-      E->fwdBytes += 4 * np * (2 * a + 1);
-      E->bwdBytes += 4 * np * (5 * a + 1) + 4 * nq * (a + 1);
-    } else if (t->kind[u] == PPSG_CHANCE) {
-      for (int b = 0; b < t->na[u]; ++b) {
-        const int ccl = t->cls[t->child_flat[t->child_base[u] + b]];
-        const double nc = (double)t->cls_n[0][ccl] + t->cls_n[1][ccl];
-        E->fwdBytes += 8 * nc;
-        E->bwdBytes += 4 * nc;
-      }
-      E->bwdBytes += 4 * (n0 + n1);
-    } else {
-      E->bwdBytes += 8 * (n0 + n1);
-    }
-  }
   if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&E->ev0) != hipSuccess || hipEventCreate(&E->ev1) != hipSuccess)
     return bail(fail("stream/event creation failed"));
-  // terminal kernels may need more than the default dynamic LDS
   if (E->terminalLds > 64 * 1024) {
     const void* ks[] = {(const void*)bwdKernel<0, 0>, (const void*)bwdKernel<0, 1>,
                         (const void*)bwdKernel<0, 2>, (const void*)bwdKernel<1, 0>,
@@ -697,7 +876,7 @@ int ppsg_set_schedule(ppsg_t* h, const float* coef, int64_t n) {
   return 0;
 }
 
-// Every launch reads coef[counter]: refuse work that would run past the table.
+// Every CFR launch reads coef[counter]: refuse work that would run past the table.
 static int scheduleCovers(ppsg_t* h, int64_t iters) {
   if (!h->coef) return fail("no schedule set");
   if (h->counter < 0 || h->counter + iters > h->coefLen)
@@ -817,14 +996,13 @@ int ppsg_info(ppsg_t* h, int64_t* out, int cap) {
 int ppsg_profile(ppsg_t* h, int iters, double* out, int cap) {
   if (!h || !out) return fail("null argument");
   if (scheduleCovers(h, iters) != 0) return -1;
-  double acc[5] = {0, 0, 0, 0, 0};
+  double acc[13] = {0};
   hipEvent_t a = nullptr, b = nullptr;
   HIP_TRY(hipEventCreate(&a));
   HIP_TRY(hipEventCreate(&b));
   auto timed = [&](auto&& launch, int slot) -> int {
     HIP_TRY(hipEventRecord(a, h->stream));
-    launch();
-    HIP_TRY(hipGetLastError());
+    if (launch() != 0) return -1;
     HIP_TRY(hipEventRecord(b, h->stream));
     HIP_TRY(hipEventSynchronize(b));
     float ms = 0.0f;
@@ -836,24 +1014,16 @@ int ppsg_profile(ppsg_t* h, int iters, double* out, int cap) {
   int rc = 0;
   for (int it = 0; it < iters && rc == 0; ++it) {
     for (int tr = 0; tr < 2 && rc == 0; ++tr) {
-      for (int dpt = 0; dpt <= h->maxDepth && rc == 0; ++dpt) {
-        const int cnt = h->fwdCount[tr][dpt];
-        if (!cnt) continue;
-        const int2* it2 = h->fwdItems[tr][dpt];
-        rc = timed([&] {
-          if (tr == 0) fwdKernel<0><<<cnt, kBlock, 0, h->stream>>>(h->dev, it2, 0);
-          else fwdKernel<1><<<cnt, kBlock, 0, h->stream>>>(h->dev, it2, 0);
-        }, 0);
-      }
+      for (int dpt = 0; dpt <= h->maxDepth && rc == 0; ++dpt)
+        if (h->fwdCount[tr][dpt])
+          rc = timed([&] { return h->launchFwd(tr, dpt, 0, h->stream); }, 0);
       for (int dpt = h->maxDepth; dpt >= 0 && rc == 0; --dpt) {
-        const int cnt = h->bwdCount[tr][dpt];
-        if (!cnt) continue;
-        const size_t lds = h->bwdTerminal[tr][dpt] ? h->terminalLds : 0;
-        const int2* it2 = h->bwdItems[tr][dpt];
-        rc = timed([&] {
-          if (tr == 0) bwdKernel<0, PPSG_MODE_CFR><<<cnt, kBlock, lds, h->stream>>>(h->dev, it2);
-          else bwdKernel<1, PPSG_MODE_CFR><<<cnt, kBlock, lds, h->stream>>>(h->dev, it2);
-        }, 2);
+        if (!h->bwdCount[tr][dpt]) continue;
+        rc = timed([&] { return h->launchBwd(tr, dpt, PPSG_MODE_CFR, -1, h->stream); }, 2);
+        // attribution: the same launch restricted to one kind
+        for (int k = 0; k < 4 && rc == 0 && cap >= 13; ++k)
+          rc = timed([&] { return h->launchBwd(tr, dpt, PPSG_MODE_CFR, 1 << k, h->stream); },
+                     5 + 2 * k);
       }
     }
     if (rc == 0) advanceCounter<<<1, 64, 0, h->stream>>>(h->ctl);
@@ -864,7 +1034,7 @@ int ppsg_profile(ppsg_t* h, int iters, double* out, int cap) {
   HIP_TRY(hipStreamSynchronize(h->stream));
   h->counter += iters;
   acc[4] = iters;
-  for (int i = 0; i < cap && i < 5; ++i) out[i] = acc[i];
+  for (int i = 0; i < cap && i < 13; ++i) out[i] = acc[i];
   return 0;
 }
 

@@ -348,26 +348,45 @@ CompiledTree compileSpot(const PostflopSpot& spot, const pf::BetConfig& cfg) {
   B.kids.clear();
   classTables(T);
 
-  // rows (64-bit offsets) and terminal values
+  // rows (64-bit offsets) and terminal values. Sibling rows are contiguous:
+  // the children of a node sit at base + k * stride, so a kernel reaches
+  // every child row from one base offset.
   T.regOff.assign(T.numNodes, 0);
   T.cfvOff.assign(T.numNodes, 0);
   T.reachOff[0].assign(T.numNodes, 0);
   T.reachOff[1].assign(T.numNodes, 0);
   T.termVal.assign((size_t)T.numNodes * 8, 0.0f);
   T.decideOf.assign(T.numNodes, -1);
+  auto kidsOf = [&](int64_t u, int32_t& b, int32_t& e) {
+    b = T.childBase[u];
+    e = u + 1 < T.numNodes ? T.childBase[u + 1] : (int32_t)T.childFlat.size();
+  };
+  T.cfvTotal = std::max(T.listSize(0, 0), T.listSize(1, 0));
+  T.reachTotal[0] = T.listSize(1, 0);
+  T.reachTotal[1] = T.listSize(0, 0);
+  for (int64_t u = 0; u < T.numNodes; ++u) {
+    int32_t b, e;
+    kidsOf(u, b, e);
+    if (b == e) continue;
+    int stride = 0;
+    for (int32_t k = b; k < e; ++k)
+      stride = std::max(stride, std::max(T.listSize(0, T.childFlat[k]), T.listSize(1, T.childFlat[k])));
+    for (int32_t k = b; k < e; ++k) T.cfvOff[T.childFlat[k]] = T.cfvTotal + (int64_t)(k - b) * stride;
+    T.cfvTotal += (int64_t)(e - b) * stride;
+    for (int tr = 0; tr < 2; ++tr) {
+      if (T.kind[u] == PPSG_DECIDE && T.owner[u] == tr) {
+        for (int32_t k = b; k < e; ++k) T.reachOff[tr][T.childFlat[k]] = T.reachOff[tr][u];
+        continue;  // traverser acts: children share the row
+      }
+      int rs = 0;
+      for (int32_t k = b; k < e; ++k) rs = std::max(rs, T.listSize(1 - tr, T.childFlat[k]));
+      for (int32_t k = b; k < e; ++k)
+        T.reachOff[tr][T.childFlat[k]] = T.reachTotal[tr] + (int64_t)(k - b) * rs;
+      T.reachTotal[tr] += (int64_t)(e - b) * rs;
+    }
+  }
   for (int64_t u = 0; u < T.numNodes; ++u) {
     const int n0 = T.listSize(0, u), n1 = T.listSize(1, u);
-    T.cfvOff[u] = T.cfvTotal;
-    T.cfvTotal += std::max(n0, n1);
-    for (int tr = 0; tr < 2; ++tr) {
-      const int par = T.parent[u];
-      if (par >= 0 && T.kind[par] == PPSG_DECIDE && T.owner[par] == tr) {
-        T.reachOff[tr][u] = T.reachOff[tr][par];  // traverser acts: same row
-      } else {
-        T.reachOff[tr][u] = T.reachTotal[tr];
-        T.reachTotal[tr] += tr == 0 ? n1 : n0;
-      }
-    }
     if (T.kind[u] == PPSG_DECIDE) {
       T.decideOf[u] = (int32_t)T.decideNodes.size();
       T.decideNodes.push_back((int32_t)u);

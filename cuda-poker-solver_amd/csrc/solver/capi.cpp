@@ -325,10 +325,10 @@ double pps_solver_iteration_bytes(pps_solver* s) {
   return v;
 }
 
-int pps_solver_profile(pps_solver* s, int iters, double out5[5]) {
+int pps_solver_profile(pps_solver* s, int iters, int detail, double* out) {
   return guard([&] {
-    const std::vector<double> v = S(s).profileIterations(iters);
-    for (size_t i = 0; i < 5 && i < v.size(); ++i) out5[i] = v[i];
+    const std::vector<double> v = S(s).profileIterations(iters, detail != 0);
+    for (size_t i = 0; i < v.size(); ++i) out[i] = v[i];
     return 0;
   });
 }

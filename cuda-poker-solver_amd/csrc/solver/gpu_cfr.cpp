@@ -572,7 +572,7 @@ void GpuPostflopSolver::halfStep(int tr, int t, const std::string& algo, int pla
   check(ppsg_set_counter(I->eng, saved));
 }
 
-std::vector<double> GpuPostflopSolver::profileIterations(int iters) {
+std::vector<double> GpuPostflopSolver::profileIterations(int iters, bool detail) {
   Impl* I = impl_;
   if (I->T.empty || iters <= 0) return {};
   if (totalIters_ == 0 || I->lastAlgo == "hs30")
@@ -580,8 +580,8 @@ std::vector<double> GpuPostflopSolver::profileIterations(int iters) {
   const int64_t t0 = totalIters_, tEnd = t0 + iters;
   const std::vector<float> tab = scheduleTable(t0, tEnd, (int)tEnd, I->lastAlgo);
   check(ppsg_set_schedule(I->eng, tab.data(), tEnd));
-  std::vector<double> out(5, 0.0);
-  check(ppsg_profile(I->eng, iters, out.data(), 5));
+  std::vector<double> out(detail ? 13 : 5, 0.0);
+  check(ppsg_profile(I->eng, iters, out.data(), (int)out.size()));
   totalIters_ = tEnd;
   return out;
 }

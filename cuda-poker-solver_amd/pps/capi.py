@@ -42,7 +42,7 @@ CAPI_SYMBOLS = [
 PPSG_SYMBOLS = [
     "ppsg_create", "ppsg_destroy", "ppsg_set_schedule", "ppsg_set_counter", "ppsg_get_counter",
     "ppsg_reset", "ppsg_iterate", "ppsg_half_step", "ppsg_walk", "ppsg_read", "ppsg_write",
-    "ppsg_last_iterate_ms", "ppsg_info", "ppsg_profile", "ppsg_last_error",
+    "ppsg_last_iterate_ms", "ppsg_info", "ppsg_profile", "ppsg_copy_probe", "ppsg_last_error",
 ]
 
 _lib = None
@@ -95,12 +95,24 @@ def lib():
                                             P(ctypes.c_float), P(ctypes.c_double)]
         L.pps_solver_iteration_bytes.restype = ctypes.c_double
         L.pps_solver_iteration_bytes.argtypes = [ctypes.c_void_p]
-        L.pps_solver_profile.argtypes = [ctypes.c_void_p, ctypes.c_int, P(ctypes.c_double)]
+        L.pps_solver_profile.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                         P(ctypes.c_double)]
         L.pps_solver_engine_info.argtypes = [ctypes.c_void_p, P(ctypes.c_int64)]
         L.pps_solver_last_solve_ms.restype = ctypes.c_double
         L.pps_solver_last_solve_ms.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
+
+
+def copy_probe(nbytes, reps):
+    """Plain device copy of nbytes (read + write), reps launches; ms/launch."""
+    L = lib()
+    L.ppsg_copy_probe.argtypes = [ctypes.c_int64, ctypes.c_int, P(ctypes.c_double)]
+    L.ppsg_last_error.restype = ctypes.c_char_p
+    ms = ctypes.c_double()
+    if L.ppsg_copy_probe(int(nbytes), int(reps), ctypes.byref(ms)) != 0:
+        raise CapiError(L.ppsg_last_error().decode())
+    return ms.value
 
 
 class CapiError(RuntimeError):
@@ -280,11 +292,15 @@ class Solver:
     def iteration_bytes(self):
         return float(lib().pps_solver_iteration_bytes(self.h))
 
-    def profile(self, iters):
-        out = (ctypes.c_double * 5)()
-        _check(lib().pps_solver_profile(self.h, int(iters), out))
-        return {"fwd_ms": out[0], "fwd_launches": int(out[1]), "bwd_ms": out[2],
-                "bwd_launches": int(out[3]), "iters": int(out[4])}
+    def profile(self, iters, detail=False):
+        out = (ctypes.c_double * 13)()
+        _check(lib().pps_solver_profile(self.h, int(iters), int(detail), out))
+        d = {"fwd_ms": out[0], "fwd_launches": int(out[1]), "bwd_ms": out[2],
+             "bwd_launches": int(out[3]), "iters": int(out[4])}
+        if detail:
+            for k, name in enumerate(["decide", "showdown", "fold", "chance"]):
+                d[f"bwd_{name}_ms"] = out[5 + 2 * k]
+        return d
 
     def engine_info(self):
         out = (ctypes.c_int64 * 6)()

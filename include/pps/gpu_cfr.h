@@ -145,8 +145,9 @@ class GpuPostflopSolver {
   double rootValue(int tr, int mode, std::vector<float>* rootCfv);
   // Continues a DCFR-family solve for `iters` iterations launch by launch,
   // timing each launch: [forward ms, forward launches, backward ms,
-  // backward launches, iterations].
-  std::vector<double> profileIterations(int iters);
+  // backward launches, iterations]; detail: also per-kind backward
+  // attribution (13 values; perturbs the rows — measurement only).
+  std::vector<double> profileIterations(int iters, bool detail = false);
   // Engine facts (include/ppsg.h ppsg_info).
   std::vector<int64_t> engineInfo() const;
 

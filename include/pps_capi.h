@@ -100,8 +100,9 @@ int pps_solver_half_step(pps_solver* s, int tr, int t, const char* algo, int pla
 /* root value walk: mode 1 EV, 2 BR; fills the root cfv row; returns value */
 int pps_solver_root_value(pps_solver* s, int tr, int mode, float* root_cfv, double* value);
 double pps_solver_iteration_bytes(pps_solver* s);
-/* per-launch event timing of `iters` more iterations (profileIterations) */
-int pps_solver_profile(pps_solver* s, int iters, double out5[5]);
+/* per-launch event timing of `iters` more iterations (profileIterations);
+   detail != 0: 13 values incl. per-kind backward attribution */
+int pps_solver_profile(pps_solver* s, int iters, int detail, double* out);
 /* ppsg_info of the solver's engine: 6 values */
 int pps_solver_engine_info(pps_solver* s, int64_t out6[6]);
 double pps_solver_last_solve_ms(pps_solver* s);

@@ -109,9 +109,16 @@ int ppsg_info(ppsg_t* h, int64_t* out, int cap);
 /* Runs `iters` CFR iterations launch by launch (no graph), timing every
    launch with events on the engine stream: out = [forward ms, forward
    launches, backward ms, backward launches, iterations]. Advances the
-   counter like ppsg_iterate. (No reference counterpart; the reference's
+   counter like ppsg_iterate. With cap >= 13 every backward launch is also
+   re-run restricted to one node kind (decide, showdown, fold, chance) and
+   out[5 + 2k], out[6 + 2k] get that kind's ms and launches; those re-runs
+   apply the decide updates again, so use it for attribution only. (No reference counterpart; the reference's
    GPU_CFR_PROFILE mode plays this role, cuda/gpu_cfr.cu:1895-1987.) */
 int ppsg_profile(ppsg_t* h, int iters, double* out, int cap);
+/* Calibration for hardware byte counters (no reference counterpart): a
+   plain device copy of `bytes` (multiple of 16), `reps` launches on the
+   null stream; mean launch time out. */
+int ppsg_copy_probe(int64_t bytes, int reps, double* ms_per_launch);
 const char* ppsg_last_error(void);
 
 #ifdef __cplusplus
