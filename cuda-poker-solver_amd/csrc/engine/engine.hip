@@ -123,9 +123,7 @@ __device__ __forceinline__ float sigmaOf(float x, float acc, float uniform, bool
 // ---------------------------------------------------------------------------
 
 template <int TR>
-__global__ __launch_bounds__(kBlock) void fwdKernel(Dev d, const Item* __restrict__ items,
-                                                    int fromStrat) {
-  const Item it = items[blockIdx.x];
+__device__ __forceinline__ void fwdItem(const Dev& d, const Item& it, int fromStrat) {
   float* __restrict__ R = d.reach[TR];
   const int lane = it.tile * kTile + (int)threadIdx.x;
   if (lane >= it.n) return;
@@ -236,10 +234,21 @@ __device__ __forceinline__ void chanceTile(const Dev& d, const Item& it) {
   const int16_t* __restrict__ map = d.branchMap[TR] + it.a + i;
   const float* __restrict__ kid = d.cfv + it.c;
   const int64_t stride = it.m;
+  const int nb = it.na;
+  // Branches in chunks: all map loads of a chunk, then all value loads, then
+  // the adds in branch order (keeps many loads in flight; same sum order).
+  constexpr int kChunk = 16;
   float v = 0.0f;
-  for (int b = 0; b < it.na; ++b) {
-    const int slot = map[(int64_t)b * n];
-    if (slot >= 0) v += kid[b * stride + slot];
+  for (int b0 = 0; b0 < nb; b0 += kChunk) {
+    int slot[kChunk];
+    float x[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) slot[k] = b0 + k < nb ? map[(int64_t)(b0 + k) * n] : -1;
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) x[k] = slot[k] >= 0 ? kid[(b0 + k) * stride + slot[k]] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k)
+      if (slot[k] >= 0) v += x[k];
   }
   d.cfv[it.b + i] = v;
 }
@@ -373,13 +382,35 @@ __device__ void terminalNode(const Dev& d, const Item& it, char* lds) {
   }
 }
 
-template <int TR, int MODE>
-__global__ __launch_bounds__(kBlock) void bwdKernel(Dev d, const Item* __restrict__ items,
-                                                    int kindMask) {
+// Forward launch of one depth: opponent-reach items for the next depth plus
+// the terminal nodes of this depth (their values need only this depth's
+// reach, so they run here and the backward pass only combines).
+template <int TR>
+__global__ __launch_bounds__(kBlock) void fwdKernel(Dev d, const Item* __restrict__ items,
+                                                    int fromStrat, int kindMask) {
   extern __shared__ __align__(16) char lds[];
   const Item it = items[blockIdx.x];
   // kindMask: profiling attribution only (-1 = every kind in real launches)
-  if (!((kindMask >> (it.kind - kBwdDecide)) & 1)) return;
+  if (!((kindMask >> it.kind) & 1)) return;
+  switch (it.kind) {
+    case kBwdFold:
+      if (it.e == 8) terminalNode<TR, false, 8>(d, it, lds);
+      else terminalNode<TR, false, 16>(d, it, lds);
+      break;
+    case kBwdShowdown:
+      if (it.e == 8) terminalNode<TR, true, 8>(d, it, lds);
+      else terminalNode<TR, true, 16>(d, it, lds);
+      break;
+    default:
+      fwdItem<TR>(d, it, fromStrat);
+  }
+}
+
+template <int TR, int MODE>
+__global__ __launch_bounds__(kBlock) void bwdKernel(Dev d, const Item* __restrict__ items,
+                                                    int kindMask) {
+  const Item it = items[blockIdx.x];
+  if (!((kindMask >> it.kind) & 1)) return;
   switch (it.kind) {
     case kBwdDecide:
       switch (it.na) {
@@ -392,16 +423,8 @@ __global__ __launch_bounds__(kBlock) void bwdKernel(Dev d, const Item* __restric
         default: decideTile<MODE, 0>(d, it); break;
       }
       break;
-    case kBwdChance:
-      chanceTile<TR>(d, it);
-      break;
-    case kBwdFold:
-      if (it.e == 8) terminalNode<TR, false, 8>(d, it, lds);
-      else terminalNode<TR, false, 16>(d, it, lds);
-      break;
     default:
-      if (it.e == 8) terminalNode<TR, true, 8>(d, it, lds);
-      else terminalNode<TR, true, 16>(d, it, lds);
+      chanceTile<TR>(d, it);
       break;
   }
 }
@@ -432,10 +455,11 @@ struct ppsg_engine {
   int maxDepth = 0;
   int64_t regTotal = 0, cfvTotal = 0, reachTotal[2] = {0, 0};
   size_t terminalLds = 0;
-  // per (tr, depth): device item list + count, and whether it has terminals
+  // per (tr, depth): device item lists + counts, and whether the forward
+  // launch carries terminal nodes (it then needs dynamic LDS)
   std::vector<Item*> fwdItems[2], bwdItems[2];
   std::vector<int> fwdCount[2], bwdCount[2];
-  std::vector<char> bwdTerminal[2];
+  std::vector<char> fwdTerminal[2];
   Ctl* ctl = nullptr;
   float* coef = nullptr;
   int64_t coefCap = 0;
@@ -469,7 +493,7 @@ struct ppsg_engine {
     return rc;
   }
 
-  int launchFwd(int tr, int dpt, int fromStrat, hipStream_t s);
+  int launchFwd(int tr, int dpt, int fromStrat, int kindMask, hipStream_t s);
   int launchBwd(int tr, int dpt, int mode, int kindMask, hipStream_t s);
   int launchHalf(int tr, int mode, hipStream_t s);
   int capture();
@@ -485,11 +509,12 @@ ppsg_engine::~ppsg_engine() {
   for (void* p : allocs) (void)hipFree(p);
 }
 
-int ppsg_engine::launchFwd(int tr, int dpt, int fromStrat, hipStream_t s) {
+int ppsg_engine::launchFwd(int tr, int dpt, int fromStrat, int kindMask, hipStream_t s) {
   const int cnt = fwdCount[tr][dpt];
   if (cnt == 0) return 0;
-  if (tr == 0) fwdKernel<0><<<cnt, kBlock, 0, s>>>(dev, fwdItems[0][dpt], fromStrat);
-  else fwdKernel<1><<<cnt, kBlock, 0, s>>>(dev, fwdItems[1][dpt], fromStrat);
+  const size_t lds = fwdTerminal[tr][dpt] ? terminalLds : 0;
+  if (tr == 0) fwdKernel<0><<<cnt, kBlock, lds, s>>>(dev, fwdItems[0][dpt], fromStrat, kindMask);
+  else fwdKernel<1><<<cnt, kBlock, lds, s>>>(dev, fwdItems[1][dpt], fromStrat, kindMask);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -497,9 +522,8 @@ int ppsg_engine::launchFwd(int tr, int dpt, int fromStrat, hipStream_t s) {
 int ppsg_engine::launchBwd(int tr, int dpt, int mode, int kindMask, hipStream_t s) {
   const int cnt = bwdCount[tr][dpt];
   if (cnt == 0) return 0;
-  const size_t lds = bwdTerminal[tr][dpt] ? terminalLds : 0;
   const Item* it = bwdItems[tr][dpt];
-#define PPSG_BWD(T, M) bwdKernel<T, M><<<cnt, kBlock, lds, s>>>(dev, it, kindMask)
+#define PPSG_BWD(T, M) bwdKernel<T, M><<<cnt, kBlock, 0, s>>>(dev, it, kindMask)
   if (tr == 0) {
     if (mode == PPSG_MODE_CFR) PPSG_BWD(0, PPSG_MODE_CFR);
     else if (mode == PPSG_MODE_EV) PPSG_BWD(0, PPSG_MODE_EV);
@@ -517,7 +541,7 @@ int ppsg_engine::launchBwd(int tr, int dpt, int mode, int kindMask, hipStream_t 
 int ppsg_engine::launchHalf(int tr, int mode, hipStream_t s) {
   const int fromStrat = mode == PPSG_MODE_CFR ? 0 : 1;
   for (int dpt = 0; dpt <= maxDepth; ++dpt)
-    if (launchFwd(tr, dpt, fromStrat, s) != 0) return -1;
+    if (launchFwd(tr, dpt, fromStrat, -1, s) != 0) return -1;
   for (int dpt = maxDepth; dpt >= 0; --dpt)
     if (launchBwd(tr, dpt, mode, -1, s) != 0) return -1;
   return 0;
@@ -693,8 +717,6 @@ int ppsg_create(const ppsg_tree_desc* t, ppsg_t** out) {
     if (hipMemcpy(E->ctl, &zero, sizeof(Ctl), hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail("ctl init failed"));
   }
-  // LDS of one terminal workgroup: reach (+ sentinel) and the f64 prefix array
-  E->terminalLds = (size_t)(((maxList + 1) * 4 + 15) & ~15) + (size_t)(kBlock * 16 + 1) * 8;
 
   // work items
   const int D = t->max_depth + 1;
@@ -702,9 +724,10 @@ int ppsg_create(const ppsg_tree_desc* t, ppsg_t** out) {
   for (int tr = 0; tr < 2; ++tr) {
     fwd[tr].assign(D, {});
     bwd[tr].assign(D, {});
-    E->bwdTerminal[tr].assign(D, 0);
+    E->fwdTerminal[tr].assign(D, 0);
   }
   auto tiles = [](int n) { return (n + kTile - 1) / kTile; };
+  int maxPer = 8;
   for (int64_t u = 0; u < N; ++u) {
     const int dep = t->depth[u], cl = t->cls[u], k = t->kind[u];
     int kb, ke;
@@ -790,8 +813,9 @@ int ppsg_create(const ppsg_tree_desc* t, ppsg_t** out) {
         it.v0 = tv[0];
         it.v1 = tv[1];
         it.v2 = tv[2];
-        bwd[tr][dep].push_back(it);
-        E->bwdTerminal[tr][dep] = 1;
+        fwd[tr][dep].push_back(it);
+        E->fwdTerminal[tr][dep] = 1;
+        maxPer = std::max(maxPer, it.e);
       }
     }
   }
@@ -826,7 +850,7 @@ int ppsg_create(const ppsg_tree_desc* t, ppsg_t** out) {
     E->bwdCount[tr].assign(D, 0);
     for (int dep = 0; dep < D; ++dep) {
       // terminal workgroups first: they are the long ones
-      std::stable_sort(bwd[tr][dep].begin(), bwd[tr][dep].end(), [](const Item& a, const Item& b) {
+      std::stable_sort(fwd[tr][dep].begin(), fwd[tr][dep].end(), [](const Item& a, const Item& b) {
         return a.kind >= kBwdFold && b.kind < kBwdFold;
       });
       E->fwdCount[tr][dep] = (int)fwd[tr][dep].size();
@@ -839,13 +863,13 @@ int ppsg_create(const ppsg_tree_desc* t, ppsg_t** out) {
     }
   }
   E->kernelsPerIter += 1;
+  // LDS of one terminal workgroup: reach (+ sentinel) and the f64 prefix array
+  E->terminalLds = (size_t)(((maxList + 1) * 4 + 15) & ~15) + (size_t)(kBlock * maxPer + 1) * 8;
   if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&E->ev0) != hipSuccess || hipEventCreate(&E->ev1) != hipSuccess)
     return bail(fail("stream/event creation failed"));
   if (E->terminalLds > 64 * 1024) {
-    const void* ks[] = {(const void*)bwdKernel<0, 0>, (const void*)bwdKernel<0, 1>,
-                        (const void*)bwdKernel<0, 2>, (const void*)bwdKernel<1, 0>,
-                        (const void*)bwdKernel<1, 1>, (const void*)bwdKernel<1, 2>};
+    const void* ks[] = {(const void*)fwdKernel<0>, (const void*)fwdKernel<1>};
     for (const void* k : ks)
       if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)E->terminalLds) != hipSuccess)
@@ -1014,15 +1038,21 @@ int ppsg_profile(ppsg_t* h, int iters, double* out, int cap) {
   int rc = 0;
   for (int it = 0; it < iters && rc == 0; ++it) {
     for (int tr = 0; tr < 2 && rc == 0; ++tr) {
-      for (int dpt = 0; dpt <= h->maxDepth && rc == 0; ++dpt)
-        if (h->fwdCount[tr][dpt])
-          rc = timed([&] { return h->launchFwd(tr, dpt, 0, h->stream); }, 0);
+      for (int dpt = 0; dpt <= h->maxDepth && rc == 0; ++dpt) {
+        if (!h->fwdCount[tr][dpt]) continue;
+        rc = timed([&] { return h->launchFwd(tr, dpt, 0, -1, h->stream); }, 0);
+        // attribution: terminal kinds re-run alone (they only write cfv rows)
+        for (int k = 0; k < 2 && rc == 0 && cap >= 13; ++k)
+          rc = timed([&] { return h->launchFwd(tr, dpt, 0, 1 << (kBwdFold + k), h->stream); },
+                     9 + 2 * k);
+      }
       for (int dpt = h->maxDepth; dpt >= 0 && rc == 0; --dpt) {
         if (!h->bwdCount[tr][dpt]) continue;
         rc = timed([&] { return h->launchBwd(tr, dpt, PPSG_MODE_CFR, -1, h->stream); }, 2);
-        // attribution: the same launch restricted to one kind
-        for (int k = 0; k < 4 && rc == 0 && cap >= 13; ++k)
-          rc = timed([&] { return h->launchBwd(tr, dpt, PPSG_MODE_CFR, 1 << k, h->stream); },
+        // attribution: decide and chance tiles re-run alone
+        for (int k = 0; k < 2 && rc == 0 && cap >= 13; ++k)
+          rc = timed([&] { return h->launchBwd(tr, dpt, PPSG_MODE_CFR, 1 << (kBwdDecide + k),
+                                               h->stream); },
                      5 + 2 * k);
       }
     }

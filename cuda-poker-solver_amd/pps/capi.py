@@ -298,8 +298,8 @@ class Solver:
         d = {"fwd_ms": out[0], "fwd_launches": int(out[1]), "bwd_ms": out[2],
              "bwd_launches": int(out[3]), "iters": int(out[4])}
         if detail:
-            for k, name in enumerate(["decide", "showdown", "fold", "chance"]):
-                d[f"bwd_{name}_ms"] = out[5 + 2 * k]
+            for k, name in enumerate(["decide", "chance", "fold", "showdown"]):
+                d[f"{name}_only_ms"] = out[5 + 2 * k]
         return d
 
     def engine_info(self):

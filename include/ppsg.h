@@ -109,10 +109,11 @@ int ppsg_info(ppsg_t* h, int64_t* out, int cap);
 /* Runs `iters` CFR iterations launch by launch (no graph), timing every
    launch with events on the engine stream: out = [forward ms, forward
    launches, backward ms, backward launches, iterations]. Advances the
-   counter like ppsg_iterate. With cap >= 13 every backward launch is also
-   re-run restricted to one node kind (decide, showdown, fold, chance) and
-   out[5 + 2k], out[6 + 2k] get that kind's ms and launches; those re-runs
-   apply the decide updates again, so use it for attribution only. (No reference counterpart; the reference's
+   counter like ppsg_iterate. Fold and showdown values are computed in the
+   forward launch of their depth. With cap >= 13 every launch is also re-run
+   restricted to one node kind: out[5..6] backward decide, [7..8] backward
+   chance, [9..10] fold, [11..12] showdown (ms, launches); the decide re-runs
+   apply the updates again, so use it for attribution only. (No reference counterpart; the reference's
    GPU_CFR_PROFILE mode plays this role, cuda/gpu_cfr.cu:1895-1987.) */
 int ppsg_profile(ppsg_t* h, int iters, double* out, int cap);
 /* Calibration for hardware byte counters (no reference counterpart): a
