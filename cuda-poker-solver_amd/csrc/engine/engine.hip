@@ -88,7 +88,8 @@ enum ItemKind : uint8_t {
 //   kBwdShowdown the class (opponent), d traverser list offset, e gather
 //                entries per lane (8 or 16), f per-card segment table
 //                index (class * 53), n traverser combos, m opponent combos,
-//                v0..v2 payoffs (fold: v0)
+//                v0..v2 payoffs (fold: v0), na = row registers per lane
+//                (3 when both lists fit 768 combos, else 6)
 struct alignas(16) Item {
   int64_t a, b, c, d;
   int32_t n, m, e, f;
@@ -263,7 +264,7 @@ __device__ __forceinline__ void chanceTile(const Dev& d, const Item& it) {
 //             at the compile-time cut points of its two cards' segments.
 // All global reads (reach, gather indices, per-combo metadata) are issued
 // before the first barrier.
-template <int TR, bool SD, int PER>
+template <int TR, bool SD, int PER, int ROWS>
 __device__ void terminalNode(const Dev& d, const Item& it, char* lds) {
   constexpr int OPP = 1 - TR;
   __shared__ double waveTot[kBlock / 64];
@@ -275,9 +276,9 @@ __device__ void terminalNode(const Dev& d, const Item& it, char* lds) {
 
   // 1. issue every global read
   const float* __restrict__ r = d.reach[TR] + it.a;
-  float rv[kRowRegs];
+  float rv[ROWS];
 #pragma unroll
-  for (int q = 0; q < kRowRegs; ++q) {
+  for (int q = 0; q < ROWS; ++q) {
     const int j = tid + q * kBlock;
     rv[q] = j < nO ? r[j] : 0.0f;
   }
@@ -285,10 +286,10 @@ __device__ void terminalNode(const Dev& d, const Item& it, char* lds) {
   const uint4* gsrc = reinterpret_cast<const uint4*>(d.gather[OPP] + it.c) + tid * (PER / 8);
 #pragma unroll
   for (int k = 0; k < PER / 8; ++k) gi[k] = gsrc[k];
-  uint4 meta[kRowRegs];
-  uint32_t cards[kRowRegs];
+  uint4 meta[ROWS];
+  uint32_t cards[ROWS];
 #pragma unroll
-  for (int q = 0; q < kRowRegs; ++q) {
+  for (int q = 0; q < ROWS; ++q) {
     const int i = tid + q * kBlock;
     if (SD) {
       meta[q] = i < nT ? reinterpret_cast<const uint4*>(d.sdMeta[TR] + 4 * it.d)[i]
@@ -303,25 +304,22 @@ __device__ void terminalNode(const Dev& d, const Item& it, char* lds) {
 
   // 2. stage reach (raw[nO] = 0 is the padding sentinel)
 #pragma unroll
-  for (int q = 0; q < kRowRegs; ++q) {
+  for (int q = 0; q < ROWS; ++q) {
     const int j = tid + q * kBlock;
     if (j < nO) raw[j] = rv[q];
   }
   if (tid == 0) raw[nO] = 0.0f;
   __syncthreads();
 
-  // 3. gather + blocked exclusive scan (f64)
-  double loc[PER];
+  // 3. gather + blocked exclusive scan (f64); the lane's entries are summed
+  // once for the scan and re-read from LDS when the prefixes are written
+  // (keeps the lane's running values out of registers)
   double own = 0.0;
 #pragma unroll
   for (int k = 0; k < PER / 8; ++k) {
     const uint32_t w[4] = {gi[k].x, gi[k].y, gi[k].z, gi[k].w};
 #pragma unroll
-    for (int h = 0; h < 8; ++h) {
-      const int idx = (int)((w[h >> 1] >> ((h & 1) * 16)) & 0xffffu);
-      loc[k * 8 + h] = own;
-      own += (double)raw[idx];
-    }
+    for (int h = 0; h < 8; ++h) own += (double)raw[(w[h >> 1] >> ((h & 1) * 16)) & 0xffffu];
   }
   double inc = own;
 #pragma unroll
@@ -333,9 +331,19 @@ __device__ void terminalNode(const Dev& d, const Item& it, char* lds) {
   __syncthreads();
   double base = inc - own;
   for (int w = 0; w < wave; ++w) base += waveTot[w];
+  {
+    double run = base;
 #pragma unroll
-  for (int k = 0; k < PER; ++k) X[tid * PER + k] = base + loc[k];
-  if (tid == kBlock - 1) X[kBlock * PER] = base + own;
+    for (int k = 0; k < PER / 8; ++k) {
+      const uint32_t w[4] = {gi[k].x, gi[k].y, gi[k].z, gi[k].w};
+#pragma unroll
+      for (int h = 0; h < 8; ++h) {
+        X[tid * PER + k * 8 + h] = run;
+        run += (double)raw[(w[h >> 1] >> ((h & 1) * 16)) & 0xffffu];
+      }
+    }
+    if (tid == kBlock - 1) X[kBlock * PER] = run;
+  }
   __syncthreads();
 
   // 4. per traverser combo
@@ -344,7 +352,7 @@ __device__ void terminalNode(const Dev& d, const Item& it, char* lds) {
     const double* Y = X + nO;  // per-card segment prefix (offset cancels)
     const double vWin = it.v0, vTie = it.v1, vLose = it.v2;
 #pragma unroll
-    for (int q = 0; q < kRowRegs; ++q) {
+    for (int q = 0; q < ROWS; ++q) {
       const int i = tid + q * kBlock;
       if (i >= nT) continue;
       const uint4 m = meta[q];
@@ -369,7 +377,7 @@ __device__ void terminalNode(const Dev& d, const Item& it, char* lds) {
     const double tot = 0.5 * X[2 * nO];
     const double v = it.v0;
 #pragma unroll
-    for (int q = 0; q < kRowRegs; ++q) {
+    for (int q = 0; q < ROWS; ++q) {
       const int i = tid + q * kBlock;
       if (i >= nT) continue;
       const int c1 = cards[q] & 0xff, c2 = (cards[q] >> 8) & 0xff;
@@ -394,12 +402,22 @@ __global__ __launch_bounds__(kBlock) void fwdKernel(Dev d, const Item* __restric
   if (!((kindMask >> it.kind) & 1)) return;
   switch (it.kind) {
     case kBwdFold:
-      if (it.e == 8) terminalNode<TR, false, 8>(d, it, lds);
-      else terminalNode<TR, false, 16>(d, it, lds);
+      if (it.na == 3) {
+        if (it.e == 8) terminalNode<TR, false, 8, 3>(d, it, lds);
+        else terminalNode<TR, false, 16, 3>(d, it, lds);
+      } else {
+        if (it.e == 8) terminalNode<TR, false, 8, kRowRegs>(d, it, lds);
+        else terminalNode<TR, false, 16, kRowRegs>(d, it, lds);
+      }
       break;
     case kBwdShowdown:
-      if (it.e == 8) terminalNode<TR, true, 8>(d, it, lds);
-      else terminalNode<TR, true, 16>(d, it, lds);
+      if (it.na == 3) {
+        if (it.e == 8) terminalNode<TR, true, 8, 3>(d, it, lds);
+        else terminalNode<TR, true, 16, 3>(d, it, lds);
+      } else {
+        if (it.e == 8) terminalNode<TR, true, 8, kRowRegs>(d, it, lds);
+        else terminalNode<TR, true, 16, kRowRegs>(d, it, lds);
+      }
       break;
     default:
       fwdItem<TR>(d, it, fromStrat);
@@ -809,6 +827,7 @@ int ppsg_create(const ppsg_tree_desc* t, ppsg_t** out) {
         it.f = cl * 53;
         it.n = nT;
         it.m = nO;
+        it.na = (uint8_t)(std::max(nT, nO) <= 3 * kBlock ? 3 : kRowRegs);  // lanes' row registers
         const float* tv = t->term_val + u * 8 + tr * 4;
         it.v0 = tv[0];
         it.v1 = tv[1];
