@@ -1,30 +1,37 @@
 // MI355X DCFR engine: gfx950 kernels + the ppsg C ABI (include/ppsg.h).
 //
+// Memory: one device arena per engine. Every resident spot (compiled tree)
+// owns one chunk of it (index tables + f32 rows), so one engine runs one
+// spot (ppsg_create) or many (ppsg_batch_*) with the same kernels. Work
+// items carry byte offsets into the arena.
+//
 // One DCFR iteration = two alternating half-steps (traverser 0, then 1);
-// each half-step is a forward pass over depths 0..D (opponent reach) and a
-// backward pass over depths D..0 (traverser counterfactual values with the
-// fused DCFR update). Every (half-step, depth, pass) is one launch over a
-// flat list of work items; the whole iteration (plus the schedule-counter
-// bump) is captured once in a hipGraph and replayed.
+// each half-step is a forward pass over depths 0..D (opponent reach, plus
+// the fold/showdown values of each depth) and a backward pass over depths
+// D..0 (traverser counterfactual values with the fused DCFR update). For a
+// batch, depth d of every active spot goes into the same launch (a depth
+// wavefront over uneven trees), so the launch count per iteration is set by
+// the deepest tree, not by the number of spots. The whole iteration (plus
+// the per-spot schedule-counter bumps) is captured once in a hipGraph and
+// replayed; it is captured again only when the active set changes.
 //
-// Work items are 64-byte descriptors built on the host with every row offset
-// already resolved, so a workgroup issues its data loads after a single
-// (scalar, broadcast) descriptor load instead of walking node -> class ->
-// children -> offsets. Sibling rows are contiguous (child k of a node at
-// base + k * stride), so one base offset reaches every child.
+// Work items are 64-byte descriptors built on the host with every offset
+// resolved: a workgroup issues its data loads after one (scalar) descriptor
+// load. Sibling rows are contiguous (child k at base + k * stride).
 //
-//   forward  decide (opponent acts)  one lane per opponent combo, a tile of 256
+//   forward  decide (opponent acts)  one lane per opponent combo, 256 per item
 //   forward  chance (branch, tile)   child reach = parent reach / (52-board-4)
+//   forward  fold / showdown         one workgroup per node: opponent reach
+//                                    gathered + one blocked f64 LDS scan
 //   backward decide (tile)           RM+ value + fused DCFR update, or the
 //                                    opponent-side sum of child values
 //   backward chance (tile)           sum of child values over surviving branches
-//   backward fold / showdown         one workgroup per node: one blocked f64
-//                                    LDS scan over gathered opponent reach
 // Traverser-acting decide nodes have no forward item: their children's
 // reach rows alias the node's row (cuda/gpu_cfr.cu:1485-1514).
 //
 // No float atomics anywhere: every sum has a fixed order, so results are
-// bitwise reproducible and independent of launch shape.
+// bitwise reproducible and do not depend on launch shape or on which other
+// spots share the batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,6 +50,8 @@ constexpr int kBlock = 256;  // 4 wave64s
 constexpr int kTile = 256;   // combos per tile item
 constexpr int kMaxList = 1326;
 constexpr int kRowRegs = (kMaxList + kBlock - 1) / kBlock;  // 6
+constexpr int64_t kAlign = 256;
+constexpr int kGatherHeader = 256;  // bytes: 53 int32 segment bounds + pad
 
 thread_local std::string g_err;
 
@@ -59,9 +68,13 @@ int fail(const std::string& msg) {
                   std::to_string(__LINE__) + ")");                                        \
   } while (0)
 
-struct Ctl {
+// Per-spot control block: the device-side schedule index and table, and
+// whether the spot's work items still run (retired spots stay in the merged
+// work lists until the next compaction, masked out by `alive`).
+struct alignas(32) Ctl {
   long long counter;  // schedule index of the running iteration
   const float* coef;  // 3 floats per schedule index
+  int alive;
 };
 
 enum ItemKind : uint8_t {
@@ -73,23 +86,24 @@ enum ItemKind : uint8_t {
   kBwdShowdown = 5,
 };
 
-// One work item. Field meaning per kind:
-//   kFwdDecide   a regret/strategy row, b node reach row, c children's reach
-//                block (stride n), n opponent combos, na actions
-//   kFwdChance   a child class's to-parent map, b parent reach row, c child
-//                reach row, n child opponent combos, v0 = 1/(52-board-4)
-//   kBwdDecide   a regret/strategy row, b node cfv row, c children's cfv
-//                block, m child stride, n traverser combos, e = actor is
-//                the traverser
-//   kBwdChance   a branch x parent-slot map (child slot or -1), b node cfv
-//                row, c children's cfv block, m child stride, n traverser
-//                combos (parent), na branches
-//   kBwdFold /   a node reach row, b node cfv row, c gather-index list of
-//   kBwdShowdown the class (opponent), d traverser list offset, e gather
-//                entries per lane (8 or 16), f per-card segment table
-//                index (class * 53), n traverser combos, m opponent combos,
-//                v0..v2 payoffs (fold: v0), na = row registers per lane
-//                (3 when both lists fit 768 combos, else 6)
+// One work item; a, b, c, d are byte offsets into the arena; f is the
+// owning spot's slot (its control block).
+//   kFwdDecide   a regret row, d strategy row, b node reach row, c children's
+//                reach block (stride n), n opponent combos, na actions
+//   kFwdChance   a to-parent map (u16), b parent reach row, c child reach
+//                row, n child opponent combos, v0 = 1/(52-board-4)
+//   kBwdDecide   a regret row, d strategy row, b node cfv row, c children's
+//                cfv block, m child stride (floats), n traverser combos,
+//                e = actor is the traverser
+//   kBwdChance   a branch x parent-slot map (i16: child slot or -1), b node
+//                cfv row, c children's cfv block, m child stride, n
+//                traverser combos (parent), na branches
+//   kBwdFold /   a node reach row, b node cfv row, c the class's gather
+//   kBwdShowdown block (53 i32 segment bounds, then the u16 gather list), d
+//                per traverser combo metadata (fold: u32 cards + same;
+//                showdown: 4 u32), e gather entries per lane (8 or 16), na
+//                row registers per lane (3 or 6), n traverser combos, m
+//                opponent combos, v0..v2 payoffs (fold: v0)
 struct alignas(16) Item {
   int64_t a, b, c, d;
   int32_t n, m, e, f;
@@ -100,19 +114,14 @@ struct alignas(16) Item {
 static_assert(sizeof(Item) == 64, "work item must be one 64-byte line");
 
 struct Dev {
-  float* regret;
-  float* strat;
-  float* reach[2];
-  float* cfv;
-  const uint16_t* listCards[2];
-  const int16_t* listSame[2];
-  const uint16_t* toParent[2];
-  const int16_t* branchMap[2];
-  const int32_t* segOff[2];
-  const uint16_t* gather[2];
-  const uint32_t* sdMeta[2];
+  char* arena;
   Ctl* ctl;
 };
+
+template <typename T>
+__device__ __forceinline__ T* at(const Dev& d, int64_t off) {
+  return reinterpret_cast<T*>(d.arena + off);
+}
 
 __device__ __forceinline__ float sigmaOf(float x, float acc, float uniform, bool plus) {
   if (plus) return acc > 0.0f ? (x > 0.0f ? x / acc : 0.0f) : uniform;
@@ -120,22 +129,20 @@ __device__ __forceinline__ float sigmaOf(float x, float acc, float uniform, bool
 }
 
 // ---------------------------------------------------------------------------
-// Forward.
+// Forward reach.
 // ---------------------------------------------------------------------------
 
-template <int TR>
 __device__ __forceinline__ void fwdItem(const Dev& d, const Item& it, int fromStrat) {
-  float* __restrict__ R = d.reach[TR];
   const int lane = it.tile * kTile + (int)threadIdx.x;
   if (lane >= it.n) return;
+  const float* __restrict__ in = at<float>(d, it.b);
   if (it.kind == kFwdChance) {
-    const int tp = d.toParent[1 - TR][it.a + lane];
-    R[it.c + lane] = R[it.b + tp] * it.v0;
+    at<float>(d, it.c)[lane] = in[at<uint16_t>(d, it.a)[lane]] * it.v0;
     return;
   }
   const int n = it.n, na = it.na;
-  const float r = R[it.b + lane];
-  const float* __restrict__ src = (fromStrat ? d.strat : d.regret) + it.a + lane;
+  const float r = in[lane];
+  const float* __restrict__ src = at<float>(d, fromStrat ? it.d : it.a) + lane;
   const bool plus = !fromStrat;
   float acc = 0.0f;
   for (int a = 0; a < na; ++a) {
@@ -143,117 +150,12 @@ __device__ __forceinline__ void fwdItem(const Dev& d, const Item& it, int fromSt
     acc += plus ? (x > 0.0f ? x : 0.0f) : x;
   }
   const float uniform = 1.0f / (float)na;
-  float* __restrict__ out = R + it.c + lane;
-  for (int a = 0; a < na; ++a) out[(int64_t)a * n] = r * sigmaOf(src[(int64_t)a * n], acc, uniform, plus);
-}
-
-// ---------------------------------------------------------------------------
-// Backward.
-// ---------------------------------------------------------------------------
-
-// Decide tile, one lane per traverser combo. NA > 0: action count fixed at
-// compile time (values stay in registers); NA == 0: generic loop.
-template <int MODE, int NA>
-__device__ __forceinline__ void decideTile(const Dev& d, const Item& it) {
-  const int c = it.tile * kTile + (int)threadIdx.x;
-  const int n = it.n;
-  if (c >= n) return;
-  const int na = NA > 0 ? NA : it.na;
-  const int64_t stride = it.m;
-  const float* __restrict__ kid = d.cfv + it.c + c;
-  float* __restrict__ out = d.cfv + it.b + c;
-  if (!it.e) {  // opponent acts: values add up
-    float v = 0.0f;
-    for (int a = 0; a < na; ++a) v += kid[a * stride];
-    *out = v;
-    return;
-  }
-  if (MODE == PPSG_MODE_BR) {
-    float best = -1e30f;
-    for (int a = 0; a < na; ++a) best = fmaxf(best, kid[a * stride]);
-    *out = best;
-    return;
-  }
-  constexpr bool plus = MODE == PPSG_MODE_CFR;
-  float* __restrict__ src = (plus ? d.regret : d.strat) + it.a + c;
-  const float uniform = 1.0f / (float)na;
-  if (NA > 0) {
-    float x[NA > 0 ? NA : 1], v[NA > 0 ? NA : 1], s[NA > 0 ? NA : 1];
-    float acc = 0.0f;
-#pragma unroll
-    for (int a = 0; a < NA; ++a) {
-      x[a] = src[(int64_t)a * n];
-      v[a] = kid[a * stride];
-      acc += plus ? (x[a] > 0.0f ? x[a] : 0.0f) : x[a];
-    }
-    float val = 0.0f;
-#pragma unroll
-    for (int a = 0; a < NA; ++a) {
-      s[a] = sigmaOf(x[a], acc, uniform, plus);
-      val += s[a] * v[a];
-    }
-    *out = val;
-    if (MODE != PPSG_MODE_CFR) return;
-    const float* co = d.ctl->coef + 3 * d.ctl->counter;
-    const float pos = co[0], neg = co[1], avg = co[2];
-    float* __restrict__ st = d.strat + it.a + c;
-#pragma unroll
-    for (int a = 0; a < NA; ++a) {
-      st[(int64_t)a * n] = st[(int64_t)a * n] * avg + s[a];
-      src[(int64_t)a * n] = x[a] * (x[a] >= 0.0f ? pos : neg) + (v[a] - val);
-    }
-    return;
-  }
-  float acc = 0.0f;
-  for (int a = 0; a < na; ++a) {
-    const float x = src[(int64_t)a * n];
-    acc += plus ? (x > 0.0f ? x : 0.0f) : x;
-  }
-  float val = 0.0f;
+  float* __restrict__ out = at<float>(d, it.c) + lane;
   for (int a = 0; a < na; ++a)
-    val += sigmaOf(src[(int64_t)a * n], acc, uniform, plus) * kid[a * stride];
-  *out = val;
-  if (MODE != PPSG_MODE_CFR) return;
-  const float* co = d.ctl->coef + 3 * d.ctl->counter;
-  const float pos = co[0], neg = co[1], avg = co[2];
-  float* __restrict__ st = d.strat + it.a + c;
-  for (int a = 0; a < na; ++a) {
-    const float x = src[(int64_t)a * n];
-    const float sg = sigmaOf(x, acc, uniform, plus);
-    st[(int64_t)a * n] = st[(int64_t)a * n] * avg + sg;
-    src[(int64_t)a * n] = x * (x >= 0.0f ? pos : neg) + (kid[a * stride] - val);
-  }
+    out[(int64_t)a * n] = r * sigmaOf(src[(int64_t)a * n], acc, uniform, plus);
 }
 
-// Chance tile, one lane per traverser (parent) combo: sum of the child
-// values over the branches the combo survives, in branch order.
-template <int TR>
-__device__ __forceinline__ void chanceTile(const Dev& d, const Item& it) {
-  const int i = it.tile * kTile + (int)threadIdx.x;
-  const int n = it.n;
-  if (i >= n) return;
-  const int16_t* __restrict__ map = d.branchMap[TR] + it.a + i;
-  const float* __restrict__ kid = d.cfv + it.c;
-  const int64_t stride = it.m;
-  const int nb = it.na;
-  // Branches in chunks: all map loads of a chunk, then all value loads, then
-  // the adds in branch order (keeps many loads in flight; same sum order).
-  constexpr int kChunk = 16;
-  float v = 0.0f;
-  for (int b0 = 0; b0 < nb; b0 += kChunk) {
-    int slot[kChunk];
-    float x[kChunk];
-#pragma unroll
-    for (int k = 0; k < kChunk; ++k) slot[k] = b0 + k < nb ? map[(int64_t)(b0 + k) * n] : -1;
-#pragma unroll
-    for (int k = 0; k < kChunk; ++k) x[k] = slot[k] >= 0 ? kid[(b0 + k) * stride + slot[k]] : 0.0f;
-#pragma unroll
-    for (int k = 0; k < kChunk; ++k)
-      if (slot[k] >= 0) v += x[k];
-  }
-  d.cfv[it.b + i] = v;
-}
-
+// ---------------------------------------------------------------------------
 // Fold and showdown terminals. Opponent reach is staged in LDS, then each
 // lane gathers its PER consecutive entries of the class's gather list
 // (showdown: slots in strength order followed by the per-card segments in
@@ -264,9 +166,10 @@ __device__ __forceinline__ void chanceTile(const Dev& d, const Item& it) {
 //             at the compile-time cut points of its two cards' segments.
 // All global reads (reach, gather indices, per-combo metadata) are issued
 // before the first barrier.
-template <int TR, bool SD, int PER, int ROWS>
+// ---------------------------------------------------------------------------
+
+template <bool SD, int PER, int ROWS>
 __device__ void terminalNode(const Dev& d, const Item& it, char* lds) {
-  constexpr int OPP = 1 - TR;
   __shared__ double waveTot[kBlock / 64];
   __shared__ int so[53];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -275,7 +178,7 @@ __device__ void terminalNode(const Dev& d, const Item& it, char* lds) {
   double* X = reinterpret_cast<double*>(lds + (((nO + 1) * 4 + 15) & ~15));
 
   // 1. issue every global read
-  const float* __restrict__ r = d.reach[TR] + it.a;
+  const float* __restrict__ r = at<float>(d, it.a);
   float rv[ROWS];
 #pragma unroll
   for (int q = 0; q < ROWS; ++q) {
@@ -283,7 +186,7 @@ __device__ void terminalNode(const Dev& d, const Item& it, char* lds) {
     rv[q] = j < nO ? r[j] : 0.0f;
   }
   uint4 gi[PER / 8];
-  const uint4* gsrc = reinterpret_cast<const uint4*>(d.gather[OPP] + it.c) + tid * (PER / 8);
+  const uint4* gsrc = at<uint4>(d, it.c + kGatherHeader) + tid * (PER / 8);
 #pragma unroll
   for (int k = 0; k < PER / 8; ++k) gi[k] = gsrc[k];
   uint4 meta[ROWS];
@@ -291,16 +194,10 @@ __device__ void terminalNode(const Dev& d, const Item& it, char* lds) {
 #pragma unroll
   for (int q = 0; q < ROWS; ++q) {
     const int i = tid + q * kBlock;
-    if (SD) {
-      meta[q] = i < nT ? reinterpret_cast<const uint4*>(d.sdMeta[TR] + 4 * it.d)[i]
-                       : make_uint4(0, 0, 0, 0);
-    } else {
-      cards[q] = i < nT ? ((uint32_t)d.listCards[TR][it.d + i] |
-                           ((uint32_t)(d.listSame[TR][it.d + i] + 1) << 16))
-                        : 0u;
-    }
+    if (SD) meta[q] = i < nT ? at<uint4>(d, it.d)[i] : make_uint4(0, 0, 0, 0);
+    else cards[q] = i < nT ? at<uint32_t>(d, it.d)[i] : 0u;
   }
-  if (tid < 53) so[tid] = d.segOff[OPP][it.f + tid];
+  if (tid < 53) so[tid] = at<int32_t>(d, it.c)[tid];
 
   // 2. stage reach (raw[nO] = 0 is the padding sentinel)
 #pragma unroll
@@ -329,25 +226,22 @@ __device__ void terminalNode(const Dev& d, const Item& it, char* lds) {
   }
   if (lane == 63) waveTot[wave] = inc;
   __syncthreads();
-  double base = inc - own;
-  for (int w = 0; w < wave; ++w) base += waveTot[w];
-  {
-    double run = base;
+  double run = inc - own;
+  for (int w = 0; w < wave; ++w) run += waveTot[w];
 #pragma unroll
-    for (int k = 0; k < PER / 8; ++k) {
-      const uint32_t w[4] = {gi[k].x, gi[k].y, gi[k].z, gi[k].w};
+  for (int k = 0; k < PER / 8; ++k) {
+    const uint32_t w[4] = {gi[k].x, gi[k].y, gi[k].z, gi[k].w};
 #pragma unroll
-      for (int h = 0; h < 8; ++h) {
-        X[tid * PER + k * 8 + h] = run;
-        run += (double)raw[(w[h >> 1] >> ((h & 1) * 16)) & 0xffffu];
-      }
+    for (int h = 0; h < 8; ++h) {
+      X[tid * PER + k * 8 + h] = run;
+      run += (double)raw[(w[h >> 1] >> ((h & 1) * 16)) & 0xffffu];
     }
-    if (tid == kBlock - 1) X[kBlock * PER] = run;
   }
+  if (tid == kBlock - 1) X[kBlock * PER] = run;
   __syncthreads();
 
   // 4. per traverser combo
-  float* __restrict__ out = d.cfv + it.b;
+  float* __restrict__ out = at<float>(d, it.b);
   if (SD) {
     const double* Y = X + nO;  // per-card segment prefix (offset cancels)
     const double vWin = it.v0, vTie = it.v1, vLose = it.v2;
@@ -393,57 +287,181 @@ __device__ void terminalNode(const Dev& d, const Item& it, char* lds) {
 // Forward launch of one depth: opponent-reach items for the next depth plus
 // the terminal nodes of this depth (their values need only this depth's
 // reach, so they run here and the backward pass only combines).
-template <int TR>
+// kindMask: profiling attribution only (-1 = every kind in real launches).
+// masked: some listed spot is retired; skip its items.
 __global__ __launch_bounds__(kBlock) void fwdKernel(Dev d, const Item* __restrict__ items,
-                                                    int fromStrat, int kindMask) {
+                                                    int fromStrat, int kindMask, int masked) {
   extern __shared__ __align__(16) char lds[];
   const Item it = items[blockIdx.x];
-  // kindMask: profiling attribution only (-1 = every kind in real launches)
   if (!((kindMask >> it.kind) & 1)) return;
+  if (masked && !d.ctl[it.f].alive) return;
   switch (it.kind) {
     case kBwdFold:
       if (it.na == 3) {
-        if (it.e == 8) terminalNode<TR, false, 8, 3>(d, it, lds);
-        else terminalNode<TR, false, 16, 3>(d, it, lds);
+        if (it.e == 8) terminalNode<false, 8, 3>(d, it, lds);
+        else terminalNode<false, 16, 3>(d, it, lds);
       } else {
-        if (it.e == 8) terminalNode<TR, false, 8, kRowRegs>(d, it, lds);
-        else terminalNode<TR, false, 16, kRowRegs>(d, it, lds);
+        if (it.e == 8) terminalNode<false, 8, kRowRegs>(d, it, lds);
+        else terminalNode<false, 16, kRowRegs>(d, it, lds);
       }
       break;
     case kBwdShowdown:
       if (it.na == 3) {
-        if (it.e == 8) terminalNode<TR, true, 8, 3>(d, it, lds);
-        else terminalNode<TR, true, 16, 3>(d, it, lds);
+        if (it.e == 8) terminalNode<true, 8, 3>(d, it, lds);
+        else terminalNode<true, 16, 3>(d, it, lds);
       } else {
-        if (it.e == 8) terminalNode<TR, true, 8, kRowRegs>(d, it, lds);
-        else terminalNode<TR, true, 16, kRowRegs>(d, it, lds);
+        if (it.e == 8) terminalNode<true, 8, kRowRegs>(d, it, lds);
+        else terminalNode<true, 16, kRowRegs>(d, it, lds);
       }
       break;
     default:
-      fwdItem<TR>(d, it, fromStrat);
+      fwdItem(d, it, fromStrat);
   }
 }
 
-template <int TR, int MODE>
+// ---------------------------------------------------------------------------
+// Backward.
+// ---------------------------------------------------------------------------
+
+// Decide tile, one lane per traverser combo. NA > 0: action count fixed at
+// compile time (values stay in registers); NA == 0: generic loop.
+template <int MODE, int NA>
+__device__ __forceinline__ void decideTile(const Dev& d, const Item& it) {
+  const int c = it.tile * kTile + (int)threadIdx.x;
+  const int n = it.n;
+  if (c >= n) return;
+  const int na = NA > 0 ? NA : it.na;
+  const int64_t stride = it.m;
+  const float* __restrict__ kid = at<float>(d, it.c) + c;
+  float* __restrict__ out = at<float>(d, it.b) + c;
+  if (!it.e) {  // opponent acts: values add up
+    float v = 0.0f;
+    for (int a = 0; a < na; ++a) v += kid[a * stride];
+    *out = v;
+    return;
+  }
+  if (MODE == PPSG_MODE_BR) {
+    float best = -1e30f;
+    for (int a = 0; a < na; ++a) best = fmaxf(best, kid[a * stride]);
+    *out = best;
+    return;
+  }
+  constexpr bool plus = MODE == PPSG_MODE_CFR;
+  float* __restrict__ src = at<float>(d, plus ? it.a : it.d) + c;
+  const float uniform = 1.0f / (float)na;
+  if (NA > 0) {
+    float x[NA > 0 ? NA : 1], v[NA > 0 ? NA : 1], s[NA > 0 ? NA : 1];
+    float acc = 0.0f;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      x[a] = src[(int64_t)a * n];
+      v[a] = kid[a * stride];
+      acc += plus ? (x[a] > 0.0f ? x[a] : 0.0f) : x[a];
+    }
+    float val = 0.0f;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      s[a] = sigmaOf(x[a], acc, uniform, plus);
+      val += s[a] * v[a];
+    }
+    *out = val;
+    if (MODE != PPSG_MODE_CFR) return;
+    const Ctl& ctl = d.ctl[it.f];
+    const float* co = ctl.coef + 3 * ctl.counter;
+    const float pos = co[0], neg = co[1], avg = co[2];
+    float* __restrict__ st = at<float>(d, it.d) + c;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      st[(int64_t)a * n] = st[(int64_t)a * n] * avg + s[a];
+      src[(int64_t)a * n] = x[a] * (x[a] >= 0.0f ? pos : neg) + (v[a] - val);
+    }
+    return;
+  }
+  float acc = 0.0f;
+  for (int a = 0; a < na; ++a) {
+    const float x = src[(int64_t)a * n];
+    acc += plus ? (x > 0.0f ? x : 0.0f) : x;
+  }
+  float val = 0.0f;
+  for (int a = 0; a < na; ++a)
+    val += sigmaOf(src[(int64_t)a * n], acc, uniform, plus) * kid[a * stride];
+  *out = val;
+  if (MODE != PPSG_MODE_CFR) return;
+  const Ctl& ctl = d.ctl[it.f];
+  const float* co = ctl.coef + 3 * ctl.counter;
+  const float pos = co[0], neg = co[1], avg = co[2];
+  float* __restrict__ st = at<float>(d, it.d) + c;
+  for (int a = 0; a < na; ++a) {
+    const float x = src[(int64_t)a * n];
+    const float sg = sigmaOf(x, acc, uniform, plus);
+    st[(int64_t)a * n] = st[(int64_t)a * n] * avg + sg;
+    src[(int64_t)a * n] = x * (x >= 0.0f ? pos : neg) + (kid[a * stride] - val);
+  }
+}
+
+// Chance tile, one lane per traverser (parent) combo: sum of the child
+// values over the branches the combo survives, in branch order. Branches
+// go in chunks: all map loads of a chunk, then all value loads, then the
+// adds (keeps many loads in flight; same summation order).
+__device__ __forceinline__ void chanceTile(const Dev& d, const Item& it) {
+  const int i = it.tile * kTile + (int)threadIdx.x;
+  const int n = it.n;
+  if (i >= n) return;
+  // 32-bit unsigned byte offsets from uniform bases (a branch block is at
+  // most 255 x 1326 floats): one VGPR per address instead of two
+  const char* __restrict__ map = at<char>(d, it.a);
+  const char* __restrict__ kid = at<char>(d, it.c);
+  const uint32_t stride = (uint32_t)it.m;
+  const int nb = it.na;
+  constexpr int kChunk = 16;
+  float v = 0.0f;
+  for (int b0 = 0; b0 < nb; b0 += kChunk) {
+    int slot[kChunk];
+    float x[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k)
+      slot[k] = b0 + k < nb
+                    ? *reinterpret_cast<const int16_t*>(map + 2u * ((uint32_t)(b0 + k) * (uint32_t)n + (uint32_t)i))
+                    : -1;
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k)
+      x[k] = slot[k] >= 0
+                 ? *reinterpret_cast<const float*>(kid + 4u * ((uint32_t)(b0 + k) * stride + (uint32_t)slot[k]))
+                 : 0.0f;
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k)
+      if (slot[k] >= 0) v += x[k];
+  }
+  at<float>(d, it.b)[i] = v;
+}
+
+template <int MODE>
 __global__ __launch_bounds__(kBlock) void bwdKernel(Dev d, const Item* __restrict__ items,
-                                                    int kindMask) {
+                                                    int kindMask, int masked) {
   const Item it = items[blockIdx.x];
   if (!((kindMask >> it.kind) & 1)) return;
-  switch (it.kind) {
-    case kBwdDecide:
-      switch (it.na) {
-        case 1: decideTile<MODE, 1>(d, it); break;
-        case 2: decideTile<MODE, 2>(d, it); break;
-        case 3: decideTile<MODE, 3>(d, it); break;
-        case 4: decideTile<MODE, 4>(d, it); break;
-        case 5: decideTile<MODE, 5>(d, it); break;
-        case 6: decideTile<MODE, 6>(d, it); break;
-        default: decideTile<MODE, 0>(d, it); break;
-      }
-      break;
-    default:
-      chanceTile<TR>(d, it);
-      break;
+  if (masked && !d.ctl[it.f].alive) return;
+  if (it.kind == kBwdChance) {
+    chanceTile(d, it);
+    return;
+  }
+  switch (it.na) {
+    case 1: decideTile<MODE, 1>(d, it); break;
+    case 2: decideTile<MODE, 2>(d, it); break;
+    case 3: decideTile<MODE, 3>(d, it); break;
+    case 4: decideTile<MODE, 4>(d, it); break;
+    case 5: decideTile<MODE, 5>(d, it); break;
+    case 6: decideTile<MODE, 6>(d, it); break;
+    default: decideTile<MODE, 0>(d, it); break;
+  }
+}
+
+// Bumps the schedule index of every active spot (the last node of an
+// iteration's graph).
+__global__ void advanceCounters(Ctl* ctl, const int* __restrict__ slots, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    Ctl& c = ctl[slots[i]];
+    c.counter = c.counter + 1;
   }
 }
 
@@ -455,149 +473,91 @@ __global__ __launch_bounds__(kBlock) void copyProbe(const float4* __restrict__ a
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
 }
 
-__global__ void advanceCounter(Ctl* ctl) {
-  if (threadIdx.x == 0) ctl->counter = ctl->counter + 1;
-}
-
-}  // namespace
-
 // ---------------------------------------------------------------------------
-// Host side.
+// Host: arena allocator (first fit over an offset-ordered free list; freed
+// blocks merge with free neighbours).
 // ---------------------------------------------------------------------------
 
-struct ppsg_engine {
-  Dev dev{};
-  std::vector<void*> allocs;
-  int64_t bytes = 0;
-  int n[2] = {0, 0};
-  int maxDepth = 0;
-  int64_t regTotal = 0, cfvTotal = 0, reachTotal[2] = {0, 0};
-  size_t terminalLds = 0;
-  // per (tr, depth): device item lists + counts, and whether the forward
-  // launch carries terminal nodes (it then needs dynamic LDS)
-  std::vector<Item*> fwdItems[2], bwdItems[2];
-  std::vector<int> fwdCount[2], bwdCount[2];
-  std::vector<char> fwdTerminal[2];
-  Ctl* ctl = nullptr;
-  float* coef = nullptr;
-  int64_t coefCap = 0;
-  int64_t coefLen = 0;  // valid schedule entries
-  int64_t counter = 0;  // host mirror of the device schedule index
-  hipStream_t stream = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double lastMs = -1.0;
-  int64_t kernelsPerIter = 0, itemsPerIter = 0;
-  double fwdBytes = 0.0, bwdBytes = 0.0;  // algorithmic f32 row bytes / iteration
+int64_t roundUp(int64_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
 
-  template <typename T>
-  int upload(T*& dst, const T* src, int64_t count) {
-    dst = nullptr;
-    if (count <= 0) return 0;
-    void* p = nullptr;
-    HIP_TRY(hipMalloc(&p, (size_t)count * sizeof(T)));
-    allocs.push_back(p);
-    bytes += count * (int64_t)sizeof(T);
-    if (src) HIP_TRY(hipMemcpy(p, src, (size_t)count * sizeof(T), hipMemcpyHostToDevice));
-    dst = static_cast<T*>(p);
-    return 0;
-  }
-  template <typename T>
-  int uploadC(const T*& dst, const T* src, int64_t count) {
-    T* p = nullptr;
-    const int rc = upload(p, src, count);
-    dst = p;
-    return rc;
-  }
+struct Arena {
+  char* base = nullptr;
+  int64_t cap = 0;
+  std::map<int64_t, int64_t> holes;  // offset -> bytes
 
-  int launchFwd(int tr, int dpt, int fromStrat, int kindMask, hipStream_t s);
-  int launchBwd(int tr, int dpt, int mode, int kindMask, hipStream_t s);
-  int launchHalf(int tr, int mode, hipStream_t s);
-  int capture();
-  ~ppsg_engine();
+  int64_t alloc(int64_t bytes) {
+    bytes = roundUp(bytes);
+    for (auto it = holes.begin(); it != holes.end(); ++it) {
+      if (it->second < bytes) continue;
+      const int64_t off = it->first, rest = it->second - bytes;
+      holes.erase(it);
+      if (rest > 0) holes[off + bytes] = rest;
+      return off;
+    }
+    return -1;
+  }
+  void release(int64_t off, int64_t bytes) {
+    bytes = roundUp(bytes);
+    auto it = holes.emplace(off, bytes).first;
+    auto next = std::next(it);
+    if (next != holes.end() && it->first + it->second == next->first) {
+      it->second += next->second;
+      holes.erase(next);
+    }
+    if (it != holes.begin()) {
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == it->first) {
+        prev->second += it->second;
+        holes.erase(it);
+      }
+    }
+  }
+  int64_t largestHole() const {
+    int64_t m = 0;
+    for (const auto& h : holes) m = std::max(m, h.second);
+    return m;
+  }
+  int64_t freeBytes() const {
+    int64_t m = 0;
+    for (const auto& h : holes) m += h.second;
+    return m;
+  }
 };
 
-ppsg_engine::~ppsg_engine() {
-  if (exec) (void)hipGraphExecDestroy(exec);
-  if (graph) (void)hipGraphDestroy(graph);
-  if (ev0) (void)hipEventDestroy(ev0);
-  if (ev1) (void)hipEventDestroy(ev1);
-  if (stream) (void)hipStreamDestroy(stream);
-  for (void* p : allocs) (void)hipFree(p);
-}
-
-int ppsg_engine::launchFwd(int tr, int dpt, int fromStrat, int kindMask, hipStream_t s) {
-  const int cnt = fwdCount[tr][dpt];
-  if (cnt == 0) return 0;
-  const size_t lds = fwdTerminal[tr][dpt] ? terminalLds : 0;
-  if (tr == 0) fwdKernel<0><<<cnt, kBlock, lds, s>>>(dev, fwdItems[0][dpt], fromStrat, kindMask);
-  else fwdKernel<1><<<cnt, kBlock, lds, s>>>(dev, fwdItems[1][dpt], fromStrat, kindMask);
-  HIP_TRY(hipGetLastError());
-  return 0;
-}
-
-int ppsg_engine::launchBwd(int tr, int dpt, int mode, int kindMask, hipStream_t s) {
-  const int cnt = bwdCount[tr][dpt];
-  if (cnt == 0) return 0;
-  const Item* it = bwdItems[tr][dpt];
-#define PPSG_BWD(T, M) bwdKernel<T, M><<<cnt, kBlock, 0, s>>>(dev, it, kindMask)
-  if (tr == 0) {
-    if (mode == PPSG_MODE_CFR) PPSG_BWD(0, PPSG_MODE_CFR);
-    else if (mode == PPSG_MODE_EV) PPSG_BWD(0, PPSG_MODE_EV);
-    else PPSG_BWD(0, PPSG_MODE_BR);
-  } else {
-    if (mode == PPSG_MODE_CFR) PPSG_BWD(1, PPSG_MODE_CFR);
-    else if (mode == PPSG_MODE_EV) PPSG_BWD(1, PPSG_MODE_EV);
-    else PPSG_BWD(1, PPSG_MODE_BR);
+// Byte layout of one spot's chunk.
+struct Layout {
+  int64_t bytes = 0;
+  int64_t toParent[2] = {0, 0}, bmap[2] = {0, 0}, gather[2] = {0, 0};
+  int64_t cardSame[2] = {0, 0}, sdMeta[2] = {0, 0};
+  int64_t regret = 0, strat = 0, cfv = 0, reach[2] = {0, 0};
+  int64_t add(int64_t b) {
+    const int64_t off = bytes;
+    bytes += roundUp(std::max<int64_t>(b, 1));
+    return off;
   }
-#undef PPSG_BWD
-  HIP_TRY(hipGetLastError());
-  return 0;
-}
-
-int ppsg_engine::launchHalf(int tr, int mode, hipStream_t s) {
-  const int fromStrat = mode == PPSG_MODE_CFR ? 0 : 1;
-  for (int dpt = 0; dpt <= maxDepth; ++dpt)
-    if (launchFwd(tr, dpt, fromStrat, -1, s) != 0) return -1;
-  for (int dpt = maxDepth; dpt >= 0; --dpt)
-    if (launchBwd(tr, dpt, mode, -1, s) != 0) return -1;
-  return 0;
-}
-
-int ppsg_engine::capture() {
-  if (exec) return 0;
-  HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-  int rc = launchHalf(0, PPSG_MODE_CFR, stream);
-  if (rc == 0) rc = launchHalf(1, PPSG_MODE_CFR, stream);
-  if (rc == 0) advanceCounter<<<1, 64, 0, stream>>>(ctl);
-  hipGraph_t g = nullptr;
-  const hipError_t e = hipStreamEndCapture(stream, &g);
-  if (rc != 0) {
-    if (g) (void)hipGraphDestroy(g);
-    return rc;
-  }
-  if (e != hipSuccess) return fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-  graph = g;
-  HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-  return 0;
-}
-
-namespace {
+};
 
 // Gather entries per lane for a list of `len` entries: 8 or 16 (one or two
 // 16-byte index loads).
 int perLane(int len) { return len <= kBlock * 8 ? 8 : 16; }
 
-// Per-class gather lists (fold: per-card segments; showdown: strength order
-// followed by the per-card segments), padded to kBlock * per entries with the
-// sentinel slot n (a zero in LDS).
-struct GatherTables {
-  std::vector<uint16_t> data[2];
+// Dynamic LDS of a terminal item: reach (+ sentinel) and the f64 prefixes.
+size_t terminalLdsOf(const Item& it) {
+  return (size_t)(((it.m + 1) * 4 + 15) & ~15) + (size_t)(kBlock * it.e + 1) * 8;
+}
+
+// Host-side tables of one spot, built from its description.
+struct SpotTables {
+  std::vector<int16_t> bmap[2];       // per parent class: [branch][parent slot]
+  std::map<int, int64_t> bmapOff[2];  // class -> element offset
+  // per (class, kind): [53 i32 segment bounds | pad to 256 B | u16 gather
+  // list padded to kBlock * per with the sentinel slot n]
+  std::vector<char> gather[2];
   std::map<int, int64_t> foldOff[2], sdOff[2];
   std::map<int, int> foldPer[2], sdPer[2];
+  std::vector<uint32_t> cardSame[2];  // per list entry: c1 | c2 << 8 | (same+1) << 16
 
-  void add(const ppsg_tree_desc* t, int p, int cl, bool sd) {
+  void gatherBlock(const ppsg_tree_desc* t, int p, int cl, bool sd) {
     auto& offs = sd ? sdOff[p] : foldOff[p];
     if (offs.count(cl)) return;
     const int n = t->cls_n[p][cl];
@@ -607,12 +567,785 @@ struct GatherTables {
     list.insert(list.end(), t->seg_slot[p] + 2 * lo, t->seg_slot[p] + 2 * lo + 2 * n);
     const int per = perLane((int)list.size());
     list.resize((size_t)kBlock * per, (uint16_t)n);
-    while (data[p].size() % 8) data[p].push_back(0);  // 16-byte aligned lanes
-    offs[cl] = (int64_t)data[p].size();
+    std::vector<char>& g = gather[p];
+    g.resize((g.size() + 15) / 16 * 16, 0);
+    offs[cl] = (int64_t)g.size();
     (sd ? sdPer[p] : foldPer[p])[cl] = per;
-    data[p].insert(data[p].end(), list.begin(), list.end());
+    const size_t at0 = g.size();
+    g.resize(at0 + kGatherHeader + list.size() * 2, 0);
+    std::memcpy(g.data() + at0, t->seg_off[p] + (int64_t)cl * 53, 53 * 4);
+    std::memcpy(g.data() + at0 + kGatherHeader, list.data(), list.size() * 2);
   }
 };
+
+enum SpotState { kFree = 0, kLive = 1, kParked = 2, kDoomed = 3 };
+
+struct Spot {
+  int state = kFree;
+  bool inLists = false;         // its items are in the merged work lists
+  int64_t off = -1, bytes = 0;  // arena chunk
+  float* coefDev = nullptr;     // schedule table (own allocation)
+  int64_t coefCap = 0, coefLen = 0;
+  int64_t counter = 0;          // host mirror of the device schedule index
+  int n[2] = {0, 0};
+  int maxDepth = 0;
+  Layout L;
+  int64_t regTotal = 0, cfvTotal = 0, reachTotal[2] = {0, 0};
+  int64_t rootCfv = 0;  // chunk-relative byte offset of the root cfv row
+  // work items per [traverser][0 forward, 1 backward][depth], chunk-relative
+  std::vector<std::vector<Item>> items[2][2];
+  int64_t itemCount = 0;
+  double fwdBytes = 0.0, bwdBytes = 0.0;
+};
+
+// Plans a spot: chunk layout, host tables and chunk-relative work items.
+int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
+  if (!t || t->n_nodes <= 0 || t->n_base[0] <= 0 || t->n_base[1] <= 0) return fail("empty tree");
+  const int nc = t->n_class;
+  const int64_t N = t->n_nodes;
+  for (int p = 0; p < 2; ++p)
+    for (int c = 0; c < nc; ++c)
+      if (t->cls_n[p][c] > kMaxList) return fail("combo list longer than 1326");
+  auto kidsOf = [&](int64_t u, int& b, int& e) {
+    b = t->child_base[u];
+    e = u + 1 < N ? t->child_base[u + 1] : (int)t->n_child;
+  };
+  for (int64_t u = 0; u < N; ++u) {
+    const int cl = t->cls[u];
+    if (t->kind[u] == PPSG_CHANCE) {
+      int b, e;
+      kidsOf(u, b, e);
+      if (e - b > 255) return fail("more than 255 chance branches");
+      for (int p = 0; p < 2; ++p) {
+        if (T.bmapOff[p].count(cl)) continue;
+        T.bmapOff[p][cl] = (int64_t)T.bmap[p].size();
+        const int np = t->cls_n[p][cl];
+        for (int k = b; k < e; ++k) {
+          const int ccl = t->cls[t->child_flat[k]];
+          const int16_t* fp = t->from_parent[p] + t->from_parent_off[p][ccl];
+          T.bmap[p].insert(T.bmap[p].end(), fp, fp + np);
+        }
+      }
+    } else if (t->kind[u] == PPSG_FOLD || t->kind[u] == PPSG_SHOWDOWN) {
+      for (int p = 0; p < 2; ++p) T.gatherBlock(t, p, cl, t->kind[u] == PPSG_SHOWDOWN);
+    }
+  }
+  for (int p = 0; p < 2; ++p) {
+    T.cardSame[p].resize(t->list_total[p]);
+    for (int64_t e = 0; e < t->list_total[p]; ++e)
+      T.cardSame[p][e] =
+          (uint32_t)t->list_cards[p][e] | ((uint32_t)(t->list_same[p][e] + 1) << 16);
+  }
+  Layout& L = S.L;
+  for (int p = 0; p < 2; ++p) {
+    L.toParent[p] = L.add(2 * t->list_total[p]);
+    L.bmap[p] = L.add(2 * (int64_t)T.bmap[p].size());
+    L.gather[p] = L.add((int64_t)T.gather[p].size());
+    L.cardSame[p] = L.add(4 * t->list_total[p]);
+    L.sdMeta[p] = L.add(16 * t->list_total[p]);
+  }
+  L.regret = L.add(4 * t->reg_total);
+  L.strat = L.add(4 * t->reg_total);
+  L.cfv = L.add(4 * t->cfv_total);
+  L.reach[0] = L.add(4 * t->reach_total[0]);
+  L.reach[1] = L.add(4 * t->reach_total[1]);
+  S.bytes = L.bytes;
+  S.n[0] = t->n_base[0];
+  S.n[1] = t->n_base[1];
+  S.maxDepth = t->max_depth;
+  S.regTotal = t->reg_total;
+  S.cfvTotal = t->cfv_total;
+  S.reachTotal[0] = t->reach_total[0];
+  S.reachTotal[1] = t->reach_total[1];
+  S.rootCfv = L.cfv + 4 * t->cfv_off[0];
+
+  const int D = t->max_depth + 1;
+  for (int tr = 0; tr < 2; ++tr)
+    for (int ps = 0; ps < 2; ++ps) S.items[tr][ps].assign(D, {});
+  auto tiles = [](int n) { return (n + kTile - 1) / kTile; };
+  for (int64_t u = 0; u < N; ++u) {
+    const int dep = t->depth[u], cl = t->cls[u], k = t->kind[u];
+    int kb, ke;
+    kidsOf(u, kb, ke);
+    for (int tr = 0; tr < 2; ++tr) {
+      const int op = 1 - tr;
+      const int nT = t->cls_n[tr][cl], nO = t->cls_n[op][cl];
+      std::vector<Item>& fwd = S.items[tr][0][dep];
+      std::vector<Item>& bwd = S.items[tr][1][dep];
+      if (k == PPSG_DECIDE) {
+        const int na = t->na[u];
+        const int first = t->child_flat[kb];
+        if (t->owner[u] == op) {
+          for (int q = 0; q < tiles(nO); ++q) {
+            Item it{};
+            it.kind = kFwdDecide;
+            it.a = L.regret + 4 * t->reg_off[u];
+            it.d = L.strat + 4 * t->reg_off[u];
+            it.b = L.reach[tr] + 4 * t->reach_off[tr][u];
+            it.c = L.reach[tr] + 4 * t->reach_off[tr][first];
+            it.n = nO;
+            it.na = (uint8_t)na;
+            it.tile = (uint16_t)q;
+            fwd.push_back(it);
+          }
+        }
+        const int64_t stride = na > 1 ? t->cfv_off[t->child_flat[kb + 1]] - t->cfv_off[first] : 0;
+        for (int q = 0; q < tiles(nT); ++q) {
+          Item it{};
+          it.kind = kBwdDecide;
+          it.a = L.regret + 4 * t->reg_off[u];
+          it.d = L.strat + 4 * t->reg_off[u];
+          it.b = L.cfv + 4 * t->cfv_off[u];
+          it.c = L.cfv + 4 * t->cfv_off[first];
+          it.m = (int32_t)stride;
+          it.n = nT;
+          it.na = (uint8_t)na;
+          it.e = t->owner[u] == tr;
+          it.tile = (uint16_t)q;
+          bwd.push_back(it);
+        }
+      } else if (k == PPSG_CHANCE) {
+        const int nb = ke - kb;
+        const float inv = 1.0f / (float)(52 - t->cls_board_n[cl] - 4);
+        for (int b = kb; b < ke; ++b) {
+          const int ch = t->child_flat[b];
+          const int ccl = t->cls[ch];
+          for (int q = 0; q < tiles(t->cls_n[op][ccl]); ++q) {
+            Item it{};
+            it.kind = kFwdChance;
+            it.a = L.toParent[op] + 2 * t->cls_list_off[op][ccl];
+            it.b = L.reach[tr] + 4 * t->reach_off[tr][u];
+            it.c = L.reach[tr] + 4 * t->reach_off[tr][ch];
+            it.n = t->cls_n[op][ccl];
+            it.v0 = inv;
+            it.tile = (uint16_t)q;
+            fwd.push_back(it);
+          }
+        }
+        const int first = t->child_flat[kb];
+        const int64_t stride = nb > 1 ? t->cfv_off[t->child_flat[kb + 1]] - t->cfv_off[first] : 0;
+        for (int q = 0; q < tiles(nT); ++q) {
+          Item it{};
+          it.kind = kBwdChance;
+          it.a = L.bmap[tr] + 2 * T.bmapOff[tr][cl];
+          it.b = L.cfv + 4 * t->cfv_off[u];
+          it.c = L.cfv + 4 * t->cfv_off[first];
+          it.m = (int32_t)stride;
+          it.n = nT;
+          it.na = (uint8_t)nb;
+          it.tile = (uint16_t)q;
+          bwd.push_back(it);
+        }
+      } else {
+        const bool sd = k == PPSG_SHOWDOWN;
+        Item it{};
+        it.kind = sd ? kBwdShowdown : kBwdFold;
+        it.a = L.reach[tr] + 4 * t->reach_off[tr][u];
+        it.b = L.cfv + 4 * t->cfv_off[u];
+        it.c = L.gather[op] + (sd ? T.sdOff[op][cl] : T.foldOff[op][cl]);
+        it.d = sd ? L.sdMeta[tr] + 16 * t->cls_list_off[tr][cl]
+                  : L.cardSame[tr] + 4 * t->cls_list_off[tr][cl];
+        it.e = sd ? T.sdPer[op][cl] : T.foldPer[op][cl];
+        it.na = (uint8_t)(std::max(nT, nO) <= 3 * kBlock ? 3 : kRowRegs);
+        it.n = nT;
+        it.m = nO;
+        const float* tv = t->term_val + u * 8 + tr * 4;
+        it.v0 = tv[0];
+        it.v1 = tv[1];
+        it.v2 = tv[2];
+        fwd.push_back(it);
+      }
+    }
+  }
+  for (int tr = 0; tr < 2; ++tr)
+    for (int dep = 0; dep < D; ++dep) {
+      // terminal workgroups first: they are the long ones
+      std::vector<Item>& f = S.items[tr][0][dep];
+      std::stable_partition(f.begin(), f.end(), [](const Item& x) { return x.kind >= kBwdFold; });
+      S.itemCount += (int64_t)(f.size() + S.items[tr][1][dep].size());
+    }
+  // Algorithmic bytes of one iteration, split by pass: every f32 row element
+  // read or written once per use (index tables excluded).
+  for (int64_t u = 0; u < N; ++u) {
+    const int cl = t->cls[u];
+    const double n0 = t->cls_n[0][cl], n1 = t->cls_n[1][cl];
+    if (t->kind[u] == PPSG_DECIDE) {
+      const double a = t->na[u];
+      const double np = t->owner[u] == 0 ? n0 : n1, nq = t->owner[u] == 0 ? n1 : n0;
+      S.fwdBytes += 4 * np * (2 * a + 1);
+      S.bwdBytes += 4 * np * (5 * a + 1) + 4 * nq * (a + 1);
+    } else if (t->kind[u] == PPSG_CHANCE) {
+      int b, e;
+      kidsOf(u, b, e);
+      for (int k = b; k < e; ++k) {
+        const int ccl = t->cls[t->child_flat[k]];
+        const double nck = (double)t->cls_n[0][ccl] + t->cls_n[1][ccl];
+        S.fwdBytes += 8 * nck;
+        S.bwdBytes += 4 * nck;
+      }
+      S.bwdBytes += 4 * (n0 + n1);
+    } else {
+      S.bwdBytes += 8 * (n0 + n1);
+    }
+  }
+  return 0;
+}
+
+// One merged work list of a (traverser, pass, depth): the items of every
+// listed spot, appended as spots arrive.
+struct DevList {
+  Item* buf = nullptr;
+  int64_t count = 0, cap = 0;
+  size_t lds = 0;  // forward lists: dynamic LDS of the largest terminal item
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Engine: one arena, any number of spots.
+//
+// Spot lifecycle: add -> live (items appended to the merged lists); park /
+// remove -> alive flag cleared, items stay listed but masked; compaction
+// (when masked items outnumber live ones, or when an add needs the room)
+// rebuilds the lists from the live spots and frees removed spots' chunks.
+// Any change re-captures the iteration graph on the next step.
+// ---------------------------------------------------------------------------
+
+struct ppsg_engine {
+  Arena arena;
+  Ctl* ctl = nullptr;
+  int maxSpots = 0;
+  std::vector<Spot> spots;
+  std::vector<DevList> lists[2][2];  // [traverser][pass][depth]
+  int64_t listedItems = 0, deadItems = 0;
+  std::vector<int> live;
+  int* liveDev = nullptr;
+  int64_t liveCap = 0;
+  bool graphDirty = true;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double lastMs = -1.0;
+
+  Dev dev() const { return Dev{arena.base, ctl}; }
+  bool valid(int slot) const {
+    return slot >= 0 && slot < maxSpots && spots[slot].state != kFree &&
+           spots[slot].state != kDoomed;
+  }
+  bool isLive(int slot) const { return valid(slot) && spots[slot].state == kLive; }
+  int depthCount() const { return (int)lists[0][0].size(); }
+  int masked() const { return deadItems > 0 ? 1 : 0; }
+  int init(int64_t bytes, int spotsCap);
+  int add(const ppsg_tree_desc* t, int* slot);
+  int remove(int slot);
+  int setActive(int slot, bool on);
+  int setAlive(int slot, int v);
+  int appendSpot(int slot);
+  int compact();
+  int prepare();
+  void dropGraph();
+  int launchFwd(int tr, int dpt, int fromStrat, int kindMask, hipStream_t s);
+  int launchBwd(int tr, int dpt, int mode, int kindMask, hipStream_t s);
+  int launchHalf(int tr, int mode, hipStream_t s);
+  int capture();
+  int covers(int64_t iters);
+  int64_t kernelsPerIter() const;
+  ~ppsg_engine();
+};
+
+void ppsg_engine::dropGraph() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  exec = nullptr;
+  graph = nullptr;
+  graphDirty = true;
+}
+
+ppsg_engine::~ppsg_engine() {
+  dropGraph();
+  for (int tr = 0; tr < 2; ++tr)
+    for (int ps = 0; ps < 2; ++ps)
+      for (DevList& l : lists[tr][ps])
+        if (l.buf) (void)hipFree(l.buf);
+  for (Spot& S : spots)
+    if (S.coefDev) (void)hipFree(S.coefDev);
+  if (liveDev) (void)hipFree(liveDev);
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  if (stream) (void)hipStreamDestroy(stream);
+  if (ctl) (void)hipFree(ctl);
+  if (arena.base) (void)hipFree(arena.base);
+}
+
+int ppsg_engine::init(int64_t bytes, int spotsCap) {
+  if (bytes <= 0 || spotsCap <= 0 || spotsCap > 65536) return fail("engine: bad capacity");
+  bytes = roundUp(bytes);
+  void* p = nullptr;
+  HIP_TRY(hipMalloc(&p, (size_t)bytes));
+  arena.base = static_cast<char*>(p);
+  arena.cap = bytes;
+  arena.holes[0] = bytes;
+  HIP_TRY(hipMalloc(&p, sizeof(Ctl) * (size_t)spotsCap));
+  ctl = static_cast<Ctl*>(p);
+  HIP_TRY(hipMemset(ctl, 0, sizeof(Ctl) * (size_t)spotsCap));
+  maxSpots = spotsCap;
+  spots.assign(spotsCap, Spot{});
+  HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&ev0));
+  HIP_TRY(hipEventCreate(&ev1));
+  return 0;
+}
+
+int ppsg_engine::setAlive(int slot, int v) {
+  HIP_TRY(hipStreamSynchronize(stream));
+  HIP_TRY(hipMemcpy(&ctl[slot].alive, &v, sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Appends one spot's items to the end of every merged list.
+int ppsg_engine::appendSpot(int slot) {
+  Spot& S = spots[slot];
+  const int D = S.maxDepth + 1;
+  dropGraph();
+  for (int tr = 0; tr < 2; ++tr)
+    for (int ps = 0; ps < 2; ++ps) {
+      std::vector<DevList>& L = lists[tr][ps];
+      if ((int)L.size() < D) L.resize(D);
+      for (int dep = 0; dep < D; ++dep) {
+        const std::vector<Item>& src = S.items[tr][ps][dep];
+        if (src.empty()) continue;
+        std::vector<Item> v(src);
+        for (Item& it : v) {
+          it.a += S.off;
+          it.b += S.off;
+          it.c += S.off;
+          it.d += S.off;
+          it.f = slot;
+          if (it.kind >= kBwdFold) L[dep].lds = std::max(L[dep].lds, terminalLdsOf(it));
+        }
+        DevList& dl = L[dep];
+        const int64_t need = dl.count + (int64_t)v.size();
+        if (need > dl.cap) {
+          const int64_t cap = std::max<int64_t>(need, dl.cap + dl.cap / 2);
+          void* p = nullptr;
+          HIP_TRY(hipMalloc(&p, (size_t)cap * sizeof(Item)));
+          if (dl.count)
+            HIP_TRY(hipMemcpy(p, dl.buf, (size_t)dl.count * sizeof(Item), hipMemcpyDeviceToDevice));
+          if (dl.buf) (void)hipFree(dl.buf);
+          dl.buf = static_cast<Item*>(p);
+          dl.cap = cap;
+        }
+        HIP_TRY(hipMemcpy(dl.buf + dl.count, v.data(), v.size() * sizeof(Item),
+                          hipMemcpyHostToDevice));
+        dl.count = need;
+      }
+    }
+  S.inLists = true;
+  listedItems += S.itemCount;
+  return 0;
+}
+
+// Rebuilds the merged lists from the live spots (terminal items of every
+// spot first within each forward list) and frees removed spots.
+int ppsg_engine::compact() {
+  dropGraph();
+  for (int tr = 0; tr < 2; ++tr)
+    for (int ps = 0; ps < 2; ++ps)
+      for (DevList& l : lists[tr][ps]) {
+        l.count = 0;
+        l.lds = 0;
+      }
+  listedItems = 0;
+  deadItems = 0;
+  int D = 0;
+  for (int s = 0; s < maxSpots; ++s) {
+    Spot& S = spots[s];
+    if (S.state == kDoomed) {
+      arena.release(S.off, S.bytes);
+      if (S.coefDev) (void)hipFree(S.coefDev);
+      S = Spot{};
+      continue;
+    }
+    S.inLists = false;
+    if (S.state == kLive) D = std::max(D, S.maxDepth + 1);
+  }
+  for (int tr = 0; tr < 2; ++tr)
+    for (int ps = 0; ps < 2; ++ps) {
+      std::vector<DevList>& L = lists[tr][ps];
+      for (int dep = D; dep < (int)L.size(); ++dep)
+        if (L[dep].buf) (void)hipFree(L[dep].buf);
+      L.resize(D);
+      for (int dep = 0; dep < D; ++dep) {
+        std::vector<Item> v;
+        for (int pass = 0; pass < (ps == 0 ? 2 : 1); ++pass)  // forward: terminals, then the rest
+          for (int s = 0; s < maxSpots; ++s) {
+            const Spot& S = spots[s];
+            if (S.state != kLive || dep > S.maxDepth) continue;
+            for (Item it : S.items[tr][ps][dep]) {
+              if (ps == 0 && (it.kind >= kBwdFold) != (pass == 0)) continue;
+              it.a += S.off;
+              it.b += S.off;
+              it.c += S.off;
+              it.d += S.off;
+              it.f = s;
+              if (it.kind >= kBwdFold) L[dep].lds = std::max(L[dep].lds, terminalLdsOf(it));
+              v.push_back(it);
+            }
+          }
+        DevList& dl = L[dep];
+        if ((int64_t)v.size() > dl.cap) {
+          if (dl.buf) (void)hipFree(dl.buf);
+          dl.buf = nullptr;
+          void* p = nullptr;
+          HIP_TRY(hipMalloc(&p, v.size() * sizeof(Item)));
+          dl.buf = static_cast<Item*>(p);
+          dl.cap = (int64_t)v.size();
+        }
+        if (!v.empty())
+          HIP_TRY(hipMemcpy(dl.buf, v.data(), v.size() * sizeof(Item), hipMemcpyHostToDevice));
+        dl.count = (int64_t)v.size();
+      }
+    }
+  for (int s = 0; s < maxSpots; ++s)
+    if (spots[s].state == kLive) {
+      spots[s].inLists = true;
+      listedItems += spots[s].itemCount;
+    }
+  return 0;
+}
+
+int ppsg_engine::add(const ppsg_tree_desc* t, int* slot) {
+  Spot S;
+  SpotTables T;
+  if (planSpot(t, S, T) != 0) return -1;
+  int free = -1;
+  for (int s = 0; s < maxSpots && free < 0; ++s)
+    if (spots[s].state == kFree) free = s;
+  int64_t off = arena.alloc(S.bytes);
+  bool doomed = false;
+  for (const Spot& x : spots) doomed |= x.state == kDoomed;
+  if ((free < 0 || off < 0) && doomed) {  // removed spots still hold room
+    if (off >= 0) arena.release(off, S.bytes);
+    if (compact() != 0) return -1;
+    for (int s = 0; s < maxSpots && free < 0; ++s)
+      if (spots[s].state == kFree) free = s;
+    off = arena.alloc(S.bytes);
+  }
+  if (free < 0) {
+    if (off >= 0) arena.release(off, S.bytes);
+    g_err = "batch full: no free spot slot";
+    return -2;
+  }
+  if (off < 0) {
+    g_err = "arena full: spot needs " + std::to_string(S.bytes) + " bytes, largest free block " +
+            std::to_string(arena.largestHole());
+    return -2;
+  }
+  S.off = off;
+  HIP_TRY(hipStreamSynchronize(stream));
+  char* base = arena.base + off;
+  auto put = [&](int64_t rel, const void* src, int64_t bytes) -> int {
+    if (bytes > 0) HIP_TRY(hipMemcpy(base + rel, src, (size_t)bytes, hipMemcpyHostToDevice));
+    return 0;
+  };
+  const Layout& L = S.L;
+  int rc = 0;
+  for (int p = 0; p < 2 && rc == 0; ++p) {
+    if (put(L.toParent[p], t->to_parent[p], 2 * t->list_total[p]) ||
+        put(L.bmap[p], T.bmap[p].data(), 2 * (int64_t)T.bmap[p].size()) ||
+        put(L.gather[p], T.gather[p].data(), (int64_t)T.gather[p].size()) ||
+        put(L.cardSame[p], T.cardSame[p].data(), 4 * t->list_total[p]) ||
+        put(L.sdMeta[p], t->sd_meta[p], 16 * t->list_total[p]) ||
+        put(L.reach[p] + 4 * t->reach_off[p][0], t->root_reach[p], 4 * (int64_t)t->n_base[1 - p]))
+      rc = -1;
+  }
+  // untrained rows: zero regret and strategy sums (uniform average strategy)
+  if (rc == 0 && (hipMemset(base + L.regret, 0, 4 * (size_t)t->reg_total) != hipSuccess ||
+                  hipMemset(base + L.strat, 0, 4 * (size_t)t->reg_total) != hipSuccess))
+    rc = fail("row init failed");
+  Ctl c{};
+  c.alive = 1;
+  if (rc == 0 && hipMemcpy(ctl + free, &c, sizeof(Ctl), hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail("control block init failed");
+  if (rc != 0) {
+    arena.release(off, S.bytes);
+    return -1;
+  }
+  S.state = kLive;
+  spots[free] = std::move(S);
+  if (appendSpot(free) != 0) {
+    // possibly listed in part: mask it and let the next compaction free it
+    Spot& F = spots[free];
+    F.state = kDoomed;
+    if (!F.inLists) {
+      F.inLists = true;
+      listedItems += F.itemCount;
+    }
+    deadItems += F.itemCount;
+    (void)setAlive(free, 0);
+    return -1;
+  }
+  *slot = free;
+  return 0;
+}
+
+int ppsg_engine::setActive(int slot, bool on) {
+  if (!valid(slot)) return fail("no such spot");
+  Spot& S = spots[slot];
+  if ((S.state == kLive) == on) return 0;
+  dropGraph();
+  if (on) {
+    S.state = kLive;
+    if (S.inLists) {
+      if (setAlive(slot, 1) != 0) return -1;
+      deadItems -= S.itemCount;
+    } else {
+      if (setAlive(slot, 1) != 0 || appendSpot(slot) != 0) return -1;
+    }
+  } else {
+    S.state = kParked;
+    if (setAlive(slot, 0) != 0) return -1;
+    if (S.inLists) deadItems += S.itemCount;
+  }
+  return 0;
+}
+
+int ppsg_engine::remove(int slot) {
+  if (!valid(slot)) return fail("no such spot");
+  Spot& S = spots[slot];
+  dropGraph();
+  if (S.inLists) {
+    if (S.state == kLive) {
+      if (setAlive(slot, 0) != 0) return -1;
+      deadItems += S.itemCount;
+    }
+    S.state = kDoomed;  // chunk and slot held until the lists drop its items
+    return 0;
+  }
+  arena.release(S.off, S.bytes);
+  if (S.coefDev) (void)hipFree(S.coefDev);
+  S = Spot{};
+  return 0;
+}
+
+// Before a launch sequence: compact when masked work dominates, refresh the
+// live slot list.
+int ppsg_engine::prepare() {
+  if (deadItems > 0 && 2 * deadItems > listedItems)
+    if (compact() != 0) return -1;
+  if (!graphDirty) return 0;
+  live.clear();
+  for (int s = 0; s < maxSpots; ++s)
+    if (spots[s].state == kLive) live.push_back(s);
+  if ((int64_t)live.size() > liveCap) {
+    if (liveDev) (void)hipFree(liveDev);
+    liveDev = nullptr;
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, (size_t)maxSpots * sizeof(int)));
+    liveDev = static_cast<int*>(p);
+    liveCap = maxSpots;
+  }
+  if (!live.empty())
+    HIP_TRY(hipMemcpy(liveDev, live.data(), live.size() * sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int64_t ppsg_engine::kernelsPerIter() const {
+  int64_t k = 1;  // counter bump
+  for (int tr = 0; tr < 2; ++tr)
+    for (int ps = 0; ps < 2; ++ps)
+      for (const DevList& l : lists[tr][ps]) k += l.count > 0;
+  return k;
+}
+
+int ppsg_engine::launchFwd(int tr, int dpt, int fromStrat, int kindMask, hipStream_t s) {
+  const DevList& l = lists[tr][0][dpt];
+  if (l.count == 0) return 0;
+  fwdKernel<<<(unsigned)l.count, kBlock, l.lds, s>>>(dev(), l.buf, fromStrat, kindMask, masked());
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int ppsg_engine::launchBwd(int tr, int dpt, int mode, int kindMask, hipStream_t s) {
+  const DevList& l = lists[tr][1][dpt];
+  if (l.count == 0) return 0;
+  const unsigned g = (unsigned)l.count;
+  const int mk = masked();
+  if (mode == PPSG_MODE_CFR) bwdKernel<PPSG_MODE_CFR><<<g, kBlock, 0, s>>>(dev(), l.buf, kindMask, mk);
+  else if (mode == PPSG_MODE_EV) bwdKernel<PPSG_MODE_EV><<<g, kBlock, 0, s>>>(dev(), l.buf, kindMask, mk);
+  else bwdKernel<PPSG_MODE_BR><<<g, kBlock, 0, s>>>(dev(), l.buf, kindMask, mk);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int ppsg_engine::launchHalf(int tr, int mode, hipStream_t s) {
+  const int fromStrat = mode == PPSG_MODE_CFR ? 0 : 1;
+  const int D = depthCount();
+  for (int dpt = 0; dpt < D; ++dpt)
+    if (launchFwd(tr, dpt, fromStrat, -1, s) != 0) return -1;
+  for (int dpt = D - 1; dpt >= 0; --dpt)
+    if (launchBwd(tr, dpt, mode, -1, s) != 0) return -1;
+  return 0;
+}
+
+int ppsg_engine::capture() {
+  if (prepare() != 0) return -1;
+  if (exec && !graphDirty) return 0;
+  dropGraph();
+  HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  int rc = launchHalf(0, PPSG_MODE_CFR, stream);
+  if (rc == 0) rc = launchHalf(1, PPSG_MODE_CFR, stream);
+  if (rc == 0) advanceCounters<<<1, kBlock, 0, stream>>>(ctl, liveDev, (int)live.size());
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(stream, &g);
+  if (rc != 0) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  graph = g;
+  HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  graphDirty = false;
+  return 0;
+}
+
+// Every CFR launch reads coef[counter] of each live spot: refuse work that
+// would run past any live spot's table.
+int ppsg_engine::covers(int64_t iters) {
+  for (int s = 0; s < maxSpots; ++s) {
+    const Spot& S = spots[s];
+    if (S.state != kLive) continue;
+    if (!S.coefDev) return fail("spot " + std::to_string(s) + ": no schedule set");
+    if (S.counter < 0 || S.counter + iters > S.coefLen)
+      return fail("spot " + std::to_string(s) + ": schedule table too short: index " +
+                  std::to_string(S.counter) + " + " + std::to_string(iters) + " > " +
+                  std::to_string(S.coefLen));
+  }
+  return 0;
+}
+
+namespace {
+
+int setSchedule(ppsg_t* h, int slot, const float* coef, int64_t n) {
+  if (!h) return fail("null handle");
+  if (!coef || n <= 0) return fail("bad schedule");
+  if (!h->valid(slot)) return fail("no such spot");
+  Spot& S = h->spots[slot];
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (n > S.coefCap) {
+    const int64_t cap = std::max<int64_t>(n, 2 * S.coefCap);
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, (size_t)cap * 3 * sizeof(float)));
+    if (S.coefDev) (void)hipFree(S.coefDev);
+    S.coefDev = static_cast<float*>(p);
+    S.coefCap = cap;
+    // kernels read the table through the spot's control block: no re-capture
+    const float* dp = S.coefDev;
+    HIP_TRY(hipMemcpy(&h->ctl[slot].coef, &dp, sizeof(dp), hipMemcpyHostToDevice));
+  }
+  HIP_TRY(hipMemcpy(S.coefDev, coef, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice));
+  S.coefLen = n;
+  return 0;
+}
+
+int setCounter(ppsg_t* h, int slot, int64_t t) {
+  if (!h) return fail("null handle");
+  if (!h->valid(slot)) return fail("no such spot");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  const long long v = t;
+  HIP_TRY(hipMemcpy(&h->ctl[slot].counter, &v, sizeof(v), hipMemcpyHostToDevice));
+  h->spots[slot].counter = t;
+  return 0;
+}
+
+int getCounter(ppsg_t* h, int slot, int64_t* t) {
+  if (!h || !t) return fail("null argument");
+  if (!h->valid(slot)) return fail("no such spot");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  long long v = 0;
+  HIP_TRY(hipMemcpy(&v, &h->ctl[slot].counter, sizeof(v), hipMemcpyDeviceToHost));
+  *t = v;
+  return 0;
+}
+
+int resetSpot(ppsg_t* h, int slot) {
+  if (!h) return fail("null handle");
+  if (!h->valid(slot)) return fail("no such spot");
+  const Spot& S = h->spots[slot];
+  char* base = h->arena.base + S.off;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  HIP_TRY(hipMemset(base + S.L.regret, 0, 4 * (size_t)S.regTotal));
+  HIP_TRY(hipMemset(base + S.L.strat, 0, 4 * (size_t)S.regTotal));
+  return setCounter(h, slot, 0);
+}
+
+int iterate(ppsg_t* h, int n) {
+  if (!h) return fail("null handle");
+  if (n <= 0) return 0;
+  if (h->prepare() != 0) return -1;
+  if (h->live.empty()) return fail("no live spot");
+  if (h->covers(n) != 0) return -1;
+  if (h->capture() != 0) return -1;
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  for (int i = 0; i < n; ++i) HIP_TRY(hipGraphLaunch(h->exec, h->stream));
+  HIP_TRY(hipEventRecord(h->ev1, h->stream));
+  HIP_TRY(hipEventSynchronize(h->ev1));
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->lastMs = ms;
+  for (int s : h->live) h->spots[s].counter += n;
+  return 0;
+}
+
+int walk(ppsg_t* h, int tr, int mode, int nSlots, const int* slots, float* const* rootCfv) {
+  if (!h) return fail("null handle");
+  if (nSlots > 0 && (!slots || !rootCfv)) return fail("null argument");
+  if ((tr != 0 && tr != 1) || (mode != PPSG_MODE_EV && mode != PPSG_MODE_BR))
+    return fail("walk: bad traverser or mode");
+  for (int i = 0; i < nSlots; ++i)
+    if (!h->isLive(slots[i])) return fail("walk: spot " + std::to_string(slots[i]) + " is not live");
+  if (h->prepare() != 0) return -1;
+  if (h->live.empty()) return fail("no live spot");
+  if (h->launchHalf(tr, mode, h->stream) != 0) return -1;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  for (int i = 0; i < nSlots; ++i) {
+    const Spot& S = h->spots[slots[i]];
+    if (rootCfv[i])
+      HIP_TRY(hipMemcpy(rootCfv[i], h->arena.base + S.off + S.rootCfv, sizeof(float) * S.n[tr],
+                        hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int64_t bufOf(const Spot& S, int buf, int64_t& total) {
+  switch (buf) {
+    case PPSG_REGRET: total = S.regTotal; return S.L.regret;
+    case PPSG_STRAT: total = S.regTotal; return S.L.strat;
+    case PPSG_REACH0: total = S.reachTotal[0]; return S.L.reach[0];
+    case PPSG_REACH1: total = S.reachTotal[1]; return S.L.reach[1];
+    case PPSG_CFV: total = S.cfvTotal; return S.L.cfv;
+  }
+  total = 0;
+  return -1;
+}
+
+int transfer(ppsg_t* h, int slot, int buf, int64_t off, int64_t n, void* host, bool toDevice) {
+  if (!h || (n > 0 && !host)) return fail("null argument");
+  if (!h->valid(slot)) return fail("no such spot");
+  const Spot& S = h->spots[slot];
+  int64_t total = 0;
+  const int64_t rel = bufOf(S, buf, total);
+  if (rel < 0 || off < 0 || n < 0 || off + n > total) return fail("row transfer out of range");
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  char* dev = h->arena.base + S.off + rel + 4 * off;
+  if (n) {
+    if (toDevice) HIP_TRY(hipMemcpy(dev, host, (size_t)n * 4, hipMemcpyHostToDevice));
+    else HIP_TRY(hipMemcpy(host, dev, (size_t)n * 4, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
 
 }  // namespace
 
@@ -650,249 +1383,18 @@ int ppsg_copy_probe(int64_t bytes, int reps, double* ms_per_launch) {
   return rc;
 }
 
+// ---- one spot: a batch of one, in slot 0 ----
+
 int ppsg_create(const ppsg_tree_desc* t, ppsg_t** out) {
   if (!t || !out) return fail("null argument");
   *out = nullptr;
-  if (t->n_nodes <= 0 || t->n_base[0] <= 0 || t->n_base[1] <= 0) return fail("empty tree");
+  int64_t need = 0;
+  if (ppsg_footprint(t, &need) != 0) return -1;
   auto* E = new ppsg_engine();
-  auto bail = [&](int rc) {
+  int slot = -1;
+  if (E->init(need, 1) != 0 || E->add(t, &slot) != 0) {
     delete E;
-    return rc;
-  };
-  Dev& d = E->dev;
-  const int nc = t->n_class;
-  const int64_t N = t->n_nodes;
-  int maxList = 0;
-  for (int p = 0; p < 2; ++p)
-    for (int c = 0; c < nc; ++c) maxList = std::max(maxList, t->cls_n[p][c]);
-  if (maxList > kMaxList) return bail(fail("combo list longer than 1326"));
-  auto kidsOf = [&](int64_t u, int& b, int& e) {
-    b = t->child_base[u];
-    e = u + 1 < N ? t->child_base[u + 1] : (int)t->n_child;
-  };
-
-  // branch maps per parent class: [branch][parent slot] -> child slot or -1
-  std::vector<int16_t> bmap[2];
-  std::map<int, int64_t> bmapOff[2];
-  GatherTables G;
-  for (int64_t u = 0; u < N; ++u) {
-    const int cl = t->cls[u];
-    if (t->kind[u] == PPSG_CHANCE) {
-      int b, e;
-      kidsOf(u, b, e);
-      for (int p = 0; p < 2; ++p) {
-        if (bmapOff[p].count(cl)) continue;
-        bmapOff[p][cl] = (int64_t)bmap[p].size();
-        const int np = t->cls_n[p][cl];
-        for (int k = b; k < e; ++k) {
-          const int ccl = t->cls[t->child_flat[k]];
-          const int16_t* fp = t->from_parent[p] + t->from_parent_off[p][ccl];
-          bmap[p].insert(bmap[p].end(), fp, fp + np);
-        }
-      }
-    } else if (t->kind[u] == PPSG_FOLD || t->kind[u] == PPSG_SHOWDOWN) {
-      for (int p = 0; p < 2; ++p) G.add(t, p, cl, t->kind[u] == PPSG_SHOWDOWN);
-    }
-  }
-  for (int p = 0; p < 2; ++p) {
-    if (E->uploadC(d.listCards[p], t->list_cards[p], t->list_total[p]) ||
-        E->uploadC(d.listSame[p], t->list_same[p], t->list_total[p]) ||
-        E->uploadC(d.toParent[p], t->to_parent[p], t->list_total[p]) ||
-        E->uploadC(d.branchMap[p], bmap[p].data(), std::max<int64_t>((int64_t)bmap[p].size(), 1)) ||
-        E->uploadC(d.segOff[p], t->seg_off[p], (int64_t)nc * 53) ||
-        E->uploadC(d.gather[p], G.data[p].data(), std::max<int64_t>((int64_t)G.data[p].size(), 8)) ||
-        E->uploadC(d.sdMeta[p], t->sd_meta[p], 4 * t->list_total[p]))
-      return bail(-1);
-  }
-  E->n[0] = t->n_base[0];
-  E->n[1] = t->n_base[1];
-  E->maxDepth = t->max_depth;
-  E->regTotal = t->reg_total;
-  E->cfvTotal = t->cfv_total;
-  E->reachTotal[0] = t->reach_total[0];
-  E->reachTotal[1] = t->reach_total[1];
-  if (E->upload(d.regret, (const float*)nullptr, std::max<int64_t>(t->reg_total, 1)) ||
-      E->upload(d.strat, (const float*)nullptr, std::max<int64_t>(t->reg_total, 1)) ||
-      E->upload(d.cfv, (const float*)nullptr, t->cfv_total) ||
-      E->upload(d.reach[0], (const float*)nullptr, t->reach_total[0]) ||
-      E->upload(d.reach[1], (const float*)nullptr, t->reach_total[1]))
-    return bail(-1);
-  // untrained rows: zero regret and strategy sums (uniform average strategy)
-  if (hipMemset(d.regret, 0, sizeof(float) * std::max<int64_t>(t->reg_total, 1)) != hipSuccess ||
-      hipMemset(d.strat, 0, sizeof(float) * std::max<int64_t>(t->reg_total, 1)) != hipSuccess)
-    return bail(fail("row init failed"));
-  for (int tr = 0; tr < 2; ++tr)
-    if (hipMemcpy(d.reach[tr] + t->reach_off[tr][0], t->root_reach[tr],
-                  sizeof(float) * t->n_base[1 - tr], hipMemcpyHostToDevice) != hipSuccess)
-      return bail(fail("root reach upload failed"));
-  {
-    void* p = nullptr;
-    if (hipMalloc(&p, sizeof(Ctl)) != hipSuccess) return bail(fail("ctl alloc failed"));
-    E->allocs.push_back(p);
-    E->ctl = static_cast<Ctl*>(p);
-    d.ctl = E->ctl;
-    Ctl zero{0, nullptr};
-    if (hipMemcpy(E->ctl, &zero, sizeof(Ctl), hipMemcpyHostToDevice) != hipSuccess)
-      return bail(fail("ctl init failed"));
-  }
-
-  // work items
-  const int D = t->max_depth + 1;
-  std::vector<std::vector<Item>> fwd[2], bwd[2];
-  for (int tr = 0; tr < 2; ++tr) {
-    fwd[tr].assign(D, {});
-    bwd[tr].assign(D, {});
-    E->fwdTerminal[tr].assign(D, 0);
-  }
-  auto tiles = [](int n) { return (n + kTile - 1) / kTile; };
-  int maxPer = 8;
-  for (int64_t u = 0; u < N; ++u) {
-    const int dep = t->depth[u], cl = t->cls[u], k = t->kind[u];
-    int kb, ke;
-    kidsOf(u, kb, ke);
-    for (int tr = 0; tr < 2; ++tr) {
-      const int op = 1 - tr;
-      const int nT = t->cls_n[tr][cl], nO = t->cls_n[op][cl];
-      if (k == PPSG_DECIDE) {
-        const int na = t->na[u];
-        const int first = t->child_flat[kb];
-        if (t->owner[u] == op) {
-          for (int q = 0; q < tiles(nO); ++q) {
-            Item it{};
-            it.kind = kFwdDecide;
-            it.a = t->reg_off[u];
-            it.b = t->reach_off[tr][u];
-            it.c = t->reach_off[tr][first];
-            it.n = nO;
-            it.na = (uint8_t)na;
-            it.tile = (uint16_t)q;
-            fwd[tr][dep].push_back(it);
-          }
-        }
-        const int64_t stride = na > 1 ? t->cfv_off[t->child_flat[kb + 1]] - t->cfv_off[first] : 0;
-        for (int q = 0; q < tiles(nT); ++q) {
-          Item it{};
-          it.kind = kBwdDecide;
-          it.a = t->reg_off[u];
-          it.b = t->cfv_off[u];
-          it.c = t->cfv_off[first];
-          it.m = (int32_t)stride;
-          it.n = nT;
-          it.na = (uint8_t)na;
-          it.e = t->owner[u] == tr;
-          it.tile = (uint16_t)q;
-          bwd[tr][dep].push_back(it);
-        }
-      } else if (k == PPSG_CHANCE) {
-        const int nb = ke - kb;
-        const float inv = 1.0f / (float)(52 - t->cls_board_n[cl] - 4);
-        for (int b = kb; b < ke; ++b) {
-          const int ch = t->child_flat[b];
-          const int ccl = t->cls[ch];
-          for (int q = 0; q < tiles(t->cls_n[op][ccl]); ++q) {
-            Item it{};
-            it.kind = kFwdChance;
-            it.a = t->cls_list_off[op][ccl];
-            it.b = t->reach_off[tr][u];
-            it.c = t->reach_off[tr][ch];
-            it.n = t->cls_n[op][ccl];
-            it.v0 = inv;
-            it.tile = (uint16_t)q;
-            fwd[tr][dep].push_back(it);
-          }
-        }
-        const int first = t->child_flat[kb];
-        const int64_t stride = nb > 1 ? t->cfv_off[t->child_flat[kb + 1]] - t->cfv_off[first] : 0;
-        for (int q = 0; q < tiles(nT); ++q) {
-          Item it{};
-          it.kind = kBwdChance;
-          it.a = bmapOff[tr][cl];
-          it.b = t->cfv_off[u];
-          it.c = t->cfv_off[first];
-          it.m = (int32_t)stride;
-          it.n = nT;
-          it.na = (uint8_t)nb;
-          it.tile = (uint16_t)q;
-          bwd[tr][dep].push_back(it);
-        }
-      } else {
-        const bool sd = k == PPSG_SHOWDOWN;
-        Item it{};
-        it.kind = sd ? kBwdShowdown : kBwdFold;
-        it.a = t->reach_off[tr][u];
-        it.b = t->cfv_off[u];
-        it.c = sd ? G.sdOff[op][cl] : G.foldOff[op][cl];
-        it.d = t->cls_list_off[tr][cl];
-        it.e = sd ? G.sdPer[op][cl] : G.foldPer[op][cl];
-        it.f = cl * 53;
-        it.n = nT;
-        it.m = nO;
-        it.na = (uint8_t)(std::max(nT, nO) <= 3 * kBlock ? 3 : kRowRegs);  // lanes' row registers
-        const float* tv = t->term_val + u * 8 + tr * 4;
-        it.v0 = tv[0];
-        it.v1 = tv[1];
-        it.v2 = tv[2];
-        fwd[tr][dep].push_back(it);
-        E->fwdTerminal[tr][dep] = 1;
-        maxPer = std::max(maxPer, it.e);
-      }
-    }
-  }
-  // Algorithmic bytes of one iteration, split by pass: every f32 row element
-  // read or written once per use (index tables excluded).
-  for (int64_t u = 0; u < N; ++u) {
-    const int cl = t->cls[u];
-    const double n0 = t->cls_n[0][cl], n1 = t->cls_n[1][cl];
-    if (t->kind[u] == PPSG_DECIDE) {
-      const double a = t->na[u];
-      const double np = t->owner[u] == 0 ? n0 : n1, nq = t->owner[u] == 0 ? n1 : n0;
-      E->fwdBytes += 4 * np * (2 * a + 1);
-      E->bwdBytes += 4 * np * (5 * a + 1) + 4 * nq * (a + 1);
-    } else if (t->kind[u] == PPSG_CHANCE) {
-      int b, e;
-      kidsOf(u, b, e);
-      for (int k = b; k < e; ++k) {
-        const int ccl = t->cls[t->child_flat[k]];
-        const double nck = (double)t->cls_n[0][ccl] + t->cls_n[1][ccl];
-        E->fwdBytes += 8 * nck;
-        E->bwdBytes += 4 * nck;
-      }
-      E->bwdBytes += 4 * (n0 + n1);
-    } else {
-      E->bwdBytes += 8 * (n0 + n1);
-    }
-  }
-  for (int tr = 0; tr < 2; ++tr) {
-    E->fwdItems[tr].assign(D, nullptr);
-    E->bwdItems[tr].assign(D, nullptr);
-    E->fwdCount[tr].assign(D, 0);
-    E->bwdCount[tr].assign(D, 0);
-    for (int dep = 0; dep < D; ++dep) {
-      // terminal workgroups first: they are the long ones
-      std::stable_sort(fwd[tr][dep].begin(), fwd[tr][dep].end(), [](const Item& a, const Item& b) {
-        return a.kind >= kBwdFold && b.kind < kBwdFold;
-      });
-      E->fwdCount[tr][dep] = (int)fwd[tr][dep].size();
-      E->bwdCount[tr][dep] = (int)bwd[tr][dep].size();
-      if (E->upload(E->fwdItems[tr][dep], fwd[tr][dep].data(), (int64_t)fwd[tr][dep].size()) ||
-          E->upload(E->bwdItems[tr][dep], bwd[tr][dep].data(), (int64_t)bwd[tr][dep].size()))
-        return bail(-1);
-      E->kernelsPerIter += (fwd[tr][dep].empty() ? 0 : 1) + (bwd[tr][dep].empty() ? 0 : 1);
-      E->itemsPerIter += (int64_t)(fwd[tr][dep].size() + bwd[tr][dep].size());
-    }
-  }
-  E->kernelsPerIter += 1;
-  // LDS of one terminal workgroup: reach (+ sentinel) and the f64 prefix array
-  E->terminalLds = (size_t)(((maxList + 1) * 4 + 15) & ~15) + (size_t)(kBlock * maxPer + 1) * 8;
-  if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&E->ev0) != hipSuccess || hipEventCreate(&E->ev1) != hipSuccess)
-    return bail(fail("stream/event creation failed"));
-  if (E->terminalLds > 64 * 1024) {
-    const void* ks[] = {(const void*)fwdKernel<0>, (const void*)fwdKernel<1>};
-    for (const void* k : ks)
-      if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)E->terminalLds) != hipSuccess)
-        return bail(fail("cannot raise dynamic LDS limit"));
+    return -1;
   }
   *out = E;
   return 0;
@@ -900,126 +1402,35 @@ int ppsg_create(const ppsg_tree_desc* t, ppsg_t** out) {
 
 void ppsg_destroy(ppsg_t* h) { delete h; }
 
-int ppsg_set_schedule(ppsg_t* h, const float* coef, int64_t n) {
-  if (!h || !coef || n <= 0) return fail("bad schedule");
-  HIP_TRY(hipStreamSynchronize(h->stream));
-  if (n > h->coefCap) {
-    void* p = nullptr;
-    const int64_t cap = std::max<int64_t>(n, 2 * h->coefCap);
-    HIP_TRY(hipMalloc(&p, (size_t)cap * 3 * sizeof(float)));
-    h->allocs.push_back(p);
-    h->bytes += cap * 12;
-    h->coef = static_cast<float*>(p);
-    h->coefCap = cap;
-    // kernels read the table through the control block: no re-capture
-    HIP_TRY(hipMemcpy(&h->ctl->coef, &h->coef, sizeof(float*), hipMemcpyHostToDevice));
-  }
-  HIP_TRY(hipMemcpy(h->coef, coef, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice));
-  h->coefLen = n;
-  return 0;
-}
-
-// Every CFR launch reads coef[counter]: refuse work that would run past the table.
-static int scheduleCovers(ppsg_t* h, int64_t iters) {
-  if (!h->coef) return fail("no schedule set");
-  if (h->counter < 0 || h->counter + iters > h->coefLen)
-    return fail("schedule table too short: index " + std::to_string(h->counter) + " + " +
-                std::to_string(iters) + " > " + std::to_string(h->coefLen));
-  return 0;
-}
-
-int ppsg_set_counter(ppsg_t* h, int64_t t) {
-  if (!h) return fail("null handle");
-  HIP_TRY(hipStreamSynchronize(h->stream));
-  long long v = t;
-  HIP_TRY(hipMemcpy(&h->ctl->counter, &v, sizeof(v), hipMemcpyHostToDevice));
-  h->counter = t;
-  return 0;
-}
-
-int ppsg_get_counter(ppsg_t* h, int64_t* t) {
-  if (!h || !t) return fail("null argument");
-  HIP_TRY(hipStreamSynchronize(h->stream));
-  long long v = 0;
-  HIP_TRY(hipMemcpy(&v, &h->ctl->counter, sizeof(v), hipMemcpyDeviceToHost));
-  *t = v;
-  return 0;
-}
-
-int ppsg_reset(ppsg_t* h) {
-  if (!h) return fail("null handle");
-  HIP_TRY(hipMemsetAsync(h->dev.regret, 0, (size_t)std::max<int64_t>(h->regTotal, 1) * 4, h->stream));
-  HIP_TRY(hipMemsetAsync(h->dev.strat, 0, (size_t)std::max<int64_t>(h->regTotal, 1) * 4, h->stream));
-  HIP_TRY(hipStreamSynchronize(h->stream));
-  return ppsg_set_counter(h, 0);
-}
-
-int ppsg_iterate(ppsg_t* h, int n) {
-  if (!h) return fail("null handle");
-  if (n <= 0) return 0;
-  if (scheduleCovers(h, n) != 0) return -1;
-  if (h->capture() != 0) return -1;
-  HIP_TRY(hipEventRecord(h->ev0, h->stream));
-  for (int i = 0; i < n; ++i) HIP_TRY(hipGraphLaunch(h->exec, h->stream));
-  HIP_TRY(hipEventRecord(h->ev1, h->stream));
-  HIP_TRY(hipEventSynchronize(h->ev1));
-  float ms = 0.0f;
-  HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-  h->lastMs = ms;
-  h->counter += n;
-  return 0;
-}
+int ppsg_set_schedule(ppsg_t* h, const float* coef, int64_t n) { return setSchedule(h, 0, coef, n); }
+int ppsg_set_counter(ppsg_t* h, int64_t t) { return setCounter(h, 0, t); }
+int ppsg_get_counter(ppsg_t* h, int64_t* t) { return getCounter(h, 0, t); }
+int ppsg_reset(ppsg_t* h) { return resetSpot(h, 0); }
+int ppsg_iterate(ppsg_t* h, int n) { return iterate(h, n); }
 
 int ppsg_half_step(ppsg_t* h, int tr) {
   if (!h) return fail("null handle");
   if (tr != 0 && tr != 1) return fail("half step: bad traverser");
-  if (scheduleCovers(h, 1) != 0) return -1;
+  if (h->prepare() != 0) return -1;
+  if (h->live.empty()) return fail("no live spot");
+  if (h->covers(1) != 0) return -1;
   if (h->launchHalf(tr, PPSG_MODE_CFR, h->stream) != 0) return -1;
   HIP_TRY(hipStreamSynchronize(h->stream));
   return 0;
 }
 
 int ppsg_walk(ppsg_t* h, int tr, int mode, float* root_cfv) {
-  if (!h) return fail("null handle");
-  if ((tr != 0 && tr != 1) || (mode != PPSG_MODE_EV && mode != PPSG_MODE_BR))
-    return fail("walk: bad traverser or mode");
-  if (h->launchHalf(tr, mode, h->stream) != 0) return -1;
-  HIP_TRY(hipStreamSynchronize(h->stream));
-  if (root_cfv)
-    HIP_TRY(hipMemcpy(root_cfv, h->dev.cfv, sizeof(float) * h->n[tr], hipMemcpyDeviceToHost));
-  return 0;
-}
-
-static float* bufOf(ppsg_t* h, int buf, int64_t& total) {
-  switch (buf) {
-    case PPSG_REGRET: total = h->regTotal; return h->dev.regret;
-    case PPSG_STRAT: total = h->regTotal; return h->dev.strat;
-    case PPSG_REACH0: total = h->reachTotal[0]; return h->dev.reach[0];
-    case PPSG_REACH1: total = h->reachTotal[1]; return h->dev.reach[1];
-    case PPSG_CFV: total = h->cfvTotal; return h->dev.cfv;
-  }
-  total = 0;
-  return nullptr;
+  const int slot = 0;
+  float* outs[1] = {root_cfv};
+  return walk(h, tr, mode, 1, &slot, outs);
 }
 
 int ppsg_read(ppsg_t* h, int buf, int64_t off, int64_t n, float* host) {
-  if (!h || !host) return fail("null argument");
-  int64_t total = 0;
-  float* p = bufOf(h, buf, total);
-  if (!p || off < 0 || n < 0 || off + n > total) return fail("read out of range");
-  HIP_TRY(hipStreamSynchronize(h->stream));
-  if (n) HIP_TRY(hipMemcpy(host, p + off, (size_t)n * 4, hipMemcpyDeviceToHost));
-  return 0;
+  return transfer(h, 0, buf, off, n, host, false);
 }
 
 int ppsg_write(ppsg_t* h, int buf, int64_t off, int64_t n, const float* host) {
-  if (!h || !host) return fail("null argument");
-  int64_t total = 0;
-  float* p = bufOf(h, buf, total);
-  if (!p || off < 0 || n < 0 || off + n > total) return fail("write out of range");
-  HIP_TRY(hipStreamSynchronize(h->stream));
-  if (n) HIP_TRY(hipMemcpy(p + off, host, (size_t)n * 4, hipMemcpyHostToDevice));
-  return 0;
+  return transfer(h, 0, buf, off, n, const_cast<float*>(host), true);
 }
 
 int ppsg_last_iterate_ms(ppsg_t* h, double* ms) {
@@ -1030,19 +1441,31 @@ int ppsg_last_iterate_ms(ppsg_t* h, double* ms) {
 
 int ppsg_info(ppsg_t* h, int64_t* out, int cap) {
   if (!h || !out) return fail("null argument");
-  const int64_t v[6] = {h->kernelsPerIter, h->bytes, h->itemsPerIter, h->exec ? 1 : 0,
-                        (int64_t)h->fwdBytes, (int64_t)h->bwdBytes};
+  if (h->prepare() != 0) return -1;
+  double fb = 0.0, bb = 0.0;
+  for (int s : h->live) {
+    fb += h->spots[s].fwdBytes;
+    bb += h->spots[s].bwdBytes;
+  }
+  const int64_t v[6] = {h->kernelsPerIter(), h->arena.cap, h->listedItems, h->exec ? 1 : 0,
+                        (int64_t)fb, (int64_t)bb};
   for (int i = 0; i < cap && i < 6; ++i) out[i] = v[i];
   return 0;
 }
 
 int ppsg_profile(ppsg_t* h, int iters, double* out, int cap) {
   if (!h || !out) return fail("null argument");
-  if (scheduleCovers(h, iters) != 0) return -1;
+  if (iters <= 0) return fail("profile: iters must be positive");
+  if (h->prepare() != 0) return -1;
+  if (h->live.empty()) return fail("no live spot");
+  if (h->covers(iters) != 0) return -1;
   double acc[13] = {0};
   hipEvent_t a = nullptr, b = nullptr;
   HIP_TRY(hipEventCreate(&a));
-  HIP_TRY(hipEventCreate(&b));
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return fail("profile: event creation failed");
+  }
   auto timed = [&](auto&& launch, int slot) -> int {
     HIP_TRY(hipEventRecord(a, h->stream));
     if (launch() != 0) return -1;
@@ -1054,36 +1477,124 @@ int ppsg_profile(ppsg_t* h, int iters, double* out, int cap) {
     acc[slot + 1] += 1;
     return 0;
   };
+  const bool detail = cap >= 13;
+  const int D = h->depthCount();
   int rc = 0;
   for (int it = 0; it < iters && rc == 0; ++it) {
     for (int tr = 0; tr < 2 && rc == 0; ++tr) {
-      for (int dpt = 0; dpt <= h->maxDepth && rc == 0; ++dpt) {
-        if (!h->fwdCount[tr][dpt]) continue;
+      for (int dpt = 0; dpt < D && rc == 0; ++dpt) {
+        if (!h->lists[tr][0][dpt].count) continue;
         rc = timed([&] { return h->launchFwd(tr, dpt, 0, -1, h->stream); }, 0);
-        // attribution: terminal kinds re-run alone (they only write cfv rows)
-        for (int k = 0; k < 2 && rc == 0 && cap >= 13; ++k)
+        // attribution: terminal kinds re-run alone (they only rewrite cfv rows)
+        for (int k = 0; k < 2 && rc == 0 && detail; ++k)
           rc = timed([&] { return h->launchFwd(tr, dpt, 0, 1 << (kBwdFold + k), h->stream); },
                      9 + 2 * k);
       }
-      for (int dpt = h->maxDepth; dpt >= 0 && rc == 0; --dpt) {
-        if (!h->bwdCount[tr][dpt]) continue;
+      for (int dpt = D - 1; dpt >= 0 && rc == 0; --dpt) {
+        if (!h->lists[tr][1][dpt].count) continue;
         rc = timed([&] { return h->launchBwd(tr, dpt, PPSG_MODE_CFR, -1, h->stream); }, 2);
-        // attribution: decide and chance tiles re-run alone
-        for (int k = 0; k < 2 && rc == 0 && cap >= 13; ++k)
-          rc = timed([&] { return h->launchBwd(tr, dpt, PPSG_MODE_CFR, 1 << (kBwdDecide + k),
-                                               h->stream); },
-                     5 + 2 * k);
+        for (int k = 0; k < 2 && rc == 0 && detail; ++k)
+          rc = timed(
+              [&] {
+                return h->launchBwd(tr, dpt, PPSG_MODE_CFR, 1 << (kBwdDecide + k), h->stream);
+              },
+              5 + 2 * k);
       }
     }
-    if (rc == 0) advanceCounter<<<1, 64, 0, h->stream>>>(h->ctl);
+    if (rc == 0) {
+      advanceCounters<<<1, kBlock, 0, h->stream>>>(h->ctl, h->liveDev, (int)h->live.size());
+      if (hipGetLastError() != hipSuccess) rc = fail("profile: counter launch failed");
+    }
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   if (rc != 0) return rc;
   HIP_TRY(hipStreamSynchronize(h->stream));
-  h->counter += iters;
+  for (int s : h->live) h->spots[s].counter += iters;
   acc[4] = iters;
   for (int i = 0; i < cap && i < 13; ++i) out[i] = acc[i];
+  return 0;
+}
+
+// ---- batches ----
+
+int ppsg_footprint(const ppsg_tree_desc* desc, int64_t* bytes) {
+  if (!desc || !bytes) return fail("null argument");
+  Spot S;
+  SpotTables T;
+  if (planSpot(desc, S, T) != 0) return -1;
+  *bytes = S.bytes;
+  return 0;
+}
+
+int ppsg_batch_create(int64_t arena_bytes, int max_spots, ppsg_t** out) {
+  if (!out) return fail("null argument");
+  *out = nullptr;
+  auto* E = new ppsg_engine();
+  if (E->init(arena_bytes, max_spots) != 0) {
+    delete E;
+    return -1;
+  }
+  *out = E;
+  return 0;
+}
+
+int ppsg_batch_add(ppsg_t* h, const ppsg_tree_desc* desc, int* slot) {
+  if (!h || !desc || !slot) return fail("null argument");
+  return h->add(desc, slot);
+}
+
+int ppsg_batch_remove(ppsg_t* h, int slot) {
+  if (!h) return fail("null handle");
+  return h->remove(slot);
+}
+
+int ppsg_batch_set_active(ppsg_t* h, int slot, int active) {
+  if (!h) return fail("null handle");
+  return h->setActive(slot, active != 0);
+}
+
+int ppsg_batch_set_schedule(ppsg_t* h, int slot, const float* coef, int64_t n) {
+  return setSchedule(h, slot, coef, n);
+}
+int ppsg_batch_set_counter(ppsg_t* h, int slot, int64_t t) { return setCounter(h, slot, t); }
+int ppsg_batch_get_counter(ppsg_t* h, int slot, int64_t* t) { return getCounter(h, slot, t); }
+int ppsg_batch_reset(ppsg_t* h, int slot) { return resetSpot(h, slot); }
+int ppsg_batch_step(ppsg_t* h, int n) { return iterate(h, n); }
+
+int ppsg_batch_walk(ppsg_t* h, int tr, int mode, int n_slots, const int* slots,
+                    float* const* root_cfv) {
+  return walk(h, tr, mode, n_slots, slots, root_cfv);
+}
+
+int ppsg_batch_read(ppsg_t* h, int slot, int buf, int64_t off, int64_t n, float* host) {
+  return transfer(h, slot, buf, off, n, host, false);
+}
+
+int ppsg_batch_write(ppsg_t* h, int slot, int buf, int64_t off, int64_t n, const float* host) {
+  return transfer(h, slot, buf, off, n, const_cast<float*>(host), true);
+}
+
+int ppsg_batch_free_bytes(ppsg_t* h, int64_t* total, int64_t* largest) {
+  if (!h) return fail("null handle");
+  if (total) *total = h->arena.freeBytes();
+  if (largest) *largest = h->arena.largestHole();
+  return 0;
+}
+
+int ppsg_batch_compact(ppsg_t* h, double max_dead_fraction, int* compacted) {
+  if (!h) return fail("null handle");
+  const bool go = h->deadItems > 0 && (double)h->deadItems > max_dead_fraction * (double)h->listedItems;
+  if (go && h->compact() != 0) return -1;
+  if (compacted) *compacted = go ? 1 : 0;
+  return 0;
+}
+
+int ppsg_device_memory(int64_t* free_bytes, int64_t* total_bytes) {
+  size_t f = 0, t = 0;
+  HIP_TRY(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = (int64_t)f;
+  if (total_bytes) *total_bytes = (int64_t)t;
   return 0;
 }
 

@@ -503,5 +503,26 @@ double pairMass(const CompiledTree& t) {
   return z;
 }
 
+// Range-weighted average action frequencies of a decide node from its
+// strategy-sum rows (action-major, nC combos): each combo's normalised sums
+// (uniform when empty) weighted by its range weight.
+std::vector<double> rangeFrequencies(const std::vector<float>& rows, const std::vector<double>& w,
+                                     int na, int nC) {
+  std::vector<double> freq(na, 0.0), tot(nC, 0.0);
+  double z = 0.0;
+  for (int c = 0; c < nC; ++c) {
+    for (int a = 0; a < na; ++a) tot[c] += rows[(size_t)a * nC + c];
+    z += w[c];
+  }
+  if (z <= 0.0) return freq;
+  for (int a = 0; a < na; ++a) {
+    double acc = 0.0;
+    for (int c = 0; c < nC; ++c)
+      acc += w[c] * (tot[c] > 0.0 ? rows[(size_t)a * nC + c] / tot[c] : 1.0 / na);
+    freq[a] = acc / z;
+  }
+  return freq;
+}
+
 }  // namespace gpu
 }  // namespace pps

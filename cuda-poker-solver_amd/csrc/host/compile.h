@@ -84,5 +84,10 @@ double pairMass(const CompiledTree& t);
 // (cuda/gpu_cfr.cu:585-614; "dcfr-g2" = paper DCFR, gamma 2, no restart).
 void discountCoefs(int t, int iters, const std::string& algo, double out[3]);
 
+// Range-weighted average action frequencies from a decide node's
+// strategy-sum rows (action-major over nC combos; w = range weights).
+std::vector<double> rangeFrequencies(const std::vector<float>& rows, const std::vector<double>& w,
+                                     int na, int nC);
+
 }  // namespace gpu
 }  // namespace pps

@@ -3,6 +3,7 @@
 // leaves the message in a thread-local buffer.
 #include "pps_capi.h"
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -10,6 +11,7 @@
 #include <vector>
 
 #include "host/compile.h"
+#include "pps/batch.h"
 #include "pps/cards.h"
 #include "pps/gpu_cfr.h"
 #include "pps/postflop_cfr.h"
@@ -337,6 +339,110 @@ int pps_solver_engine_info(pps_solver* s, int64_t out6[6]) {
   return guard([&] {
     const std::vector<int64_t> v = S(s).engineInfo();
     for (size_t i = 0; i < 6; ++i) out6[i] = v[i];
+    return 0;
+  });
+}
+
+void pps_serve_defaults(pps_serve_options* opt) {
+  if (!opt) return;
+  const pps::gpu::ServeOptions d;
+  opt->target_frac = d.targetFrac;
+  opt->check_every = d.checkEvery;
+  opt->max_iters = d.maxIters;
+  opt->arena_bytes = d.arenaBytes;
+  opt->max_spots = d.maxSpots;
+  opt->max_live = d.maxLive;
+  opt->compile_threads = d.compileThreads;
+  opt->real_time = d.realTime ? 1 : 0;
+  opt->prefill = d.prefill ? 1 : 0;
+}
+
+int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const double* arrival_ms,
+              const char* algo, const pps_serve_options* opt, pps_serve_result* out,
+              pps_serve_report* report) {
+  return guard([&] {
+    if (n < 0 || (n > 0 && (!spots || !cfgs || !out))) throw std::runtime_error("null argument");
+    pps::gpu::ServeOptions o;
+    if (opt) {
+      o.targetFrac = opt->target_frac;
+      o.checkEvery = opt->check_every;
+      o.maxIters = opt->max_iters;
+      o.arenaBytes = opt->arena_bytes;
+      o.maxSpots = opt->max_spots;
+      o.maxLive = opt->max_live;
+      o.compileThreads = opt->compile_threads;
+      o.realTime = opt->real_time != 0;
+      o.prefill = opt->prefill != 0;
+    }
+    if (algo) o.algo = algo;
+    std::vector<pps::gpu::SpotRequest> reqs(n);
+    for (int i = 0; i < n; ++i) {
+      reqs[i].spot = toSpot(spots + i);
+      reqs[i].cfg = toCfg(cfgs + i);
+      reqs[i].arrivalMs = arrival_ms ? arrival_ms[i] : 0.0;
+    }
+    pps::gpu::ServeReport rep;
+    const std::vector<pps::gpu::SpotResult> res = pps::gpu::serve(reqs, o, &rep);
+    for (int i = 0; i < n; ++i) {
+      const pps::gpu::SpotResult& r = res[i];
+      pps_serve_result& d = out[i];
+      std::memset(&d, 0, sizeof(d));
+      d.status = r.ok ? 0 : -1;
+      d.converged = r.converged ? 1 : 0;
+      d.iterations = r.iterations;
+      d.n_root_actions = (int32_t)std::min<size_t>(r.rootStrategy.size(), 32);
+      d.expl = r.expl;
+      d.ev0 = r.ev0;
+      d.ev1 = r.ev1;
+      for (int a = 0; a < d.n_root_actions; ++a) d.root_strategy[a] = r.rootStrategy[a];
+      d.nodes = r.numNodes;
+      d.footprint = r.footprint;
+      d.arrival_ms = r.arrivalMs;
+      d.admit_ms = r.admitMs;
+      d.retire_ms = r.retireMs;
+      std::strncpy(d.error, r.error.c_str(), sizeof(d.error) - 1);
+    }
+    if (report) {
+      report->wall_ms = rep.wallMs;
+      report->busy_ms = rep.busyMs;
+      report->device_ms = rep.deviceMs;
+      report->walk_ms = rep.walkMs;
+      report->admit_ms = rep.admitMs;
+      report->iterations = rep.iterations;
+      report->steps = rep.steps;
+      report->peak_live = rep.peakLive;
+      report->compactions = rep.compactions;
+    }
+    return 0;
+  });
+}
+
+int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, const char* algo,
+                      int iters, int64_t arena_bytes, double out[12]) {
+  return guard([&] {
+    if (n <= 0 || !spots || !cfgs || !out) throw std::runtime_error("null argument");
+    std::vector<pps::gpu::SpotRequest> reqs(n);
+    for (int i = 0; i < n; ++i) {
+      reqs[i].spot = toSpot(spots + i);
+      reqs[i].cfg = toCfg(cfgs + i);
+    }
+    const pps::gpu::BatchProfile P =
+        pps::gpu::profileBatch(reqs, algo ? algo : "dcfr", iters, arena_bytes);
+    const double v[12] = {(double)P.spots, (double)P.iters, P.graphMsPerIter, P.fwdMs,
+                          P.fwdLaunches, P.bwdMs, P.bwdLaunches, P.fwdBytes, P.bwdBytes,
+                          (double)P.kernelsPerIter, (double)P.itemsPerIter, (double)P.arenaBytes};
+    for (int i = 0; i < 12; ++i) out[i] = v[i];
+    return 0;
+  });
+}
+
+int pps_compile_footprint(const pps_spot* spot, const pps_bet_config* cfg, int64_t* bytes) {
+  return guard([&] {
+    if (!bytes) throw std::runtime_error("null argument");
+    const pps::gpu::CompiledTree t = pps::gpu::compileSpot(toSpot(spot), toCfg(cfg));
+    if (t.empty) throw std::runtime_error("empty ranges");
+    const ppsg_tree_desc d = t.desc();
+    if (ppsg_footprint(&d, bytes) != 0) throw std::runtime_error(ppsg_last_error());
     return 0;
   });
 }

@@ -266,41 +266,20 @@ pf::NodeStats GpuPostflopSolver::stats() {
   return st;
 }
 
-namespace {
-// Range-weighted average frequencies of a first-street decide node.
-std::vector<double> aggregateFreqs(const std::vector<float>& rows, const std::vector<double>& w,
-                                   int na, int nC) {
-  std::vector<double> freq(na, 0.0), tot(nC, 0.0);
-  double z = 0.0;
-  for (int c = 0; c < nC; ++c) {
-    for (int a = 0; a < na; ++a) tot[c] += rows[(size_t)a * nC + c];
-    z += w[c];
-  }
-  if (z <= 0.0) return freq;
-  for (int a = 0; a < na; ++a) {
-    double acc = 0.0;
-    for (int c = 0; c < nC; ++c)
-      acc += w[c] * (tot[c] > 0.0 ? rows[(size_t)a * nC + c] / tot[c] : 1.0 / na);
-    freq[a] = acc / z;
-  }
-  return freq;
-}
-}  // namespace
-
 std::vector<double> GpuPostflopSolver::nodeStrategy(int decideIdx) {
   Impl* I = impl_;
   if (I->T.empty || decideIdx < 0 || decideIdx >= (int)I->T.decideNodes.size()) return {};
   const int u = I->T.decideNodes[decideIdx];
   if (I->T.clsNBoard[I->T.cls[u]] != I->T.spot.nBoard) return {};
   const int p = I->T.owner[u], na = I->T.na[u], nC = I->T.listSize(p, u);
-  return aggregateFreqs(I->read(PPSG_STRAT, I->T.regOff[u], (int64_t)na * nC), I->T.w[p], na, nC);
+  return rangeFrequencies(I->read(PPSG_STRAT, I->T.regOff[u], (int64_t)na * nC), I->T.w[p], na, nC);
 }
 
 std::vector<double> GpuPostflopSolver::rootStrategy() {
   Impl* I = impl_;
   if (I->T.empty) return {};
   const int na = I->T.na[0], nC = I->T.listSize(0, 0);
-  return aggregateFreqs(I->read(PPSG_STRAT, I->T.regOff[0], (int64_t)na * nC), I->T.w[0], na, nC);
+  return rangeFrequencies(I->read(PPSG_STRAT, I->T.regOff[0], (int64_t)na * nC), I->T.w[0], na, nC);
 }
 
 int GpuPostflopSolver::numDecideNodes() const { return (int)impl_->T.decideNodes.size(); }

@@ -26,6 +26,32 @@ class BetConfig(ctypes.Structure):
                 ("force_allin", ctypes.c_double), ("merging", ctypes.c_double)]
 
 
+class ServeOptions(ctypes.Structure):
+    _fields_ = [("target_frac", ctypes.c_double), ("check_every", ctypes.c_int32),
+                ("max_iters", ctypes.c_int32), ("arena_bytes", ctypes.c_int64),
+                ("max_spots", ctypes.c_int32), ("max_live", ctypes.c_int32),
+                ("compile_threads", ctypes.c_int32), ("real_time", ctypes.c_int32),
+                ("prefill", ctypes.c_int32)]
+
+
+class ServeResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("converged", ctypes.c_int32),
+                ("iterations", ctypes.c_int32), ("n_root_actions", ctypes.c_int32),
+                ("expl", ctypes.c_double), ("ev0", ctypes.c_double), ("ev1", ctypes.c_double),
+                ("root_strategy", ctypes.c_double * 32), ("nodes", ctypes.c_int64),
+                ("footprint", ctypes.c_int64), ("arrival_ms", ctypes.c_double),
+                ("admit_ms", ctypes.c_double), ("retire_ms", ctypes.c_double),
+                ("error", ctypes.c_char * 96)]
+
+
+class ServeReport(ctypes.Structure):
+    _fields_ = [("wall_ms", ctypes.c_double), ("busy_ms", ctypes.c_double),
+                ("device_ms", ctypes.c_double), ("walk_ms", ctypes.c_double),
+                ("admit_ms", ctypes.c_double), ("iterations", ctypes.c_int64),
+                ("steps", ctypes.c_int32), ("peak_live", ctypes.c_int32),
+                ("compactions", ctypes.c_int32)]
+
+
 # every function include/pps_capi.h and include/ppsg.h declare
 CAPI_SYMBOLS = [
     "pps_last_error", "pps_hand_key", "pps_parse_range", "pps_bet_actions", "pps_schedule",
@@ -37,12 +63,18 @@ CAPI_SYMBOLS = [
     "pps_solver_num_base_combos", "pps_solver_save", "pps_solver_read_rows",
     "pps_solver_write_rows", "pps_solver_half_step", "pps_solver_root_value",
     "pps_solver_iteration_bytes", "pps_solver_last_solve_ms", "pps_solver_profile",
-    "pps_solver_engine_info",
+    "pps_solver_engine_info", "pps_serve_defaults", "pps_serve", "pps_compile_footprint",
+    "pps_batch_profile",
 ]
 PPSG_SYMBOLS = [
     "ppsg_create", "ppsg_destroy", "ppsg_set_schedule", "ppsg_set_counter", "ppsg_get_counter",
     "ppsg_reset", "ppsg_iterate", "ppsg_half_step", "ppsg_walk", "ppsg_read", "ppsg_write",
     "ppsg_last_iterate_ms", "ppsg_info", "ppsg_profile", "ppsg_copy_probe", "ppsg_last_error",
+    "ppsg_footprint", "ppsg_batch_create", "ppsg_batch_add", "ppsg_batch_remove",
+    "ppsg_batch_set_active", "ppsg_batch_set_schedule", "ppsg_batch_set_counter",
+    "ppsg_batch_get_counter", "ppsg_batch_reset", "ppsg_batch_step", "ppsg_batch_walk",
+    "ppsg_batch_read", "ppsg_batch_write", "ppsg_batch_free_bytes", "ppsg_batch_compact",
+    "ppsg_device_memory",
 ]
 
 _lib = None
@@ -100,6 +132,13 @@ def lib():
         L.pps_solver_engine_info.argtypes = [ctypes.c_void_p, P(ctypes.c_int64)]
         L.pps_solver_last_solve_ms.restype = ctypes.c_double
         L.pps_solver_last_solve_ms.argtypes = [ctypes.c_void_p]
+        L.pps_serve_defaults.restype = None
+        L.pps_serve_defaults.argtypes = [P(ServeOptions)]
+        L.pps_serve.argtypes = [ctypes.c_int, P(Spot), P(BetConfig), P(ctypes.c_double),
+                                ctypes.c_char_p, P(ServeOptions), P(ServeResult), P(ServeReport)]
+        L.pps_compile_footprint.argtypes = [P(Spot), P(BetConfig), P(ctypes.c_int64)]
+        L.pps_batch_profile.argtypes = [ctypes.c_int, P(Spot), P(BetConfig), ctypes.c_char_p,
+                                        ctypes.c_int, ctypes.c_int64, P(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -312,3 +351,60 @@ class Solver:
     @property
     def last_solve_ms(self):
         return float(lib().pps_solver_last_solve_ms(self.h))
+
+
+def compile_footprint(spot, cfg):
+    """Device bytes the compiled spot takes in a batch arena (host only)."""
+    b = ctypes.c_int64()
+    _check(lib().pps_compile_footprint(ctypes.byref(spot), ctypes.byref(cfg), ctypes.byref(b)))
+    return b.value
+
+
+def serve_defaults():
+    o = ServeOptions()
+    lib().pps_serve_defaults(ctypes.byref(o))
+    return o
+
+
+def serve(spots, cfgs, algo="dcfr", arrival_ms=None, **options):
+    """Continuous-batching solve of many spots (pps_serve). spots / cfgs:
+    lists of Spot / BetConfig; options: ServeOptions field names. Returns
+    (results, report) as dicts, results in request order."""
+    n = len(spots)
+    if len(cfgs) != n:
+        raise ValueError("one bet config per spot")
+    o = serve_defaults()
+    for k, v in options.items():
+        if not hasattr(o, k):
+            raise ValueError(f"unknown serve option {k}")
+        setattr(o, k, v)
+    S = (Spot * n)(*spots)
+    C = (BetConfig * n)(*cfgs)
+    A = (ctypes.c_double * n)(*arrival_ms) if arrival_ms is not None else None
+    R = (ServeResult * n)()
+    rep = ServeReport()
+    _check(lib().pps_serve(n, S, C, A, algo.encode(), ctypes.byref(o), R, ctypes.byref(rep)))
+    out = []
+    for r in R:
+        out.append({
+            "ok": r.status == 0, "error": r.error.decode(), "converged": bool(r.converged),
+            "iterations": r.iterations, "expl": r.expl, "ev0": r.ev0, "ev1": r.ev1,
+            "root_strategy": [r.root_strategy[a] for a in range(r.n_root_actions)],
+            "nodes": r.nodes, "footprint": r.footprint, "arrival_ms": r.arrival_ms,
+            "admit_ms": r.admit_ms, "retire_ms": r.retire_ms,
+        })
+    report = {k: getattr(rep, k) for k, _ in ServeReport._fields_}
+    return out, report
+
+
+def batch_profile(spots, cfgs, iters, algo="dcfr", arena_bytes=0):
+    """Launch-level timing of the spots resident in one arena (pps_batch_profile)."""
+    n = len(spots)
+    S = (Spot * n)(*spots)
+    C = (BetConfig * n)(*cfgs)
+    out = (ctypes.c_double * 12)()
+    _check(lib().pps_batch_profile(n, S, C, algo.encode(), int(iters), int(arena_bytes), out))
+    keys = ["spots", "iters", "graph_ms_per_iter", "fwd_ms", "fwd_launches", "bwd_ms",
+            "bwd_launches", "fwd_bytes", "bwd_bytes", "kernels_per_iter", "items_per_iter",
+            "arena_bytes"]
+    return dict(zip(keys, list(out)))

@@ -9,6 +9,8 @@ keep round(fraction * N)) and one weight in (0, 1] per kept combo in combo
 order.
 """
 
+import math
+
 MASK = (1 << 64) - 1
 
 
@@ -113,6 +115,65 @@ def river_spot(seed):
     s = random_spot(seed, 5, frac, pot, stack, sizes, [2.5], True, cap)
     s.update(name=f"river-{seed}", algo="dcfr")
     return s
+
+
+def _nodes(spot):
+    from . import capi
+    sp = capi.make_spot(spot["board"], spot["w0"], spot["w1"], spot["pot"], spot["stack"])
+    cf = capi.make_config(spot["bets"], spot["raises"], spot["allin"], False, spot["max_raises"])
+    return capi.compile_info(sp, cf)["nodes"]
+
+
+RESAMPLE_STEP = 100003
+
+
+def _sized(make, seed, lo, hi):
+    """make(seed) for the first of seed, seed + RESAMPLE_STEP, ... whose tree
+    has lo..hi nodes (the size filter of the batched and served streams)."""
+    for k in range(64):
+        s = make(seed + k * RESAMPLE_STEP)
+        n = _nodes(s)
+        if lo <= n <= hi:
+            s["nodes"] = n
+            return s
+    raise RuntimeError(f"no spot with {lo}..{hi} nodes from seed {seed}")
+
+
+def _parallel(fn, args, threads=16):
+    # compile_info runs in the native library, which releases the GIL
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(fn, args))
+
+
+def batch_stream(n=256, first_seed=1000):
+    """The batched-turn workload: batch_turn spots from seeds first_seed ..
+    first_seed + n - 1, each resampled until its tree has 2k-60k nodes."""
+    return _parallel(lambda sd: _sized(batch_turn, sd, 2000, 60000),
+                     range(first_seed, first_seed + n))
+
+
+def serving_stream(n, qps, seed=7):
+    """The serving workload: n requests with Poisson arrivals at `qps` per
+    second (exponential gaps from one seeded stream), 70% batched-turn
+    spots (2k-60k nodes) and 30% river spots. River trees with these bet
+    menus have 9-105 nodes, so they are taken as drawn (no size filter).
+    Each spot dict gains "arrival_ms"."""
+    rng = XorShift64Star(seed * 1000003 + 11)
+    plan, t = [], 0.0
+    for i in range(n):
+        t += -math.log(1.0 - rng.next_double()) / qps * 1e3
+        plan.append((t, rng.next_double() < 0.7, seed * 100000 + i))
+    def make(p):
+        t_ms, turn, sd = p
+        if turn:
+            s = _sized(batch_turn, sd, 2000, 60000)
+        else:
+            s = river_spot(sd)
+            s["nodes"] = _nodes(s)
+        s["arrival_ms"] = t_ms
+        return s
+    return _parallel(make, plan)
 
 
 def large_flop():

@@ -107,6 +107,65 @@ int pps_solver_profile(pps_solver* s, int iters, int detail, double* out);
 int pps_solver_engine_info(pps_solver* s, int64_t out6[6]);
 double pps_solver_last_solve_ms(pps_solver* s);
 
+/* ---- many spots on one GPU (no reference counterpart) ----
+   Continuous batching: every admitted spot stays resident in one device
+   arena and all live spots advance together, `check_every` iterations per
+   step, with one batched exploitability check after each step. A spot
+   retires at expl <= target_frac * pot or at max_iters; requests are
+   admitted in arrival order as slots and arena memory free up. A spot's
+   result does not depend on the rest of the batch and equals
+   pps_solver_solve(iterations, algo) of the same spot bit for bit.
+   (The reference serves one solve at a time: python/pps_api.py.) */
+typedef struct pps_serve_options {
+  double target_frac;       /* 0.005 */
+  int32_t check_every;      /* 32 */
+  int32_t max_iters;        /* 4096 (rounded up to check_every) */
+  int64_t arena_bytes;      /* 0: 80% of free device memory */
+  int32_t max_spots;        /* resident slots, 1024 */
+  int32_t max_live;         /* 0: memory and slots only; 1: one at a time */
+  int32_t compile_threads;  /* 8 */
+  int32_t real_time;        /* 1: honour arrival_ms; 0: all arrive at once */
+  int32_t prefill;          /* 1: admit all that fit before the first step */
+} pps_serve_options;
+
+typedef struct pps_serve_result {
+  int32_t status;           /* 0 solved, -1 rejected (see error) */
+  int32_t converged;        /* reached the target before max_iters */
+  int32_t iterations;
+  int32_t n_root_actions;
+  double expl, ev0, ev1;    /* as pps_solver_stats */
+  double root_strategy[32]; /* as pps_solver_root_strategy */
+  int64_t nodes, footprint; /* compiled nodes, arena bytes */
+  double arrival_ms, admit_ms, retire_ms;  /* stream clock */
+  char error[96];
+} pps_serve_result;
+
+typedef struct pps_serve_report {
+  double wall_ms;           /* stream start to last retirement */
+  double busy_ms;           /* steps + checks + retirements: spots resident */
+  double device_ms, walk_ms, admit_ms;
+  int64_t iterations;       /* sum of per-spot iterations */
+  int32_t steps, peak_live, compactions;
+} pps_serve_report;
+
+/* Fills options with the defaults above. */
+void pps_serve_defaults(pps_serve_options* opt);
+/* Solves n spots (all with bet config cfgs[i], schedule `algo`: "dcfr" or
+   "dcfr-g2"); arrival_ms may be NULL (all at 0). out[n]; report may be
+   NULL. Returns 0, or -1 on an engine error (pps_last_error). */
+int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const double* arrival_ms,
+              const char* algo, const pps_serve_options* opt, pps_serve_result* out,
+              pps_serve_report* report);
+/* Launch-level timing of n spots resident in one arena (all admitted at
+   once): out = [spots, iters, graph ms per batched iteration, forward ms,
+   forward launches, backward ms, backward launches, forward bytes and
+   backward bytes per batched iteration (algorithmic f32 rows), kernels
+   per iteration, work items per iteration, arena bytes]. */
+int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, const char* algo,
+                      int iters, int64_t arena_bytes, double out[12]);
+/* Device bytes the compiled spot takes in an arena (host only). */
+int pps_compile_footprint(const pps_spot* spot, const pps_bet_config* cfg, int64_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -122,6 +122,54 @@ int ppsg_profile(ppsg_t* h, int iters, double* out, int cap);
 int ppsg_copy_probe(int64_t bytes, int reps, double* ms_per_launch);
 const char* ppsg_last_error(void);
 
+/* ---- batches: many resident spots in one device arena ----
+ * No reference counterpart: the reference solves one spot per solver
+ * object (cuda/gpu_cfr.h:141-264), one CUDA graph each. A batch handle
+ * is an engine whose arena holds any number of compiled spots; every
+ * step runs one DCFR iteration of every ACTIVE spot, depth d of all of
+ * them in the same launch, so the launches per iteration do not grow
+ * with the number of spots. Each spot keeps its own schedule table and
+ * counter, so a spot's rows evolve exactly as they would alone (sums
+ * have a fixed order: results are bitwise independent of the batch).
+ * ppsg_create is a batch of one spot in slot 0; the single-spot calls
+ * above act on slot 0 of such a handle (ppsg_iterate, ppsg_profile and
+ * ppsg_info cover every active spot of any handle). */
+/* Device bytes one spot of this description takes in an arena. */
+int ppsg_footprint(const ppsg_tree_desc* desc, int64_t* bytes);
+/* Allocates an arena of arena_bytes and room for max_spots spots. */
+int ppsg_batch_create(int64_t arena_bytes, int max_spots, ppsg_t** out);
+/* Uploads a spot (active, zero rows, counter 0, no schedule yet); returns
+   -2 when no slot or no large enough arena block is free, even after
+   compacting (nothing changed). */
+int ppsg_batch_add(ppsg_t* h, const ppsg_tree_desc* desc, int* slot);
+/* Retires a spot at once; its slot and arena block are reused after the
+   next compaction (immediately when its items are not listed). */
+int ppsg_batch_remove(ppsg_t* h, int slot);
+/* Inactive (parked) spots stay resident but are skipped by steps and
+   walks; removal and parking take effect at once (the spot's work items
+   are masked until the engine next compacts its work lists). */
+int ppsg_batch_set_active(ppsg_t* h, int slot, int active);
+int ppsg_batch_set_schedule(ppsg_t* h, int slot, const float* coef, int64_t n);
+int ppsg_batch_set_counter(ppsg_t* h, int slot, int64_t t);
+int ppsg_batch_get_counter(ppsg_t* h, int slot, int64_t* t);
+int ppsg_batch_reset(ppsg_t* h, int slot);
+/* n DCFR iterations of every active spot (one graph replay each). */
+int ppsg_batch_step(ppsg_t* h, int n);
+/* EV / BR half-step of every active spot; copies the root cfv row of each
+   listed (active) spot to root_cfv[i] (n_base[tr] floats; NULL skips). */
+int ppsg_batch_walk(ppsg_t* h, int tr, int mode, int n_slots, const int* slots,
+                    float* const* root_cfv);
+int ppsg_batch_read(ppsg_t* h, int slot, int buf, int64_t off, int64_t n, float* host);
+int ppsg_batch_write(ppsg_t* h, int slot, int buf, int64_t off, int64_t n, const float* host);
+/* Free arena bytes in total and in the largest block (either may be NULL). */
+int ppsg_batch_free_bytes(ppsg_t* h, int64_t* total, int64_t* largest);
+/* Drops retired spots' masked items from the work lists (and frees their
+   slots and arena blocks) when they exceed max_dead_fraction of all listed
+   items; *compacted = 1 if it did (may be NULL). */
+int ppsg_batch_compact(ppsg_t* h, double max_dead_fraction, int* compacted);
+/* Free and total memory of the current device (for sizing an arena). */
+int ppsg_device_memory(int64_t* free_bytes, int64_t* total_bytes);
+
 #ifdef __cplusplus
 }
 #endif

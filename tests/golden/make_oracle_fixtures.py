@@ -1,8 +1,12 @@
-"""Writes tests/golden/oracle_runs.json: the CPU oracle's solve results on the
-full-size measured spots (plain turn and large turn, 500 iterations), so the
-GPU parity gates need no long CPU run on the GPU host.
+"""Writes the CPU oracle's solve results that the GPU parity gates compare
+against, so no long CPU run is needed on the GPU host:
 
-    python tests/golden/make_oracle_fixtures.py
+  oracle_runs.json   the plain turn and large turn spots, 500 iterations
+  oracle_batch.json  spots of the batched-turn and serving streams (small
+                     trees), 500 iterations each, for the per-spot gates
+                     of the batched engine
+
+    python tests/golden/make_oracle_fixtures.py [runs] [batch]
 """
 import json
 import os
@@ -31,12 +35,32 @@ def run(spot, iters):
             "cpu_seconds": dt, "threads": os.cpu_count()}
 
 
-def main():
-    out = {"plain-turn": run(configs.plain_turn(), 500), "large-turn": run(configs.large_turn(), 500)}
-    with open(os.path.join(HERE, "oracle_runs.json"), "w") as f:
-        json.dump(out, f, indent=1)
-    print(json.dumps(out, indent=1))
+# (kind, seed): small trees from the batched-turn stream and river spots
+BATCH_SPOTS = [("turn", 1001), ("turn", 1013), ("turn", 1002), ("river", 3000), ("river", 3007)]
+
+
+def batch_spot(kind, seed):
+    return configs.batch_turn(seed) if kind == "turn" else configs.river_spot(seed)
+
+
+def main(which):
+    if "runs" in which:
+        out = {"plain-turn": run(configs.plain_turn(), 500),
+               "large-turn": run(configs.large_turn(), 500)}
+        with open(os.path.join(HERE, "oracle_runs.json"), "w") as f:
+            json.dump(out, f, indent=1)
+        print(json.dumps(out, indent=1))
+    if "batch" in which:
+        out = []
+        for kind, seed in BATCH_SPOTS:
+            r = run(batch_spot(kind, seed), 500)
+            r.update(kind=kind, seed=seed)
+            del r["node1_strategy"]
+            out.append(r)
+            print(json.dumps(r), flush=True)
+        with open(os.path.join(HERE, "oracle_batch.json"), "w") as f:
+            json.dump(out, f, indent=1)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:] or ["runs", "batch"])

@@ -113,3 +113,47 @@ def test_errors_are_reported_not_raised_across_the_abi():
     sp = capi.make_spot(board_of("Qs9h2d7c"), np.zeros(1326), np.ones(1326), 100, 100)
     with pytest.raises(capi.CapiError, match="empty"):
         capi.compile_info(sp, capi.make_config([0.5]))
+
+
+def test_arena_footprint_covers_the_rows():
+    # host-only planning of a spot's arena block: at least the f32 rows
+    # (regret + strategy, two reach layouts, cfv) and grows with the tree
+    small, large = configs.batch_turn(1001), configs.batch_turn(1006)
+    fp = []
+    for s in (small, large):
+        sp = capi.make_spot(s["board"], s["w0"], s["w1"], s["pot"], s["stack"])
+        cf = capi.make_config(s["bets"], s["raises"], s["allin"], False, s["max_raises"])
+        info = capi.compile_info(sp, cf)
+        b = capi.compile_footprint(sp, cf)
+        assert b >= 2 * 4 * info["reg_total"]
+        assert b % 256 == 0
+        fp.append(b)
+    assert fp[1] > fp[0]
+
+
+def test_serve_rejects_bad_options_before_touching_the_device():
+    s = configs.river_spot(3000)
+    sp = capi.make_spot(s["board"], s["w0"], s["w1"], s["pot"], s["stack"])
+    cf = capi.make_config(s["bets"], s["raises"], s["allin"], False, s["max_raises"])
+    with pytest.raises(capi.CapiError, match="algo"):
+        capi.serve([sp], [cf], "hs30")
+    with pytest.raises(capi.CapiError, match="bad options"):
+        capi.serve([sp], [cf], "dcfr", check_every=0)
+    with pytest.raises(ValueError):
+        capi.serve([sp], [cf], "dcfr", no_such_option=1)
+    assert capi.serve([], [], "dcfr")[0] == []
+
+
+def test_workload_streams_are_seeded_and_size_filtered():
+    a = configs.batch_stream(12, first_seed=1000)
+    b = configs.batch_stream(12, first_seed=1000)
+    assert [x["nodes"] for x in a] == [x["nodes"] for x in b]
+    assert all(2000 <= x["nodes"] <= 60000 for x in a)
+    # seed 1003 compiles to 537 nodes: resampled at 1003 + RESAMPLE_STEP
+    assert a[3]["name"] == f"batch-turn-{1003 + configs.RESAMPLE_STEP}"
+    s = configs.serving_stream(40, qps=10.0, seed=7)
+    t = [x["arrival_ms"] for x in s]
+    assert t == sorted(t) and t[0] > 0
+    rivers = [x for x in s if len(x["board"]) == 5]
+    assert 0 < len(rivers) < len(s)
+    assert all(2000 <= x["nodes"] <= 60000 for x in s if len(x["board"]) == 4)
