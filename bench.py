@@ -1,21 +1,33 @@
 #!/usr/bin/env python3
-"""DCFR iterations/s of the MI355X solver on one GPU.
+"""Aggregate DCFR iters/s on one MI355X: batched vs one spot at a time, and
+served-stream latency, every spot solved to exploitability <= 0.5% of pot.
 
-Workload: the large turn spot (seeded 4-card board, 60% ranges with weights
-in (0, 1], pot 60, stack 240, bets 33/75/150% + all-in, raises 2.5x/4x,
-capped at 3 bets; reference DCFR schedule, f32 rows). A step is one DCFR
-iteration (both alternating half-steps) over the whole tree, replayed from
-the captured hipGraph; the compiled tree and all rows are resident in HBM
-before the timed region starts.
+Default workload (batched turn): 256 seeded turn spots (seeds 1000..1255,
+pot 20-200, SPR 0.5-8, 1-3 sizes from {33,50,75,100,150}% + all-in, raise
+cap 1-3, 30-100% ranges with weights in (0, 1], trees of 2k-60k nodes;
+reference DCFR schedule, f32 rows). A step is one DCFR iteration (both
+alternating half-steps) of one spot; spots are checked every 32 iterations
+and retire at exploitability <= 0.005 * pot.
 
-Also reported, in the same JSON line:
-  roofline      the dominant pass (forward or backward launches), timed
-                launch by launch with HIP events on the engine's stream:
-                algorithmic f32 row bytes per launch / mean launch time,
-                against MI355X's 8 TB/s HBM peak
-  cpu_baseline  the CPU oracle (tests' checker, a C++ port of the same
-                algorithm) on a bounded sample of the same workload, on this
-                host's cores
+  value           batched: all 256 spots resident in one arena, advanced
+                  together (depth d of every live spot in one launch);
+                  sum of per-spot iterations / wall time of the steps,
+                  checks and retirements (inputs already in HBM)
+  single          the same spots one at a time through the same engine
+                  (a batch of one), same definition of time
+  serving         a Poisson stream (1000 requests at 40/s, 70% turn spots as
+                  above, 30% river spots): p50/p99 arrival-to-retirement
+                  latency and aggregate iters/s over the stream's wall time
+  roofline        the dominant pass of the resident 256-spot batch, timed
+                  launch by launch with HIP events on the engine's stream:
+                  algorithmic f32 row bytes per launch / mean launch time,
+                  against MI355X's 8 TB/s HBM peak
+  cpu_baseline    the CPU oracle (the tests' checker, a C++ OpenMP port of
+                  the same algorithm) on the first spots of the stream, each
+                  run for the iteration count the GPU needed, ~15 s
+
+--workload large-turn / plain-turn measures one spot alone (2000 / 500
+iterations from the captured graph) instead.
 """
 import argparse
 import json
@@ -28,21 +40,54 @@ sys.path.insert(0, os.path.join(ROOT, "cuda-poker-solver_amd"))
 
 METRIC = "aggregate DCFR iters/s + p50/p99 solve latency @ expl≤0.5% pot, batched vs single"
 HBM_PEAK_GBS = 8000.0
+TARGET = 0.005
+CHECK_EVERY = 32
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(spot, seconds):
-    """Times the CPU oracle on the same spot for about `seconds`."""
+def spot_cfg(capi, s):
+    return (capi.make_spot(s["board"], s["w0"], s["w1"], s["pot"], s["stack"]),
+            capi.make_config(s["bets"], s["raises"], s["allin"], False, s["max_raises"]))
+
+
+def oracle_of(s, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc  # checker only: the timed CPU leg
+    return orc.Oracle(orc.make_spot(s["board"], s["w0"], s["w1"], s["pot"], s["stack"],
+                                    s["bets"], s["raises"], s["allin"], False, s["max_raises"]),
+                      threads=threads)
 
-    threads = max(1, min(16, os.cpu_count() or 1))
-    osp = orc.make_spot(spot["board"], spot["w0"], spot["w1"], spot["pot"], spot["stack"],
-                        spot["bets"], spot["raises"], spot["allin"], False, spot["max_raises"])
-    o = orc.Oracle(osp, threads=threads)
+
+def cpu_threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_baseline_spots(spots, iters, seconds):
+    """The CPU oracle on the first spots of the stream, each for the
+    iteration count the GPU needed, until about `seconds` have passed."""
+    threads = cpu_threads()
+    done, total, n = 0, 0.0, 0
+    for s, it in zip(spots, iters):
+        o = oracle_of(s, threads)
+        t0 = time.perf_counter()
+        o.solve(it, s["algo"])
+        total += time.perf_counter() - t0
+        done += it
+        n += 1
+        if total >= seconds:
+            break
+    return {"value": round(done / total, 3), "unit": "iters/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} spots of the batched-turn stream, {done} DCFR iterations in "
+                      f"total (each spot's GPU iteration count), C++ CPU oracle with OpenMP over "
+                      f"river deals, {total:.1f} s"}
+
+
+def cpu_baseline_single(spot, seconds):
+    threads = cpu_threads()
+    o = oracle_of(spot, threads)
     t0 = time.perf_counter()
     o.solve(2, spot["algo"])
     per = max((time.perf_counter() - t0) / 2, 1e-4)
@@ -55,15 +100,196 @@ def cpu_baseline(spot, seconds):
                       f"(C++ CPU oracle, OpenMP over river deals), {dt:.1f} s"}
 
 
+def roofline_of(prof, iters_key="iters"):
+    """Dominant pass: algorithmic bytes per launch / mean launch time."""
+    it = prof[iters_key]
+    fwd_per_iter, bwd_per_iter = prof["fwd_launches"] / it, prof["bwd_launches"] / it
+    fwd_launch_ms = prof["fwd_ms"] / max(prof["fwd_launches"], 1)
+    bwd_launch_ms = prof["bwd_ms"] / max(prof["bwd_launches"], 1)
+    if prof["fwd_ms"] >= prof["bwd_ms"]:
+        kname, nbytes, lms = "fwdKernel", prof["fwd_bytes"] / fwd_per_iter, fwd_launch_ms
+    else:
+        kname, nbytes, lms = "bwdKernel", prof["bwd_bytes"] / bwd_per_iter, bwd_launch_ms
+    achieved = nbytes / (lms * 1e-3) / 1e9
+    return {
+        "bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "bytes_per_launch": round(nbytes), "mean_launch_us": round(lms * 1e3, 2),
+        "fwd_launch_us": round(fwd_launch_ms * 1e3, 2),
+        "bwd_launch_us": round(bwd_launch_ms * 1e3, 2),
+        "fwd_gbs": round(prof["fwd_bytes"] / (prof["fwd_ms"] / it * 1e-3) / 1e9, 1),
+        "bwd_gbs": round(prof["bwd_bytes"] / (prof["bwd_ms"] / it * 1e-3) / 1e9, 1),
+    }
+
+
+def serve_stats(res, rep):
+    lat = sorted(r["retire_ms"] - r["arrival_ms"] for r in res if r["ok"])
+    pick = lambda q: lat[min(len(lat) - 1, int(q * len(lat)))] if lat else None
+    its = sorted(r["iterations"] for r in res if r["ok"])
+    return {
+        "spots": len(res), "solved": len(lat), "converged": sum(r["converged"] for r in res),
+        "iterations": rep["iterations"], "wall_s": round(rep["wall_ms"] / 1e3, 3),
+        "busy_s": round(rep["busy_ms"] / 1e3, 3),
+        "iters_per_s_wall": round(rep["iterations"] / (rep["wall_ms"] * 1e-3), 1),
+        "iters_per_s_busy": round(rep["iterations"] / (rep["busy_ms"] * 1e-3), 1),
+        "p50_ms": round(pick(0.50), 2) if lat else None,
+        "p99_ms": round(pick(0.99), 2) if lat else None,
+        "iters_per_spot_median": its[len(its) // 2] if its else None,
+        "peak_live": rep["peak_live"], "steps": rep["steps"],
+    }
+
+
+def bench_batched(args, capi, configs):
+    t0 = time.perf_counter()
+    spots = configs.batch_stream(args.spots)
+    pairs = [spot_cfg(capi, s) for s in spots]
+    S, C = [p[0] for p in pairs], [p[1] for p in pairs]
+    log(f"bench: {len(spots)} batched-turn spots, {sum(s['nodes'] for s in spots)} nodes in "
+        f"total, generated in {time.perf_counter() - t0:.1f}s")
+    opt = dict(target_frac=TARGET, check_every=CHECK_EVERY, max_iters=args.steps, real_time=0,
+               prefill=1)
+    if args.warmup > 0:  # code objects, allocator, graph machinery
+        capi.serve(S[:8], C[:8], "dcfr", None, **dict(opt, target_frac=0.0, check_every=args.warmup,
+                                                     max_iters=args.warmup))
+    if args.profile_only:
+        prof = capi.batch_profile(S, C, args.profile_iters)
+        log(f"bench: profile {json.dumps(prof)}")
+        print(json.dumps({"profile": prof, "roofline": roofline_of(prof)}), flush=True)
+        return 0
+
+    res, rep = capi.serve(S, C, "dcfr", None, **opt)
+    bad = [r["error"] for r in res if not r["ok"]]
+    if bad:
+        raise RuntimeError(f"batched run rejected spots: {bad[:3]}")
+    batched = serve_stats(res, rep)
+    value = rep["iterations"] / (rep["busy_ms"] * 1e-3)
+    log(f"bench: batched {json.dumps(batched)}")
+
+    single = None
+    if not args.no_single:
+        res1, rep1 = capi.serve(S, C, "dcfr", None, **dict(opt, max_live=1))
+        single = serve_stats(res1, rep1)
+        log(f"bench: one at a time {json.dumps(single)}")
+        same = all(a["iterations"] == b["iterations"] and a["ev0"] == b["ev0"]
+                   and a["expl"] == b["expl"] for a, b in zip(res, res1))
+        single["identical_to_batched"] = same
+
+    serving = None
+    if args.serve_spots > 0:
+        t0 = time.perf_counter()
+        stream = configs.serving_stream(args.serve_spots, args.qps)
+        log(f"bench: serving stream of {len(stream)} generated in {time.perf_counter() - t0:.1f}s")
+        sp = [spot_cfg(capi, s) for s in stream]
+        res2, rep2 = capi.serve([p[0] for p in sp], [p[1] for p in sp], "dcfr",
+                                [s["arrival_ms"] for s in stream],
+                                **dict(opt, real_time=1, prefill=0))
+        serving = serve_stats(res2, rep2)
+        serving["qps"] = args.qps
+        serving["mix"] = "70% batched-turn spots, 30% river spots"
+        log(f"bench: serving {json.dumps(serving)}")
+
+    prof = capi.batch_profile(S, C, args.profile_iters)
+    log(f"bench: profile {json.dumps(prof)}")
+    roof = roofline_of(prof)
+
+    steps = rep["steps"] * CHECK_EVERY
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(rep["busy_ms"] / steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched-turn-{len(spots)}",
+            "spots": len(spots), "target_expl_frac_pot": TARGET, "check_every": CHECK_EVERY,
+            "max_iters": args.steps, "nodes_total": sum(s["nodes"] for s in spots),
+            "batched": batched,
+            "single": single,
+            "batched_vs_single": round(value / single["iters_per_s_busy"], 3) if single else None,
+            "serving": serving,
+            "resident_batch": {
+                "graph_ms_per_batched_iter": round(prof["graph_ms_per_iter"], 4),
+                "iters_per_s": round(prof["spots"] / (prof["graph_ms_per_iter"] * 1e-3), 1),
+                "alg_bytes_per_batched_iter": prof["fwd_bytes"] + prof["bwd_bytes"],
+                "gbs": round((prof["fwd_bytes"] + prof["bwd_bytes"])
+                             / (prof["graph_ms_per_iter"] * 1e-3) / 1e9, 1),
+                "kernels_per_iter": int(prof["kernels_per_iter"]),
+                "items_per_iter": int(prof["items_per_iter"]),
+            },
+        },
+        "roofline": roof,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_spots(spots, [r["iterations"] for r in res],
+                                                  args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_single(args, capi, configs):
+    spot = configs.large_turn() if args.workload == "large-turn" else configs.plain_turn()
+    sp, cf = spot_cfg(capi, spot)
+    t0 = time.perf_counter()
+    solver = capi.Solver(sp, cf)
+    log(f"bench: {spot['name']} nodes={solver.num_nodes} compile+upload "
+        f"{time.perf_counter() - t0:.2f}s")
+    steps = args.steps if args.steps != 4096 else spot["iters"]
+    solver.solve(args.warmup, spot["algo"])  # graph capture + warm caches
+    t0 = time.perf_counter()
+    solver.continue_solve(steps)
+    wall = time.perf_counter() - t0
+    value = steps / wall
+    st = solver.stats()
+    info = solver.engine_info()
+    prof = solver.profile(args.profile_iters)
+    prof.update(fwd_bytes=info["fwd_bytes"], bwd_bytes=info["bwd_bytes"])
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "iters/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": spot["name"], "nodes": solver.num_nodes,
+            "alg_bytes_per_iter": solver.iteration_bytes,
+            "device_ms_per_step": round(solver.last_solve_ms / steps, 4),
+            "kernels_per_iter": info["kernels_per_iter"],
+            "exploitability_pct_pot": round(100 * st["exploitability"] / spot["pot"], 4),
+        },
+        "roofline": roofline_of(prof),
+        "cpu_baseline": None,
+    }
+    solver.close()
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_single(spot, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--profile-iters", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=4096,
+                    help="per-spot iteration cap (batched) / timed iterations (single spot)")
+    ap.add_argument("--warmup", type=int, default=64)
+    ap.add_argument("--workload", default="batched-turn",
+                    choices=["batched-turn", "large-turn", "plain-turn"])
+    ap.add_argument("--spots", type=int, default=256)
+    ap.add_argument("--serve-spots", type=int, default=1000)
+    ap.add_argument("--qps", type=float, default=40.0)
+    ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--profile-iters", type=int, default=8)
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only the launch-level profile of the resident batch")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="large-turn", choices=["large-turn", "plain-turn"])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy kernel used to scale byte counters")
     args = ap.parse_args()
@@ -80,80 +306,10 @@ def main():
               f"across GPUs with no exchange, so N GPUs run N independent benches)",
               file=sys.stderr)
         return 2
-
     from pps import capi, configs
-
-    spot = configs.large_turn() if args.workload == "large-turn" else configs.plain_turn()
-    sp = capi.make_spot(spot["board"], spot["w0"], spot["w1"], spot["pot"], spot["stack"])
-    cf = capi.make_config(spot["bets"], spot["raises"], spot["allin"], False, spot["max_raises"])
-    t0 = time.perf_counter()
-    solver = capi.Solver(sp, cf)
-    compile_s = time.perf_counter() - t0
-    log(f"bench: {spot['name']} nodes={solver.num_nodes} compile+upload {compile_s:.2f}s")
-
-    solver.solve(args.warmup, spot["algo"])  # graph capture + warm caches
-    t0 = time.perf_counter()
-    solver.continue_solve(args.steps)
-    wall = time.perf_counter() - t0
-    dev_ms = solver.last_solve_ms
-    value = args.steps / wall
-    log(f"bench: {args.steps} iterations in {wall * 1e3:.1f} ms wall, {dev_ms:.1f} ms device")
-
-    st = solver.stats()
-    log(f"bench: stats after {solver.total_iterations} iterations: {st}")
-    info = solver.engine_info()
-    alg_bytes = solver.iteration_bytes
-    prof = solver.profile(args.profile_iters)
-    fwd_launch_ms = prof["fwd_ms"] / max(prof["fwd_launches"], 1)
-    bwd_launch_ms = prof["bwd_ms"] / max(prof["bwd_launches"], 1)
-    per_iter_fwd = prof["fwd_launches"] / prof["iters"]
-    per_iter_bwd = prof["bwd_launches"] / prof["iters"]
-    if prof["bwd_ms"] >= prof["fwd_ms"]:
-        kname, nbytes, lms = "bwdKernel", info["bwd_bytes"] / per_iter_bwd, bwd_launch_ms
-    else:
-        kname, nbytes, lms = "fwdKernel", info["fwd_bytes"] / per_iter_fwd, fwd_launch_ms
-    achieved = nbytes / (lms * 1e-3) / 1e9
-
-    line = {
-        "metric": METRIC,
-        "value": round(value, 3),
-        "unit": "iters/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(wall * 1e3 / args.steps, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": spot["name"], "nodes": solver.num_nodes, "board":
-            "".join(configs.card_text(c) for c in spot["board"]),
-            "combos": [sum(1 for w in spot["w0"] if w > 0), sum(1 for w in spot["w1"] if w > 0)],
-            "pot": spot["pot"], "stack": spot["stack"], "bets": spot["bets"],
-            "raises": spot["raises"], "max_raises": spot["max_raises"], "algo": spot["algo"],
-            "alg_bytes_per_iter": alg_bytes,
-            "iteration_gbs": round(alg_bytes * value / 1e9, 1),
-            "device_ms_per_step": round(dev_ms / args.steps, 4),
-            "kernels_per_iter": info["kernels_per_iter"],
-            "exploitability_pct_pot": round(100 * st["exploitability"] / spot["pot"], 4),
-        },
-        "roofline": {
-            "bound": "hbm", "kernel": kname,
-            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "bytes_per_launch": round(nbytes), "mean_launch_us": round(lms * 1e3, 2),
-            "fwd_launch_us": round(fwd_launch_ms * 1e3, 2),
-            "bwd_launch_us": round(bwd_launch_ms * 1e3, 2),
-        },
-        "cpu_baseline": None,
-    }
-    solver.close()
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(spot, args.cpu_seconds)
-    print(json.dumps(line), flush=True)
-    return 0
+    if args.workload == "batched-turn":
+        return bench_batched(args, capi, configs)
+    return bench_single(args, capi, configs)
 
 
 if __name__ == "__main__":

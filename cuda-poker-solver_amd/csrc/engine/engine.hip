@@ -764,8 +764,9 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
       std::stable_partition(f.begin(), f.end(), [](const Item& x) { return x.kind >= kBwdFold; });
       S.itemCount += (int64_t)(f.size() + S.items[tr][1][dep].size());
     }
-  // Algorithmic bytes of one iteration, split by pass: every f32 row element
-  // read or written once per use (index tables excluded).
+  // Algorithmic bytes of one iteration, split by the launches that move
+  // them: every f32 row element read or written once per use (index tables
+  // excluded). Fold/showdown values are computed in the forward launches.
   for (int64_t u = 0; u < N; ++u) {
     const int cl = t->cls[u];
     const double n0 = t->cls_n[0][cl], n1 = t->cls_n[1][cl];
@@ -785,7 +786,7 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
       }
       S.bwdBytes += 4 * (n0 + n1);
     } else {
-      S.bwdBytes += 8 * (n0 + n1);
+      S.fwdBytes += 8 * (n0 + n1);
     }
   }
   return 0;

@@ -104,7 +104,8 @@ int ppsg_write(ppsg_t* h, int buf, int64_t off, int64_t n, const float* host);
 int ppsg_last_iterate_ms(ppsg_t* h, double* ms);
 /* Engine facts: [0] kernels per iteration, [1] device bytes allocated,
    [2] work items per iteration, [3] graph captured (0/1), [4] forward and
-   [5] backward algorithmic f32 row bytes per iteration. */
+   [5] backward algorithmic f32 row bytes per iteration (fold and showdown
+   rows count as forward: they are computed in the forward launches). */
 int ppsg_info(ppsg_t* h, int64_t* out, int cap);
 /* Runs `iters` CFR iterations launch by launch (no graph), timing every
    launch with events on the engine stream: out = [forward ms, forward
