@@ -52,6 +52,9 @@ constexpr int kMaxList = 1326;
 constexpr int kRowRegs = (kMaxList + kBlock - 1) / kBlock;  // 6
 constexpr int64_t kAlign = 256;
 constexpr int kGatherHeader = 256;  // bytes: 53 int32 segment bounds + pad
+// combos per lane (kTile apart) of decide and chance items
+constexpr int kRDecide = 2;
+constexpr int kRChance = 4;
 
 thread_local std::string g_err;
 
@@ -92,26 +95,43 @@ enum ItemKind : uint8_t {
 //                reach block (stride n), n opponent combos, na actions
 //   kFwdChance   a to-parent map (u16), b parent reach row, c child reach
 //                row, n child opponent combos, v0 = 1/(52-board-4)
+//   (decide and chance items cover combos [tile * kTile * r, +kTile * r),
+//   lane t taking t, t + kTile, ... so that r loads per row are in flight)
 //   kBwdDecide   a regret row, d strategy row, b node cfv row, c children's
 //                cfv block, m child stride (floats), n traverser combos,
 //                e = actor is the traverser
 //   kBwdChance   a branch x parent-slot map (i16: child slot or -1), b node
 //                cfv row, c children's cfv block, m child stride, n
 //                traverser combos (parent), na branches
-//   kBwdFold /   a node reach row, b node cfv row, c the class's gather
-//   kBwdShowdown block (53 i32 segment bounds, then the u16 gather list), d
-//                per traverser combo metadata (fold: u32 cards + same;
-//                showdown: 4 u32), e gather entries per lane (8 or 16), na
-//                row registers per lane (3 or 6), n traverser combos, m
-//                opponent combos, v0..v2 payoffs (fold: v0)
+//   kBwdFold /   a group of fold (or showdown) nodes of one board class:
+//   kBwdShowdown a the group's first TermNode, cnt nodes in the group, b the
+//                spot's chunk (TermNode offsets are relative to it), c
+//                showdown: the class's gather block (53 i32 segment bounds,
+//                then the u16 gather list); fold: the opponent's u32 cards
+//                per combo, d per traverser combo metadata (fold: u32 cards
+//                + same; showdown: 4 u32), e gather entries per lane
+//                (showdown: 8, 12 or 16), na row registers per lane (3 or
+//                6), n traverser combos, m opponent combos
 struct alignas(16) Item {
   int64_t a, b, c, d;
   int32_t n, m, e, f;
   uint8_t kind, na;
-  uint16_t tile;
-  float v0, v1, v2;
+  uint8_t tile;  // decide / chance: which run of kTile * r combos
+  uint8_t r;     // decide / chance: combos per lane (kTile apart)
+  float v0;
+  int32_t cnt;   // terminal groups: nodes in the group
+  int32_t pad;
 };
 static_assert(sizeof(Item) == 64, "work item must be one 64-byte line");
+
+// One fold or showdown node of a terminal group: its opponent reach row and
+// its cfv row (byte offsets in the spot's chunk) and payoffs (fold: v0 per
+// unit of opponent mass; showdown: win, tie, lose).
+struct alignas(16) TermNode {
+  int64_t reach, out;
+  float v0, v1, v2, pad;
+};
+static_assert(sizeof(TermNode) == 32, "terminal node entry is 32 bytes");
 
 struct Dev {
   char* arena;
@@ -132,119 +152,267 @@ __device__ __forceinline__ float sigmaOf(float x, float acc, float uniform, bool
 // Forward reach.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ void fwdItem(const Dev& d, const Item& it, int fromStrat) {
-  const int lane = it.tile * kTile + (int)threadIdx.x;
-  if (lane >= it.n) return;
-  const float* __restrict__ in = at<float>(d, it.b);
-  if (it.kind == kFwdChance) {
-    at<float>(d, it.c)[lane] = in[at<uint16_t>(d, it.a)[lane]] * it.v0;
+// Decide (opponent acts): child reach a = reach * sigma_a. NA > 0: action
+// count fixed at compile time (the row values stay in registers); every
+// lane handles up to R combos kTile apart, all loads issued first.
+template <int NA, int R>
+__device__ __forceinline__ void fwdDecide(const Dev& d, const Item& it, int fromStrat) {
+  const int c0 = it.tile * kTile * R + (int)threadIdx.x;
+  if (c0 >= it.n) return;
+  const int n = it.n, na = NA > 0 ? NA : it.na;
+  const float* __restrict__ reach = at<float>(d, it.b);
+  const float* __restrict__ src = at<float>(d, fromStrat ? it.d : it.a);
+  float* __restrict__ out = at<float>(d, it.c);
+  const bool plus = !fromStrat;
+  const float uniform = 1.0f / (float)na;
+  if (NA > 0) {
+    float x[R][NA > 0 ? NA : 1], r[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int c = c0 + k * kTile;
+      r[k] = c < n ? reach[c] : 0.0f;
+#pragma unroll
+      for (int a = 0; a < NA; ++a) x[k][a] = c < n ? src[(int64_t)a * n + c] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int c = c0 + k * kTile;
+      if (c >= n) continue;
+      float acc = 0.0f;
+#pragma unroll
+      for (int a = 0; a < NA; ++a) acc += plus ? (x[k][a] > 0.0f ? x[k][a] : 0.0f) : x[k][a];
+#pragma unroll
+      for (int a = 0; a < NA; ++a) out[(int64_t)a * n + c] = r[k] * sigmaOf(x[k][a], acc, uniform, plus);
+    }
     return;
   }
-  const int n = it.n, na = it.na;
-  const float r = in[lane];
-  const float* __restrict__ src = at<float>(d, fromStrat ? it.d : it.a) + lane;
-  const bool plus = !fromStrat;
-  float acc = 0.0f;
-  for (int a = 0; a < na; ++a) {
-    const float x = src[(int64_t)a * n];
-    acc += plus ? (x > 0.0f ? x : 0.0f) : x;
+  for (int k = 0; k < R; ++k) {
+    const int c = c0 + k * kTile;
+    if (c >= n) return;
+    const float rr = reach[c];
+    float acc = 0.0f;
+    for (int a = 0; a < na; ++a) {
+      const float x = src[(int64_t)a * n + c];
+      acc += plus ? (x > 0.0f ? x : 0.0f) : x;
+    }
+    for (int a = 0; a < na; ++a)
+      out[(int64_t)a * n + c] = rr * sigmaOf(src[(int64_t)a * n + c], acc, uniform, plus);
   }
-  const float uniform = 1.0f / (float)na;
-  float* __restrict__ out = at<float>(d, it.c) + lane;
-  for (int a = 0; a < na; ++a)
-    out[(int64_t)a * n] = r * sigmaOf(src[(int64_t)a * n], acc, uniform, plus);
+}
+
+__device__ __forceinline__ void fwdItem(const Dev& d, const Item& it, int fromStrat) {
+  if (it.kind == kFwdChance) {
+    const int c0 = it.tile * kTile * kRChance + (int)threadIdx.x;
+    const float* __restrict__ in = at<float>(d, it.b);
+    const uint16_t* __restrict__ tp = at<uint16_t>(d, it.a);
+    float* __restrict__ out = at<float>(d, it.c);
+    int p[kRChance];
+#pragma unroll
+    for (int k = 0; k < kRChance; ++k) p[k] = c0 + k * kTile < it.n ? tp[c0 + k * kTile] : -1;
+#pragma unroll
+    for (int k = 0; k < kRChance; ++k)
+      if (p[k] >= 0) out[c0 + k * kTile] = in[p[k]] * it.v0;
+    return;
+  }
+  switch (it.na) {
+    case 1: fwdDecide<1, kRDecide>(d, it, fromStrat); break;
+    case 2: fwdDecide<2, kRDecide>(d, it, fromStrat); break;
+    case 3: fwdDecide<3, kRDecide>(d, it, fromStrat); break;
+    case 4: fwdDecide<4, kRDecide>(d, it, fromStrat); break;
+    case 5: fwdDecide<5, kRDecide>(d, it, fromStrat); break;
+    case 6: fwdDecide<6, kRDecide>(d, it, fromStrat); break;
+    default: fwdDecide<0, kRDecide>(d, it, fromStrat); break;
+  }
 }
 
 // ---------------------------------------------------------------------------
-// Fold and showdown terminals. Opponent reach is staged in LDS, then each
-// lane gathers its PER consecutive entries of the class's gather list
-// (showdown: slots in strength order followed by the per-card segments in
-// strength order; fold: the per-card segments) and one blocked f64 scan
-// turns them into prefix sums X. Per traverser combo:
-//   fold:     mass = total - (its two cards' segment sums) + identical combo
-//   showdown: weaker / weaker-or-equal / all masses from X at le, te, n and
-//             at the compile-time cut points of its two cards' segments.
-// All global reads (reach, gather indices, per-combo metadata) are issued
-// before the first barrier.
+// Fold and showdown terminals, one workgroup per group of nodes that share a
+// board class (and so the class's index tables, which stay in registers
+// across the group). The next node's reach row is loaded while the current
+// one is evaluated.
 // ---------------------------------------------------------------------------
 
-template <bool SD, int PER, int ROWS>
-__device__ void terminalNode(const Dev& d, const Item& it, char* lds) {
+// Fixed point for the fold card sums: reach (<= 1 per combo, <= 1326
+// combos) times 2^46 fits an int64 with room to spare; integer sums are
+// exact in any order, so LDS atomics stay deterministic.
+constexpr float kFixScale = 70368744177664.0f;  // 2^46
+constexpr double kFixInv = 1.0 / 70368744177664.0;
+
+__device__ __forceinline__ long long toFix(float r) { return __float2ll_rn(r * kFixScale); }
+
+// Fold group: mass(i) = total - cardSum[c1] - cardSum[c2] + r[same(i)],
+// with the 52 card sums accumulated by fixed-point LDS atomics (each
+// opponent combo adds its reach to its two cards; the total is half the
+// sum of the card sums).
+template <int ROWS>
+__device__ void foldGroup(const Dev& d, const Item& it, char* lds) {
+  __shared__ long long cs[53];
+  const int tid = threadIdx.x;
+  const int nO = it.m, nT = it.n;
+  float* raw = reinterpret_cast<float*>(lds);
+  const TermNode* __restrict__ nodes = at<TermNode>(d, it.a);
+  const char* base = d.arena + it.b;
+  uint32_t oc[ROWS], tc[ROWS];
+#pragma unroll
+  for (int q = 0; q < ROWS; ++q) {
+    const int j = tid + q * kBlock;
+    oc[q] = j < nO ? at<uint32_t>(d, it.c)[j] : 0u;
+    tc[q] = j < nT ? at<uint32_t>(d, it.d)[j] : 0u;
+  }
+  float rv[ROWS];
+  {
+    const float* __restrict__ r = reinterpret_cast<const float*>(base + nodes[0].reach);
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q) {
+      const int j = tid + q * kBlock;
+      rv[q] = j < nO ? r[j] : 0.0f;
+    }
+  }
+  for (int g = 0; g < it.cnt; ++g) {
+    const TermNode nd = nodes[g];
+    if (g > 0) __syncthreads();  // the previous node's queries are done
+    if (tid < 53) cs[tid] = 0;
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q) {
+      const int j = tid + q * kBlock;
+      if (j < nO) raw[j] = rv[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q) {
+      const int j = tid + q * kBlock;
+      if (j < nO) {
+        const unsigned long long v = (unsigned long long)toFix(rv[q]);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&cs[oc[q] & 0xff]), v);
+        atomicAdd(reinterpret_cast<unsigned long long*>(&cs[(oc[q] >> 8) & 0xff]), v);
+      }
+    }
+    if (g + 1 < it.cnt) {  // prefetch the next node's reach
+      const float* __restrict__ r = reinterpret_cast<const float*>(base + nodes[g + 1].reach);
+#pragma unroll
+      for (int q = 0; q < ROWS; ++q) {
+        const int j = tid + q * kBlock;
+        rv[q] = j < nO ? r[j] : 0.0f;
+      }
+    }
+    __syncthreads();
+    if (tid < 64) {  // total = half the sum of the card sums (exact)
+      long long x = tid < 52 ? cs[tid] : 0;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+      if (tid == 0) cs[52] = x / 2;
+    }
+    __syncthreads();
+    float* __restrict__ out = reinterpret_cast<float*>(d.arena + it.b + nd.out);
+    const long long tot = cs[52];
+    const double v = (double)nd.v0 * kFixInv;
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q) {
+      const int i = tid + q * kBlock;
+      if (i >= nT) continue;
+      const int c1 = tc[q] & 0xff, c2 = (tc[q] >> 8) & 0xff;
+      const int same = (int)(tc[q] >> 16) - 1;
+      long long m = tot - cs[c1] - cs[c2];
+      if (same >= 0) m += toFix(raw[same]);
+      out[i] = (float)((double)m * v);
+    }
+  }
+}
+
+// Showdown group. Per node: opponent reach is staged in LDS, then each lane
+// gathers its PER consecutive entries of the class's gather list (slots in
+// strength order, then the per-card segments in strength order) and one
+// blocked f64 scan turns them into prefix sums X. Per traverser combo the
+// weaker / weaker-or-equal / all masses come from X at le, te, n and at the
+// compile-time cut points of its two cards' segments.
+template <int PER, int ROWS>
+__device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
   __shared__ double waveTot[kBlock / 64];
   __shared__ int so[53];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nO = it.m, nT = it.n;
   float* raw = reinterpret_cast<float*>(lds);
   double* X = reinterpret_cast<double*>(lds + (((nO + 1) * 4 + 15) & ~15));
+  const TermNode* __restrict__ nodes = at<TermNode>(d, it.a);
+  const char* base = d.arena + it.b;
 
-  // 1. issue every global read
-  const float* __restrict__ r = at<float>(d, it.a);
-  float rv[ROWS];
+  // class data for the whole group: 4 gather indices per 8-byte load
+  uint2 gi[PER / 4];
+  const uint2* gsrc = at<uint2>(d, it.c + kGatherHeader) + tid * (PER / 4);
 #pragma unroll
-  for (int q = 0; q < ROWS; ++q) {
-    const int j = tid + q * kBlock;
-    rv[q] = j < nO ? r[j] : 0.0f;
-  }
-  uint4 gi[PER / 8];
-  const uint4* gsrc = at<uint4>(d, it.c + kGatherHeader) + tid * (PER / 8);
-#pragma unroll
-  for (int k = 0; k < PER / 8; ++k) gi[k] = gsrc[k];
+  for (int k = 0; k < PER / 4; ++k) gi[k] = gsrc[k];
   uint4 meta[ROWS];
-  uint32_t cards[ROWS];
 #pragma unroll
   for (int q = 0; q < ROWS; ++q) {
     const int i = tid + q * kBlock;
-    if (SD) meta[q] = i < nT ? at<uint4>(d, it.d)[i] : make_uint4(0, 0, 0, 0);
-    else cards[q] = i < nT ? at<uint32_t>(d, it.d)[i] : 0u;
+    meta[q] = i < nT ? at<uint4>(d, it.d)[i] : make_uint4(0, 0, 0, 0);
   }
   if (tid < 53) so[tid] = at<int32_t>(d, it.c)[tid];
-
-  // 2. stage reach (raw[nO] = 0 is the padding sentinel)
+  float rv[ROWS];
+  {
+    const float* __restrict__ r = reinterpret_cast<const float*>(base + nodes[0].reach);
 #pragma unroll
-  for (int q = 0; q < ROWS; ++q) {
-    const int j = tid + q * kBlock;
-    if (j < nO) raw[j] = rv[q];
-  }
-  if (tid == 0) raw[nO] = 0.0f;
-  __syncthreads();
-
-  // 3. gather + blocked exclusive scan (f64); the lane's entries are summed
-  // once for the scan and re-read from LDS when the prefixes are written
-  // (keeps the lane's running values out of registers)
-  double own = 0.0;
-#pragma unroll
-  for (int k = 0; k < PER / 8; ++k) {
-    const uint32_t w[4] = {gi[k].x, gi[k].y, gi[k].z, gi[k].w};
-#pragma unroll
-    for (int h = 0; h < 8; ++h) own += (double)raw[(w[h >> 1] >> ((h & 1) * 16)) & 0xffffu];
-  }
-  double inc = own;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const double t = __shfl_up(inc, off, 64);
-    if (lane >= off) inc += t;
-  }
-  if (lane == 63) waveTot[wave] = inc;
-  __syncthreads();
-  double run = inc - own;
-  for (int w = 0; w < wave; ++w) run += waveTot[w];
-#pragma unroll
-  for (int k = 0; k < PER / 8; ++k) {
-    const uint32_t w[4] = {gi[k].x, gi[k].y, gi[k].z, gi[k].w};
-#pragma unroll
-    for (int h = 0; h < 8; ++h) {
-      X[tid * PER + k * 8 + h] = run;
-      run += (double)raw[(w[h >> 1] >> ((h & 1) * 16)) & 0xffffu];
+    for (int q = 0; q < ROWS; ++q) {
+      const int j = tid + q * kBlock;
+      rv[q] = j < nO ? r[j] : 0.0f;
     }
   }
-  if (tid == kBlock - 1) X[kBlock * PER] = run;
-  __syncthreads();
 
-  // 4. per traverser combo
-  float* __restrict__ out = at<float>(d, it.b);
-  if (SD) {
+  for (int g = 0; g < it.cnt; ++g) {
+    const TermNode nd = nodes[g];
+    if (g > 0) __syncthreads();  // the previous node's queries are done with raw / X
+    // 1. stage reach (raw[nO] = 0 is the padding sentinel)
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q) {
+      const int j = tid + q * kBlock;
+      if (j < nO) raw[j] = rv[q];
+    }
+    if (tid == 0) raw[nO] = 0.0f;
+    __syncthreads();
+    if (g + 1 < it.cnt) {  // prefetch the next node's reach
+      const float* __restrict__ r = reinterpret_cast<const float*>(base + nodes[g + 1].reach);
+#pragma unroll
+      for (int q = 0; q < ROWS; ++q) {
+        const int j = tid + q * kBlock;
+        rv[q] = j < nO ? r[j] : 0.0f;
+      }
+    }
+
+    // 2. gather + blocked exclusive scan (f64); the lane's entries are summed
+    // once for the scan and re-read from LDS when the prefixes are written
+    double own = 0.0;
+#pragma unroll
+    for (int k = 0; k < PER / 4; ++k) {
+      const uint32_t w[2] = {gi[k].x, gi[k].y};
+#pragma unroll
+      for (int h = 0; h < 4; ++h) own += (double)raw[(w[h >> 1] >> ((h & 1) * 16)) & 0xffffu];
+    }
+    double inc = own;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const double t = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += t;
+    }
+    if (lane == 63) waveTot[wave] = inc;
+    __syncthreads();
+    double run = inc - own;
+    for (int w = 0; w < wave; ++w) run += waveTot[w];
+#pragma unroll
+    for (int k = 0; k < PER / 4; ++k) {
+      const uint32_t w[2] = {gi[k].x, gi[k].y};
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        X[tid * PER + k * 4 + h] = run;
+        run += (double)raw[(w[h >> 1] >> ((h & 1) * 16)) & 0xffffu];
+      }
+    }
+    if (tid == kBlock - 1) X[kBlock * PER] = run;
+    __syncthreads();
+
+    // 3. per traverser combo
+    float* __restrict__ out = reinterpret_cast<float*>(d.arena + it.b + nd.out);
     const double* Y = X + nO;  // per-card segment prefix (offset cancels)
-    const double vWin = it.v0, vTie = it.v1, vLose = it.v2;
+    const double vWin = nd.v0, vTie = nd.v1, vLose = nd.v2;
 #pragma unroll
     for (int q = 0; q < ROWS; ++q) {
       const int i = tid + q * kBlock;
@@ -267,135 +435,154 @@ __device__ void terminalNode(const Dev& d, const Item& it, char* lds) {
       if (lose < 0.0) lose = 0.0;
       out[i] = (float)(win * vWin + tie * vTie + lose * vLose);
     }
-  } else {
-    const double tot = 0.5 * X[2 * nO];
-    const double v = it.v0;
-#pragma unroll
-    for (int q = 0; q < ROWS; ++q) {
-      const int i = tid + q * kBlock;
-      if (i >= nT) continue;
-      const int c1 = cards[q] & 0xff, c2 = (cards[q] >> 8) & 0xff;
-      const int same = (int)(cards[q] >> 16) - 1;
-      double m = tot - (X[so[c1 + 1]] - X[so[c1]]) - (X[so[c2 + 1]] - X[so[c2]]);
-      if (same >= 0) m += (double)raw[same];
-      if (m < 0.0) m = 0.0;
-      out[i] = (float)(m * v);
-    }
   }
 }
 
-// Forward launch of one depth: opponent-reach items for the next depth plus
-// the terminal nodes of this depth (their values need only this depth's
-// reach, so they run here and the backward pass only combines).
-// kindMask: profiling attribution only (-1 = every kind in real launches).
+// Terminal launches of a half-step (after the forward pass has produced all
+// reach rows): one for the fold groups, one for the showdown groups, so each
+// kernel keeps its own register budget.
 // masked: some listed spot is retired; skip its items.
+__global__ __launch_bounds__(kBlock) void foldKernel(Dev d, const Item* __restrict__ items,
+                                                     int masked) {
+  extern __shared__ __align__(16) char lds[];
+  const Item it = items[blockIdx.x];
+  if (masked && !d.ctl[it.f].alive) return;
+  if (it.na == 3) foldGroup<3>(d, it, lds);
+  else foldGroup<kRowRegs>(d, it, lds);
+}
+
+__global__ __launch_bounds__(kBlock) void showdownKernel(Dev d, const Item* __restrict__ items,
+                                                         int masked) {
+  extern __shared__ __align__(16) char lds[];
+  const Item it = items[blockIdx.x];
+  if (masked && !d.ctl[it.f].alive) return;
+  if (it.na == 3) {
+    if (it.e == 8) showdownGroup<8, 3>(d, it, lds);
+    else if (it.e == 12) showdownGroup<12, 3>(d, it, lds);
+    else showdownGroup<16, 3>(d, it, lds);
+  } else {
+    if (it.e == 8) showdownGroup<8, kRowRegs>(d, it, lds);
+    else if (it.e == 12) showdownGroup<12, kRowRegs>(d, it, lds);
+    else showdownGroup<16, kRowRegs>(d, it, lds);
+  }
+}
+
+// Forward launch of one depth: opponent reach of the next depth.
+// kindMask: profiling attribution only (-1 = every kind in real launches).
 __global__ __launch_bounds__(kBlock) void fwdKernel(Dev d, const Item* __restrict__ items,
                                                     int fromStrat, int kindMask, int masked) {
-  extern __shared__ __align__(16) char lds[];
   const Item it = items[blockIdx.x];
   if (!((kindMask >> it.kind) & 1)) return;
   if (masked && !d.ctl[it.f].alive) return;
-  switch (it.kind) {
-    case kBwdFold:
-      if (it.na == 3) {
-        if (it.e == 8) terminalNode<false, 8, 3>(d, it, lds);
-        else terminalNode<false, 16, 3>(d, it, lds);
-      } else {
-        if (it.e == 8) terminalNode<false, 8, kRowRegs>(d, it, lds);
-        else terminalNode<false, 16, kRowRegs>(d, it, lds);
-      }
-      break;
-    case kBwdShowdown:
-      if (it.na == 3) {
-        if (it.e == 8) terminalNode<true, 8, 3>(d, it, lds);
-        else terminalNode<true, 16, 3>(d, it, lds);
-      } else {
-        if (it.e == 8) terminalNode<true, 8, kRowRegs>(d, it, lds);
-        else terminalNode<true, 16, kRowRegs>(d, it, lds);
-      }
-      break;
-    default:
-      fwdItem(d, it, fromStrat);
-  }
+  fwdItem(d, it, fromStrat);
 }
 
 // ---------------------------------------------------------------------------
 // Backward.
 // ---------------------------------------------------------------------------
 
-// Decide tile, one lane per traverser combo. NA > 0: action count fixed at
-// compile time (values stay in registers); NA == 0: generic loop.
-template <int MODE, int NA>
+// Decide tile: every lane handles up to R traverser combos kTile apart
+// (all row loads issued first). NA > 0: action count fixed at compile time
+// (values stay in registers); NA == 0: generic loop.
+template <int MODE, int NA, int R>
 __device__ __forceinline__ void decideTile(const Dev& d, const Item& it) {
-  const int c = it.tile * kTile + (int)threadIdx.x;
+  const int c0 = it.tile * kTile * R + (int)threadIdx.x;
   const int n = it.n;
-  if (c >= n) return;
+  if (c0 >= n) return;
   const int na = NA > 0 ? NA : it.na;
   const int64_t stride = it.m;
-  const float* __restrict__ kid = at<float>(d, it.c) + c;
-  float* __restrict__ out = at<float>(d, it.b) + c;
-  if (!it.e) {  // opponent acts: values add up
-    float v = 0.0f;
-    for (int a = 0; a < na; ++a) v += kid[a * stride];
-    *out = v;
-    return;
-  }
-  if (MODE == PPSG_MODE_BR) {
-    float best = -1e30f;
-    for (int a = 0; a < na; ++a) best = fmaxf(best, kid[a * stride]);
-    *out = best;
+  const float* __restrict__ kid = at<float>(d, it.c);
+  float* __restrict__ out = at<float>(d, it.b);
+  if (!it.e || MODE == PPSG_MODE_BR) {
+    // opponent acts: the child values add up; best response: their maximum
+    const bool sum = !it.e;
+    float v[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = sum ? 0.0f : -1e30f;
+    for (int a = 0; a < na; ++a)
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int c = c0 + k * kTile;
+        if (c < n) {
+          const float x = kid[a * stride + c];
+          v[k] = sum ? v[k] + x : fmaxf(v[k], x);
+        }
+      }
+#pragma unroll
+    for (int k = 0; k < R; ++k)
+      if (c0 + k * kTile < n) out[c0 + k * kTile] = v[k];
     return;
   }
   constexpr bool plus = MODE == PPSG_MODE_CFR;
-  float* __restrict__ src = at<float>(d, plus ? it.a : it.d) + c;
+  float* __restrict__ src = at<float>(d, plus ? it.a : it.d);
+  float* __restrict__ st = at<float>(d, it.d);
   const float uniform = 1.0f / (float)na;
   if (NA > 0) {
-    float x[NA > 0 ? NA : 1], v[NA > 0 ? NA : 1], s[NA > 0 ? NA : 1];
-    float acc = 0.0f;
+    constexpr int A = NA > 0 ? NA : 1;
+    float x[R][A], v[R][A], sm[R][A];
 #pragma unroll
-    for (int a = 0; a < NA; ++a) {
-      x[a] = src[(int64_t)a * n];
-      v[a] = kid[a * stride];
-      acc += plus ? (x[a] > 0.0f ? x[a] : 0.0f) : x[a];
+    for (int k = 0; k < R; ++k) {
+      const int c = c0 + k * kTile;
+#pragma unroll
+      for (int a = 0; a < A; ++a) {
+        x[k][a] = c < n ? src[(int64_t)a * n + c] : 0.0f;
+        v[k][a] = c < n ? kid[a * stride + c] : 0.0f;
+        sm[k][a] = plus && c < n ? st[(int64_t)a * n + c] : 0.0f;
+      }
     }
-    float val = 0.0f;
-#pragma unroll
-    for (int a = 0; a < NA; ++a) {
-      s[a] = sigmaOf(x[a], acc, uniform, plus);
-      val += s[a] * v[a];
+    float pos = 0.0f, neg = 0.0f, avg = 0.0f;
+    if (MODE == PPSG_MODE_CFR) {
+      const Ctl& ctl = d.ctl[it.f];
+      const float* co = ctl.coef + 3 * ctl.counter;
+      pos = co[0];
+      neg = co[1];
+      avg = co[2];
     }
-    *out = val;
-    if (MODE != PPSG_MODE_CFR) return;
-    const Ctl& ctl = d.ctl[it.f];
-    const float* co = ctl.coef + 3 * ctl.counter;
-    const float pos = co[0], neg = co[1], avg = co[2];
-    float* __restrict__ st = at<float>(d, it.d) + c;
 #pragma unroll
-    for (int a = 0; a < NA; ++a) {
-      st[(int64_t)a * n] = st[(int64_t)a * n] * avg + s[a];
-      src[(int64_t)a * n] = x[a] * (x[a] >= 0.0f ? pos : neg) + (v[a] - val);
+    for (int k = 0; k < R; ++k) {
+      const int c = c0 + k * kTile;
+      if (c >= n) continue;
+      float acc = 0.0f;
+#pragma unroll
+      for (int a = 0; a < A; ++a) acc += plus ? (x[k][a] > 0.0f ? x[k][a] : 0.0f) : x[k][a];
+      float s[A], val = 0.0f;
+#pragma unroll
+      for (int a = 0; a < A; ++a) {
+        s[a] = sigmaOf(x[k][a], acc, uniform, plus);
+        val += s[a] * v[k][a];
+      }
+      out[c] = val;
+      if (MODE != PPSG_MODE_CFR) continue;
+#pragma unroll
+      for (int a = 0; a < A; ++a) {
+        st[(int64_t)a * n + c] = sm[k][a] * avg + s[a];
+        src[(int64_t)a * n + c] = x[k][a] * (x[k][a] >= 0.0f ? pos : neg) + (v[k][a] - val);
+      }
     }
     return;
   }
-  float acc = 0.0f;
-  for (int a = 0; a < na; ++a) {
-    const float x = src[(int64_t)a * n];
-    acc += plus ? (x > 0.0f ? x : 0.0f) : x;
-  }
-  float val = 0.0f;
-  for (int a = 0; a < na; ++a)
-    val += sigmaOf(src[(int64_t)a * n], acc, uniform, plus) * kid[a * stride];
-  *out = val;
-  if (MODE != PPSG_MODE_CFR) return;
-  const Ctl& ctl = d.ctl[it.f];
-  const float* co = ctl.coef + 3 * ctl.counter;
-  const float pos = co[0], neg = co[1], avg = co[2];
-  float* __restrict__ st = at<float>(d, it.d) + c;
-  for (int a = 0; a < na; ++a) {
-    const float x = src[(int64_t)a * n];
-    const float sg = sigmaOf(x, acc, uniform, plus);
-    st[(int64_t)a * n] = st[(int64_t)a * n] * avg + sg;
-    src[(int64_t)a * n] = x * (x >= 0.0f ? pos : neg) + (kid[a * stride] - val);
+  for (int k = 0; k < R; ++k) {
+    const int c = c0 + k * kTile;
+    if (c >= n) return;
+    float acc = 0.0f;
+    for (int a = 0; a < na; ++a) {
+      const float x = src[(int64_t)a * n + c];
+      acc += plus ? (x > 0.0f ? x : 0.0f) : x;
+    }
+    float val = 0.0f;
+    for (int a = 0; a < na; ++a)
+      val += sigmaOf(src[(int64_t)a * n + c], acc, uniform, plus) * kid[a * stride + c];
+    out[c] = val;
+    if (MODE != PPSG_MODE_CFR) continue;
+    const Ctl& ctl = d.ctl[it.f];
+    const float* co = ctl.coef + 3 * ctl.counter;
+    const float pos = co[0], neg = co[1], avg = co[2];
+    for (int a = 0; a < na; ++a) {
+      const float x = src[(int64_t)a * n + c];
+      const float sg = sigmaOf(x, acc, uniform, plus);
+      st[(int64_t)a * n + c] = st[(int64_t)a * n + c] * avg + sg;
+      src[(int64_t)a * n + c] = x * (x >= 0.0f ? pos : neg) + (kid[a * stride + c] - val);
+    }
   }
 }
 
@@ -446,13 +633,14 @@ __global__ __launch_bounds__(kBlock) void bwdKernel(Dev d, const Item* __restric
     return;
   }
   switch (it.na) {
-    case 1: decideTile<MODE, 1>(d, it); break;
-    case 2: decideTile<MODE, 2>(d, it); break;
-    case 3: decideTile<MODE, 3>(d, it); break;
-    case 4: decideTile<MODE, 4>(d, it); break;
-    case 5: decideTile<MODE, 5>(d, it); break;
-    case 6: decideTile<MODE, 6>(d, it); break;
-    default: decideTile<MODE, 0>(d, it); break;
+    case 1: decideTile<MODE, 1, kRDecide>(d, it); break;
+    case 2: decideTile<MODE, 2, kRDecide>(d, it); break;
+    case 3: decideTile<MODE, 3, kRDecide>(d, it); break;
+    case 4: decideTile<MODE, 4, kRDecide>(d, it); break;
+    // 5 and 6 actions: one combo per lane keeps the kernel under 72 VGPRs
+    case 5: decideTile<MODE, 5, 1>(d, it); break;
+    case 6: decideTile<MODE, 6, 1>(d, it); break;
+    default: decideTile<MODE, 0, 1>(d, it); break;
   }
 }
 
@@ -530,6 +718,7 @@ struct Layout {
   int64_t toParent[2] = {0, 0}, bmap[2] = {0, 0}, gather[2] = {0, 0};
   int64_t cardSame[2] = {0, 0}, sdMeta[2] = {0, 0};
   int64_t regret = 0, strat = 0, cfv = 0, reach[2] = {0, 0};
+  int64_t termNodes = 0;  // TermNode entries of every terminal group
   int64_t add(int64_t b) {
     const int64_t off = bytes;
     bytes += roundUp(std::max<int64_t>(b, 1));
@@ -537,40 +726,43 @@ struct Layout {
   }
 };
 
-// Gather entries per lane for a list of `len` entries: 8 or 16 (one or two
-// 16-byte index loads).
-int perLane(int len) { return len <= kBlock * 8 ? 8 : 16; }
+// Gather entries per lane for a list of `len` entries: 8, 12 or 16 (two to
+// four 8-byte index loads).
+int perLane(int len) { return len <= kBlock * 8 ? 8 : len <= kBlock * 12 ? 12 : 16; }
 
-// Dynamic LDS of a terminal item: reach (+ sentinel) and the f64 prefixes.
+// Dynamic LDS of a terminal item: reach (+ sentinel), and for showdown the
+// f64 prefixes.
 size_t terminalLdsOf(const Item& it) {
-  return (size_t)(((it.m + 1) * 4 + 15) & ~15) + (size_t)(kBlock * it.e + 1) * 8;
+  const size_t reach = (size_t)(((it.m + 1) * 4 + 15) & ~15);
+  return it.kind == kBwdShowdown ? reach + (size_t)(kBlock * it.e + 1) * 8 : reach;
 }
 
 // Host-side tables of one spot, built from its description.
 struct SpotTables {
   std::vector<int16_t> bmap[2];       // per parent class: [branch][parent slot]
   std::map<int, int64_t> bmapOff[2];  // class -> element offset
-  // per (class, kind): [53 i32 segment bounds | pad to 256 B | u16 gather
-  // list padded to kBlock * per with the sentinel slot n]
+  // per showdown class: [53 i32 segment bounds | pad to 256 B | u16 gather
+  // list (strength order, then per-card segments) padded to kBlock * per
+  // with the sentinel slot n]
   std::vector<char> gather[2];
-  std::map<int, int64_t> foldOff[2], sdOff[2];
-  std::map<int, int> foldPer[2], sdPer[2];
+  std::map<int, int64_t> sdOff[2];
+  std::map<int, int> sdPer[2];
   std::vector<uint32_t> cardSame[2];  // per list entry: c1 | c2 << 8 | (same+1) << 16
+  std::vector<TermNode> termNodes;    // grouped: each group's nodes are contiguous
 
-  void gatherBlock(const ppsg_tree_desc* t, int p, int cl, bool sd) {
-    auto& offs = sd ? sdOff[p] : foldOff[p];
+  void gatherBlock(const ppsg_tree_desc* t, int p, int cl) {
+    auto& offs = sdOff[p];
     if (offs.count(cl)) return;
     const int n = t->cls_n[p][cl];
     const int64_t lo = t->cls_list_off[p][cl];
-    std::vector<uint16_t> list;
-    if (sd) list.insert(list.end(), t->sorted[p] + lo, t->sorted[p] + lo + n);
+    std::vector<uint16_t> list(t->sorted[p] + lo, t->sorted[p] + lo + n);
     list.insert(list.end(), t->seg_slot[p] + 2 * lo, t->seg_slot[p] + 2 * lo + 2 * n);
     const int per = perLane((int)list.size());
     list.resize((size_t)kBlock * per, (uint16_t)n);
     std::vector<char>& g = gather[p];
     g.resize((g.size() + 15) / 16 * 16, 0);
     offs[cl] = (int64_t)g.size();
-    (sd ? sdPer[p] : foldPer[p])[cl] = per;
+    sdPer[p][cl] = per;
     const size_t at0 = g.size();
     g.resize(at0 + kGatherHeader + list.size() * 2, 0);
     std::memcpy(g.data() + at0, t->seg_off[p] + (int64_t)cl * 53, 53 * 4);
@@ -579,6 +771,16 @@ struct SpotTables {
 };
 
 enum SpotState { kFree = 0, kLive = 1, kParked = 2, kDoomed = 3 };
+
+// The fold (or showdown) nodes of one board class for one traverser; the
+// engine cuts it into work items of up to its group size.
+struct TermGroup {
+  uint8_t kind = 0;
+  uint8_t rows = 0;
+  int per = 8, nT = 0, nO = 0, count = 0;
+  int64_t first = 0;          // index of its first TermNode
+  int64_t gather = 0, meta = 0;  // chunk-relative
+};
 
 struct Spot {
   int state = kFree;
@@ -594,8 +796,10 @@ struct Spot {
   int64_t rootCfv = 0;  // chunk-relative byte offset of the root cfv row
   // work items per [traverser][0 forward, 1 backward][depth], chunk-relative
   std::vector<std::vector<Item>> items[2][2];
-  int64_t itemCount = 0;
+  std::vector<TermGroup> groups[2];  // per traverser
+  int64_t itemCount = 0;  // listed items (terminal items at the group size in use)
   double fwdBytes = 0.0, bwdBytes = 0.0;
+  double kindBytes[6] = {0, 0, 0, 0, 0, 0};  // per ItemKind, one iteration
 };
 
 // Plans a spot: chunk layout, host tables and chunk-relative work items.
@@ -626,8 +830,8 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
           T.bmap[p].insert(T.bmap[p].end(), fp, fp + np);
         }
       }
-    } else if (t->kind[u] == PPSG_FOLD || t->kind[u] == PPSG_SHOWDOWN) {
-      for (int p = 0; p < 2; ++p) T.gatherBlock(t, p, cl, t->kind[u] == PPSG_SHOWDOWN);
+    } else if (t->kind[u] == PPSG_SHOWDOWN) {
+      for (int p = 0; p < 2; ++p) T.gatherBlock(t, p, cl);
     }
   }
   for (int p = 0; p < 2; ++p) {
@@ -649,7 +853,6 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
   L.cfv = L.add(4 * t->cfv_total);
   L.reach[0] = L.add(4 * t->reach_total[0]);
   L.reach[1] = L.add(4 * t->reach_total[1]);
-  S.bytes = L.bytes;
   S.n[0] = t->n_base[0];
   S.n[1] = t->n_base[1];
   S.maxDepth = t->max_depth;
@@ -662,7 +865,9 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
   const int D = t->max_depth + 1;
   for (int tr = 0; tr < 2; ++tr)
     for (int ps = 0; ps < 2; ++ps) S.items[tr][ps].assign(D, {});
-  auto tiles = [](int n) { return (n + kTile - 1) / kTile; };
+  auto tiles = [](int n, int r) { return (n + kTile * r - 1) / (kTile * r); };
+  std::map<int, int> groupOf[2];  // (class, kind) -> group index + 1
+  std::vector<std::vector<TermNode>> members[2];
   for (int64_t u = 0; u < N; ++u) {
     const int dep = t->depth[u], cl = t->cls[u], k = t->kind[u];
     int kb, ke;
@@ -676,23 +881,26 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
         const int na = t->na[u];
         const int first = t->child_flat[kb];
         if (t->owner[u] == op) {
-          for (int q = 0; q < tiles(nO); ++q) {
+          for (int q = 0; q < tiles(nO, kRDecide); ++q) {
             Item it{};
             it.kind = kFwdDecide;
+            it.r = kRDecide;
             it.a = L.regret + 4 * t->reg_off[u];
             it.d = L.strat + 4 * t->reg_off[u];
             it.b = L.reach[tr] + 4 * t->reach_off[tr][u];
             it.c = L.reach[tr] + 4 * t->reach_off[tr][first];
             it.n = nO;
             it.na = (uint8_t)na;
-            it.tile = (uint16_t)q;
+            it.tile = (uint8_t)q;
             fwd.push_back(it);
           }
         }
         const int64_t stride = na > 1 ? t->cfv_off[t->child_flat[kb + 1]] - t->cfv_off[first] : 0;
-        for (int q = 0; q < tiles(nT); ++q) {
+        const int rb = na <= 4 ? kRDecide : 1;  // must match bwdKernel's dispatch
+        for (int q = 0; q < tiles(nT, rb); ++q) {
           Item it{};
           it.kind = kBwdDecide;
+          it.r = (uint8_t)rb;
           it.a = L.regret + 4 * t->reg_off[u];
           it.d = L.strat + 4 * t->reg_off[u];
           it.b = L.cfv + 4 * t->cfv_off[u];
@@ -701,7 +909,7 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
           it.n = nT;
           it.na = (uint8_t)na;
           it.e = t->owner[u] == tr;
-          it.tile = (uint16_t)q;
+          it.tile = (uint8_t)q;
           bwd.push_back(it);
         }
       } else if (k == PPSG_CHANCE) {
@@ -710,85 +918,105 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
         for (int b = kb; b < ke; ++b) {
           const int ch = t->child_flat[b];
           const int ccl = t->cls[ch];
-          for (int q = 0; q < tiles(t->cls_n[op][ccl]); ++q) {
+          for (int q = 0; q < tiles(t->cls_n[op][ccl], kRChance); ++q) {
             Item it{};
             it.kind = kFwdChance;
+            it.r = kRChance;
             it.a = L.toParent[op] + 2 * t->cls_list_off[op][ccl];
             it.b = L.reach[tr] + 4 * t->reach_off[tr][u];
             it.c = L.reach[tr] + 4 * t->reach_off[tr][ch];
             it.n = t->cls_n[op][ccl];
             it.v0 = inv;
-            it.tile = (uint16_t)q;
+            it.tile = (uint8_t)q;
             fwd.push_back(it);
           }
         }
         const int first = t->child_flat[kb];
         const int64_t stride = nb > 1 ? t->cfv_off[t->child_flat[kb + 1]] - t->cfv_off[first] : 0;
-        for (int q = 0; q < tiles(nT); ++q) {
+        for (int q = 0; q < tiles(nT, 1); ++q) {
           Item it{};
           it.kind = kBwdChance;
+          it.r = 1;
           it.a = L.bmap[tr] + 2 * T.bmapOff[tr][cl];
           it.b = L.cfv + 4 * t->cfv_off[u];
           it.c = L.cfv + 4 * t->cfv_off[first];
           it.m = (int32_t)stride;
           it.n = nT;
           it.na = (uint8_t)nb;
-          it.tile = (uint16_t)q;
+          it.tile = (uint8_t)q;
           bwd.push_back(it);
         }
       } else {
         const bool sd = k == PPSG_SHOWDOWN;
-        Item it{};
-        it.kind = sd ? kBwdShowdown : kBwdFold;
-        it.a = L.reach[tr] + 4 * t->reach_off[tr][u];
-        it.b = L.cfv + 4 * t->cfv_off[u];
-        it.c = L.gather[op] + (sd ? T.sdOff[op][cl] : T.foldOff[op][cl]);
-        it.d = sd ? L.sdMeta[tr] + 16 * t->cls_list_off[tr][cl]
-                  : L.cardSame[tr] + 4 * t->cls_list_off[tr][cl];
-        it.e = sd ? T.sdPer[op][cl] : T.foldPer[op][cl];
-        it.na = (uint8_t)(std::max(nT, nO) <= 3 * kBlock ? 3 : kRowRegs);
-        it.n = nT;
-        it.m = nO;
+        const int key = cl * 2 + (sd ? 1 : 0);
+        auto& slot = groupOf[tr][key];
+        if (slot == 0) {
+          TermGroup g;
+          g.kind = sd ? kBwdShowdown : kBwdFold;
+          // showdown: the class's gather block and per-combo bounds; fold:
+          // both players' cards tables of the class
+          g.gather = sd ? L.gather[op] + T.sdOff[op][cl] : L.cardSame[op] + 4 * t->cls_list_off[op][cl];
+          g.meta = sd ? L.sdMeta[tr] + 16 * t->cls_list_off[tr][cl]
+                      : L.cardSame[tr] + 4 * t->cls_list_off[tr][cl];
+          g.per = sd ? T.sdPer[op][cl] : 0;
+          g.rows = (uint8_t)(std::max(nT, nO) <= 3 * kBlock ? 3 : kRowRegs);
+          g.nT = nT;
+          g.nO = nO;
+          S.groups[tr].push_back(g);
+          members[tr].emplace_back();
+          slot = (int)S.groups[tr].size();
+        }
+        TermNode e{};
+        e.reach = L.reach[tr] + 4 * t->reach_off[tr][u];
+        e.out = L.cfv + 4 * t->cfv_off[u];
         const float* tv = t->term_val + u * 8 + tr * 4;
-        it.v0 = tv[0];
-        it.v1 = tv[1];
-        it.v2 = tv[2];
-        fwd.push_back(it);
+        e.v0 = tv[0];
+        e.v1 = tv[1];
+        e.v2 = tv[2];
+        members[tr][slot - 1].push_back(e);
       }
     }
   }
+  // terminal node entries, contiguous per group, after the rows
   for (int tr = 0; tr < 2; ++tr)
-    for (int dep = 0; dep < D; ++dep) {
-      // terminal workgroups first: they are the long ones
-      std::vector<Item>& f = S.items[tr][0][dep];
-      std::stable_partition(f.begin(), f.end(), [](const Item& x) { return x.kind >= kBwdFold; });
-      S.itemCount += (int64_t)(f.size() + S.items[tr][1][dep].size());
+    for (size_t g = 0; g < S.groups[tr].size(); ++g) {
+      S.groups[tr][g].first = (int64_t)T.termNodes.size();
+      S.groups[tr][g].count = (int)members[tr][g].size();
+      T.termNodes.insert(T.termNodes.end(), members[tr][g].begin(), members[tr][g].end());
     }
-  // Algorithmic bytes of one iteration, split by the launches that move
-  // them: every f32 row element read or written once per use (index tables
-  // excluded). Fold/showdown values are computed in the forward launches.
+  L.termNodes = L.add((int64_t)(T.termNodes.size() * sizeof(TermNode)));
+  S.bytes = L.bytes;
+  for (int tr = 0; tr < 2; ++tr)
+    for (int dep = 0; dep < D; ++dep)
+      S.itemCount += (int64_t)(S.items[tr][0][dep].size() + S.items[tr][1][dep].size());
+  // Algorithmic bytes of one iteration per node kind: every f32 row element
+  // read or written once per use (index tables excluded). Fold/showdown
+  // count with the forward pass (their launches precede the backward one).
   for (int64_t u = 0; u < N; ++u) {
     const int cl = t->cls[u];
     const double n0 = t->cls_n[0][cl], n1 = t->cls_n[1][cl];
     if (t->kind[u] == PPSG_DECIDE) {
       const double a = t->na[u];
       const double np = t->owner[u] == 0 ? n0 : n1, nq = t->owner[u] == 0 ? n1 : n0;
-      S.fwdBytes += 4 * np * (2 * a + 1);
-      S.bwdBytes += 4 * np * (5 * a + 1) + 4 * nq * (a + 1);
+      S.kindBytes[kFwdDecide] += 4 * np * (2 * a + 1);
+      S.kindBytes[kBwdDecide] += 4 * np * (5 * a + 1) + 4 * nq * (a + 1);
     } else if (t->kind[u] == PPSG_CHANCE) {
       int b, e;
       kidsOf(u, b, e);
       for (int k = b; k < e; ++k) {
         const int ccl = t->cls[t->child_flat[k]];
         const double nck = (double)t->cls_n[0][ccl] + t->cls_n[1][ccl];
-        S.fwdBytes += 8 * nck;
-        S.bwdBytes += 4 * nck;
+        S.kindBytes[kFwdChance] += 8 * nck;
+        S.kindBytes[kBwdChance] += 4 * nck;
       }
-      S.bwdBytes += 4 * (n0 + n1);
+      S.kindBytes[kBwdChance] += 4 * (n0 + n1);
     } else {
-      S.fwdBytes += 8 * (n0 + n1);
+      S.kindBytes[t->kind[u] == PPSG_FOLD ? kBwdFold : kBwdShowdown] += 8 * (n0 + n1);
     }
   }
+  S.fwdBytes = S.kindBytes[kFwdDecide] + S.kindBytes[kFwdChance] + S.kindBytes[kBwdFold] +
+               S.kindBytes[kBwdShowdown];
+  S.bwdBytes = S.kindBytes[kBwdDecide] + S.kindBytes[kBwdChance];
   return 0;
 }
 
@@ -817,7 +1045,10 @@ struct ppsg_engine {
   Ctl* ctl = nullptr;
   int maxSpots = 0;
   std::vector<Spot> spots;
-  std::vector<DevList> lists[2][2];  // [traverser][pass][depth]
+  // [traverser][0 forward / 1 backward per depth, 2 fold groups, 3 showdown
+  // groups (one list each)]
+  std::vector<DevList> lists[2][4];
+  int groupSize = 4;  // terminal nodes per work item
   int64_t listedItems = 0, deadItems = 0;
   std::vector<int> live;
   int* liveDev = nullptr;
@@ -842,11 +1073,14 @@ struct ppsg_engine {
   int remove(int slot);
   int setActive(int slot, bool on);
   int setAlive(int slot, int v);
+  std::vector<Item> spotItems(int slot, int tr, int ps, int dep) const;
+  int64_t countItems(int slot) const;
   int appendSpot(int slot);
   int compact();
   int prepare();
   void dropGraph();
   int launchFwd(int tr, int dpt, int fromStrat, int kindMask, hipStream_t s);
+  int launchTerm(int tr, int ps, hipStream_t s);
   int launchBwd(int tr, int dpt, int mode, int kindMask, hipStream_t s);
   int launchHalf(int tr, int mode, hipStream_t s);
   int capture();
@@ -867,7 +1101,7 @@ void ppsg_engine::dropGraph() {
 ppsg_engine::~ppsg_engine() {
   dropGraph();
   for (int tr = 0; tr < 2; ++tr)
-    for (int ps = 0; ps < 2; ++ps)
+    for (int ps = 0; ps < 4; ++ps)
       for (DevList& l : lists[tr][ps])
         if (l.buf) (void)hipFree(l.buf);
   for (Spot& S : spots)
@@ -905,62 +1139,113 @@ int ppsg_engine::setAlive(int slot, int v) {
   return 0;
 }
 
+// A spot's work items of one list with arena offsets: forward / backward
+// items of a depth, or (ps 2 / 3) its fold / showdown groups cut at the
+// group size.
+std::vector<Item> ppsg_engine::spotItems(int slot, int tr, int ps, int dep) const {
+  const Spot& S = spots[slot];
+  std::vector<Item> v;
+  if (ps < 2) {
+    if (dep > S.maxDepth) return v;
+    v = S.items[tr][ps][dep];
+    for (Item& it : v) {
+      it.a += S.off;
+      it.b += S.off;
+      it.c += S.off;
+      it.d += S.off;
+      it.f = slot;
+    }
+    return v;
+  }
+  const uint8_t kind = ps == 2 ? kBwdFold : kBwdShowdown;
+  for (const TermGroup& g : S.groups[tr]) {
+    if (g.kind != kind) continue;
+    for (int s0 = 0; s0 < g.count; s0 += groupSize) {
+      Item it{};
+      it.kind = g.kind;
+      it.a = S.off + S.L.termNodes + (int64_t)sizeof(TermNode) * (g.first + s0);
+      it.b = S.off;
+      it.c = S.off + g.gather;
+      it.d = S.off + g.meta;
+      it.n = g.nT;
+      it.m = g.nO;
+      it.e = g.per;
+      it.na = g.rows;
+      it.cnt = std::min(groupSize, g.count - s0);
+      it.f = slot;
+      v.push_back(it);
+    }
+  }
+  // largest groups first: they are the long ones
+  std::stable_sort(v.begin(), v.end(), [](const Item& x, const Item& y) {
+    return (int64_t)x.tile * x.m > (int64_t)y.tile * y.m;
+  });
+  return v;
+}
+
+int64_t ppsg_engine::countItems(int slot) const {
+  const Spot& S = spots[slot];
+  int64_t n = 0;
+  for (int tr = 0; tr < 2; ++tr) {
+    for (int ps = 0; ps < 2; ++ps)
+      for (const auto& v : S.items[tr][ps]) n += (int64_t)v.size();
+    for (const TermGroup& g : S.groups[tr]) n += (g.count + groupSize - 1) / groupSize;
+  }
+  return n;
+}
+
+namespace {
+
+int uploadTo(DevList& dl, const std::vector<Item>& v, bool append) {
+  const int64_t at0 = append ? dl.count : 0;
+  const int64_t need = at0 + (int64_t)v.size();
+  if (need > dl.cap) {
+    const int64_t cap = std::max<int64_t>(need, dl.cap + dl.cap / 2);
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, (size_t)cap * sizeof(Item)));
+    if (at0) HIP_TRY(hipMemcpy(p, dl.buf, (size_t)at0 * sizeof(Item), hipMemcpyDeviceToDevice));
+    if (dl.buf) (void)hipFree(dl.buf);
+    dl.buf = static_cast<Item*>(p);
+    dl.cap = cap;
+  }
+  if (!v.empty())
+    HIP_TRY(hipMemcpy(dl.buf + at0, v.data(), v.size() * sizeof(Item), hipMemcpyHostToDevice));
+  dl.count = need;
+  for (const Item& it : v)
+    if (it.kind >= kBwdFold) dl.lds = std::max(dl.lds, terminalLdsOf(it));
+  return 0;
+}
+
+}  // namespace
+
 // Appends one spot's items to the end of every merged list.
 int ppsg_engine::appendSpot(int slot) {
   Spot& S = spots[slot];
   const int D = S.maxDepth + 1;
   dropGraph();
   for (int tr = 0; tr < 2; ++tr)
-    for (int ps = 0; ps < 2; ++ps) {
+    for (int ps = 0; ps < 4; ++ps) {
       std::vector<DevList>& L = lists[tr][ps];
-      if ((int)L.size() < D) L.resize(D);
-      for (int dep = 0; dep < D; ++dep) {
-        const std::vector<Item>& src = S.items[tr][ps][dep];
-        if (src.empty()) continue;
-        std::vector<Item> v(src);
-        for (Item& it : v) {
-          it.a += S.off;
-          it.b += S.off;
-          it.c += S.off;
-          it.d += S.off;
-          it.f = slot;
-          if (it.kind >= kBwdFold) L[dep].lds = std::max(L[dep].lds, terminalLdsOf(it));
-        }
-        DevList& dl = L[dep];
-        const int64_t need = dl.count + (int64_t)v.size();
-        if (need > dl.cap) {
-          const int64_t cap = std::max<int64_t>(need, dl.cap + dl.cap / 2);
-          void* p = nullptr;
-          HIP_TRY(hipMalloc(&p, (size_t)cap * sizeof(Item)));
-          if (dl.count)
-            HIP_TRY(hipMemcpy(p, dl.buf, (size_t)dl.count * sizeof(Item), hipMemcpyDeviceToDevice));
-          if (dl.buf) (void)hipFree(dl.buf);
-          dl.buf = static_cast<Item*>(p);
-          dl.cap = cap;
-        }
-        HIP_TRY(hipMemcpy(dl.buf + dl.count, v.data(), v.size() * sizeof(Item),
-                          hipMemcpyHostToDevice));
-        dl.count = need;
+      const int nl = ps < 2 ? D : 1;
+      if ((int)L.size() < nl) L.resize(nl);
+      for (int dep = 0; dep < nl; ++dep) {
+        const std::vector<Item> v = spotItems(slot, tr, ps, dep);
+        if (!v.empty() && uploadTo(L[dep], v, true) != 0) return -1;
       }
     }
+  S.itemCount = countItems(slot);
   S.inLists = true;
   listedItems += S.itemCount;
   return 0;
 }
 
-// Rebuilds the merged lists from the live spots (terminal items of every
-// spot first within each forward list) and frees removed spots.
+// Rebuilds the merged lists from the live spots and frees removed spots.
 int ppsg_engine::compact() {
   dropGraph();
-  for (int tr = 0; tr < 2; ++tr)
-    for (int ps = 0; ps < 2; ++ps)
-      for (DevList& l : lists[tr][ps]) {
-        l.count = 0;
-        l.lds = 0;
-      }
   listedItems = 0;
   deadItems = 0;
   int D = 0;
+  bool any = false;
   for (int s = 0; s < maxSpots; ++s) {
     Spot& S = spots[s];
     if (S.state == kDoomed) {
@@ -970,48 +1255,37 @@ int ppsg_engine::compact() {
       continue;
     }
     S.inLists = false;
-    if (S.state == kLive) D = std::max(D, S.maxDepth + 1);
+    if (S.state == kLive) {
+      D = std::max(D, S.maxDepth + 1);
+      any = true;
+    }
   }
   for (int tr = 0; tr < 2; ++tr)
-    for (int ps = 0; ps < 2; ++ps) {
+    for (int ps = 0; ps < 4; ++ps) {
       std::vector<DevList>& L = lists[tr][ps];
-      for (int dep = D; dep < (int)L.size(); ++dep)
+      const int nl = ps < 2 ? D : (any ? 1 : 0);
+      for (int dep = nl; dep < (int)L.size(); ++dep)
         if (L[dep].buf) (void)hipFree(L[dep].buf);
-      L.resize(D);
-      for (int dep = 0; dep < D; ++dep) {
+      L.resize(nl);
+      for (int dep = 0; dep < nl; ++dep) {
         std::vector<Item> v;
-        for (int pass = 0; pass < (ps == 0 ? 2 : 1); ++pass)  // forward: terminals, then the rest
-          for (int s = 0; s < maxSpots; ++s) {
-            const Spot& S = spots[s];
-            if (S.state != kLive || dep > S.maxDepth) continue;
-            for (Item it : S.items[tr][ps][dep]) {
-              if (ps == 0 && (it.kind >= kBwdFold) != (pass == 0)) continue;
-              it.a += S.off;
-              it.b += S.off;
-              it.c += S.off;
-              it.d += S.off;
-              it.f = s;
-              if (it.kind >= kBwdFold) L[dep].lds = std::max(L[dep].lds, terminalLdsOf(it));
-              v.push_back(it);
-            }
-          }
-        DevList& dl = L[dep];
-        if ((int64_t)v.size() > dl.cap) {
-          if (dl.buf) (void)hipFree(dl.buf);
-          dl.buf = nullptr;
-          void* p = nullptr;
-          HIP_TRY(hipMalloc(&p, v.size() * sizeof(Item)));
-          dl.buf = static_cast<Item*>(p);
-          dl.cap = (int64_t)v.size();
+        for (int s = 0; s < maxSpots; ++s) {
+          if (spots[s].state != kLive) continue;
+          const std::vector<Item> w = spotItems(s, tr, ps, dep);
+          v.insert(v.end(), w.begin(), w.end());
         }
-        if (!v.empty())
-          HIP_TRY(hipMemcpy(dl.buf, v.data(), v.size() * sizeof(Item), hipMemcpyHostToDevice));
-        dl.count = (int64_t)v.size();
+        if (ps >= 2)  // largest groups of every spot first
+          std::stable_sort(v.begin(), v.end(), [](const Item& x, const Item& y) {
+            return (int64_t)x.tile * x.m > (int64_t)y.tile * y.m;
+          });
+        L[dep].lds = 0;
+        if (uploadTo(L[dep], v, false) != 0) return -1;
       }
     }
   for (int s = 0; s < maxSpots; ++s)
     if (spots[s].state == kLive) {
       spots[s].inLists = true;
+      spots[s].itemCount = countItems(s);
       listedItems += spots[s].itemCount;
     }
   return 0;
@@ -1062,6 +1336,9 @@ int ppsg_engine::add(const ppsg_tree_desc* t, int* slot) {
         put(L.reach[p] + 4 * t->reach_off[p][0], t->root_reach[p], 4 * (int64_t)t->n_base[1 - p]))
       rc = -1;
   }
+  if (rc == 0 &&
+      put(L.termNodes, T.termNodes.data(), (int64_t)(T.termNodes.size() * sizeof(TermNode))))
+    rc = -1;
   // untrained rows: zero regret and strategy sums (uniform average strategy)
   if (rc == 0 && (hipMemset(base + L.regret, 0, 4 * (size_t)t->reg_total) != hipSuccess ||
                   hipMemset(base + L.strat, 0, 4 * (size_t)t->reg_total) != hipSuccess))
@@ -1156,7 +1433,7 @@ int ppsg_engine::prepare() {
 int64_t ppsg_engine::kernelsPerIter() const {
   int64_t k = 1;  // counter bump
   for (int tr = 0; tr < 2; ++tr)
-    for (int ps = 0; ps < 2; ++ps)
+    for (int ps = 0; ps < 4; ++ps)
       for (const DevList& l : lists[tr][ps]) k += l.count > 0;
   return k;
 }
@@ -1164,7 +1441,17 @@ int64_t ppsg_engine::kernelsPerIter() const {
 int ppsg_engine::launchFwd(int tr, int dpt, int fromStrat, int kindMask, hipStream_t s) {
   const DevList& l = lists[tr][0][dpt];
   if (l.count == 0) return 0;
-  fwdKernel<<<(unsigned)l.count, kBlock, l.lds, s>>>(dev(), l.buf, fromStrat, kindMask, masked());
+  fwdKernel<<<(unsigned)l.count, kBlock, 0, s>>>(dev(), l.buf, fromStrat, kindMask, masked());
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int ppsg_engine::launchTerm(int tr, int ps, hipStream_t s) {
+  if (lists[tr][ps].empty()) return 0;
+  const DevList& l = lists[tr][ps][0];
+  if (l.count == 0) return 0;
+  if (ps == 2) foldKernel<<<(unsigned)l.count, kBlock, l.lds, s>>>(dev(), l.buf, masked());
+  else showdownKernel<<<(unsigned)l.count, kBlock, l.lds, s>>>(dev(), l.buf, masked());
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1186,6 +1473,7 @@ int ppsg_engine::launchHalf(int tr, int mode, hipStream_t s) {
   const int D = depthCount();
   for (int dpt = 0; dpt < D; ++dpt)
     if (launchFwd(tr, dpt, fromStrat, -1, s) != 0) return -1;
+  if (launchTerm(tr, 2, s) != 0 || launchTerm(tr, 3, s) != 0) return -1;
   for (int dpt = D - 1; dpt >= 0; --dpt)
     if (launchBwd(tr, dpt, mode, -1, s) != 0) return -1;
   return 0;
@@ -1460,7 +1748,7 @@ int ppsg_profile(ppsg_t* h, int iters, double* out, int cap) {
   if (h->prepare() != 0) return -1;
   if (h->live.empty()) return fail("no live spot");
   if (h->covers(iters) != 0) return -1;
-  double acc[13] = {0};
+  double acc[17] = {0};
   hipEvent_t a = nullptr, b = nullptr;
   HIP_TRY(hipEventCreate(&a));
   if (hipEventCreate(&b) != hipSuccess) {
@@ -1486,10 +1774,22 @@ int ppsg_profile(ppsg_t* h, int iters, double* out, int cap) {
       for (int dpt = 0; dpt < D && rc == 0; ++dpt) {
         if (!h->lists[tr][0][dpt].count) continue;
         rc = timed([&] { return h->launchFwd(tr, dpt, 0, -1, h->stream); }, 0);
-        // attribution: terminal kinds re-run alone (they only rewrite cfv rows)
-        for (int k = 0; k < 2 && rc == 0 && detail; ++k)
-          rc = timed([&] { return h->launchFwd(tr, dpt, 0, 1 << (kBwdFold + k), h->stream); },
-                     9 + 2 * k);
+        // attribution: forward decide / chance re-run alone (they rewrite the
+        // same reach rows from the same inputs)
+        for (int k = 0; k < 2 && rc == 0 && cap >= 17; ++k)
+          rc = timed([&] { return h->launchFwd(tr, dpt, 0, 1 << (kFwdDecide + k), h->stream); },
+                     13 + 2 * k);
+      }
+      double lastFwd = acc[0];
+      for (int ps = 2; ps < 4 && rc == 0; ++ps) {
+        if (h->lists[tr][ps].empty() || !h->lists[tr][ps][0].count) continue;
+        rc = timed([&] { return h->launchTerm(tr, ps, h->stream); }, 0);
+        // attribution: fold and showdown have launches of their own
+        if (rc == 0 && detail) {
+          acc[9 + 2 * (ps - 2)] += acc[0] - lastFwd;
+          acc[10 + 2 * (ps - 2)] += 1;
+        }
+        lastFwd = acc[0];
       }
       for (int dpt = D - 1; dpt >= 0 && rc == 0; --dpt) {
         if (!h->lists[tr][1][dpt].count) continue;
@@ -1513,7 +1813,7 @@ int ppsg_profile(ppsg_t* h, int iters, double* out, int cap) {
   HIP_TRY(hipStreamSynchronize(h->stream));
   for (int s : h->live) h->spots[s].counter += iters;
   acc[4] = iters;
-  for (int i = 0; i < cap && i < 13; ++i) out[i] = acc[i];
+  for (int i = 0; i < cap && i < 17; ++i) out[i] = acc[i];
   return 0;
 }
 
@@ -1588,6 +1888,55 @@ int ppsg_batch_compact(ppsg_t* h, double max_dead_fraction, int* compacted) {
   const bool go = h->deadItems > 0 && (double)h->deadItems > max_dead_fraction * (double)h->listedItems;
   if (go && h->compact() != 0) return -1;
   if (compacted) *compacted = go ? 1 : 0;
+  return 0;
+}
+
+int ppsg_kind_bytes(ppsg_t* h, double out[6]) {
+  if (!h || !out) return fail("null argument");
+  if (h->prepare() != 0) return -1;
+  for (int k = 0; k < 6; ++k) out[k] = 0.0;
+  for (int s : h->live)
+    for (int k = 0; k < 6; ++k) out[k] += h->spots[s].kindBytes[k];
+  return 0;
+}
+
+int ppsg_tune(ppsg_t* h, int param, int value) {
+  if (!h) return fail("null handle");
+  if (param != PPSG_TUNE_GROUP) return fail("tune: unknown parameter");
+  if (value < 1 || value > 1024) return fail("tune: group size must be 1..1024");
+  if (value == h->groupSize) return 0;
+  h->groupSize = value;
+  return h->compact();
+}
+
+int ppsg_autotune(ppsg_t* h, const int* candidates, int n, int reps, int* chosen, double* ms) {
+  if (!h || (n > 0 && !candidates)) return fail("null argument");
+  if (n <= 0 || reps <= 0) return fail("autotune: nothing to try");
+  if (h->prepare() != 0) return -1;
+  if (h->live.empty()) return fail("no live spot");
+  int best = h->groupSize;
+  double bestMs = 1e300;
+  for (int i = 0; i < n; ++i) {
+    if (ppsg_tune(h, PPSG_TUNE_GROUP, candidates[i]) != 0) return -1;
+    if (h->prepare() != 0) return -1;
+    // EV walks read the strategy sums only: timing them changes no state
+    if (h->launchHalf(0, PPSG_MODE_EV, h->stream) != 0) return -1;  // warm
+    HIP_TRY(hipEventRecord(h->ev0, h->stream));
+    for (int r = 0; r < reps; ++r)
+      for (int tr = 0; tr < 2; ++tr)
+        if (h->launchHalf(tr, PPSG_MODE_EV, h->stream) != 0) return -1;
+    HIP_TRY(hipEventRecord(h->ev1, h->stream));
+    HIP_TRY(hipEventSynchronize(h->ev1));
+    float t = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&t, h->ev0, h->ev1));
+    if (ms) ms[i] = t / reps;
+    if (t < bestMs) {
+      bestMs = t;
+      best = candidates[i];
+    }
+  }
+  if (ppsg_tune(h, PPSG_TUNE_GROUP, best) != 0) return -1;
+  if (chosen) *chosen = best;
   return 0;
 }
 

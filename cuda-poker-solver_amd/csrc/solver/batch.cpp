@@ -336,14 +336,15 @@ std::vector<SpotResult> serve(const std::vector<SpotRequest>& reqs, const ServeO
 }
 
 BatchProfile profileBatch(const std::vector<SpotRequest>& reqs, const std::string& algo, int iters,
-                          int64_t arenaBytes) {
+                          int64_t arenaBytes, bool detail) {
   if (algo != "dcfr" && algo != "dcfr-g2")
     throw std::runtime_error("profile: algo must be dcfr or dcfr-g2");
   if (iters <= 0 || reqs.empty()) throw std::runtime_error("profile: nothing to run");
-  std::vector<float> coef((size_t)2 * iters * 3);
-  for (int t = 0; t < 2 * iters; ++t) {
+  const int tab = (detail ? 3 : 2) * iters;  // replay + launch timing (+ attribution)
+  std::vector<float> coef((size_t)tab * 3);
+  for (int t = 0; t < tab; ++t) {
     double c[3];
-    discountCoefs(t, 2 * iters, algo, c);
+    discountCoefs(t, tab, algo, c);
     for (int k = 0; k < 3; ++k) coef[(size_t)t * 3 + k] = (float)c[k];
   }
   if (arenaBytes <= 0) {
@@ -377,27 +378,40 @@ BatchProfile profileBatch(const std::vector<SpotRequest>& reqs, const std::strin
     const ppsg_tree_desc d = trees[k].desc();
     int slot = -1;
     check(ppsg_batch_add(E.h, &d, &slot), "profile: batch add");
-    check(ppsg_batch_set_schedule(E.h, slot, coef.data(), 2 * iters), "profile: schedule");
+    check(ppsg_batch_set_schedule(E.h, slot, coef.data(), tab), "profile: schedule");
     trees[k] = CompiledTree();
     ++P.spots;
   }
+  P.tuneMs.assign(kNumGroupCandidates, 0.0);
+  check(ppsg_autotune(E.h, kGroupCandidates, kNumGroupCandidates, 2, &P.groupSize, P.tuneMs.data()),
+        "profile: autotune");
   check(ppsg_batch_step(E.h, iters), "profile: replay");
   double ms = 0.0;
   check(ppsg_last_iterate_ms(E.h, &ms), "profile: replay time");
   P.graphMsPerIter = ms / iters;
-  double o[5] = {0};
+  double o[17] = {0};
   check(ppsg_profile(E.h, iters, o, 5), "profile: launches");
   P.fwdMs = o[0];
   P.fwdLaunches = o[1];
   P.bwdMs = o[2];
   P.bwdLaunches = o[3];
   P.iters = iters;
+  if (detail) {
+    check(ppsg_profile(E.h, iters, o, 17), "profile: attribution");
+    P.decideMs = o[5] / iters;
+    P.chanceMs = o[7] / iters;
+    P.foldMs = o[9] / iters;
+    P.showdownMs = o[11] / iters;
+    P.fwdDecideMs = o[13] / iters;
+    P.fwdChanceMs = o[15] / iters;
+  }
   int64_t info[6] = {0};
   check(ppsg_info(E.h, info, 6), "profile: info");
   P.kernelsPerIter = info[0];
   P.itemsPerIter = info[2];
   P.fwdBytes = (double)info[4];
   P.bwdBytes = (double)info[5];
+  check(ppsg_kind_bytes(E.h, P.kindBytes), "profile: bytes by kind");
   return P;
 }
 

@@ -418,7 +418,7 @@ int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const do
 }
 
 int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, const char* algo,
-                      int iters, int64_t arena_bytes, double out[12]) {
+                      int iters, int64_t arena_bytes, int detail, double out[31]) {
   return guard([&] {
     if (n <= 0 || !spots || !cfgs || !out) throw std::runtime_error("null argument");
     std::vector<pps::gpu::SpotRequest> reqs(n);
@@ -427,11 +427,17 @@ int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, 
       reqs[i].cfg = toCfg(cfgs + i);
     }
     const pps::gpu::BatchProfile P =
-        pps::gpu::profileBatch(reqs, algo ? algo : "dcfr", iters, arena_bytes);
-    const double v[12] = {(double)P.spots, (double)P.iters, P.graphMsPerIter, P.fwdMs,
+        pps::gpu::profileBatch(reqs, algo ? algo : "dcfr", iters, arena_bytes, detail != 0);
+    const double v[16] = {(double)P.spots, (double)P.iters, P.graphMsPerIter, P.fwdMs,
                           P.fwdLaunches, P.bwdMs, P.bwdLaunches, P.fwdBytes, P.bwdBytes,
-                          (double)P.kernelsPerIter, (double)P.itemsPerIter, (double)P.arenaBytes};
-    for (int i = 0; i < 12; ++i) out[i] = v[i];
+                          (double)P.kernelsPerIter, (double)P.itemsPerIter, (double)P.arenaBytes,
+                          P.decideMs, P.chanceMs, P.foldMs, P.showdownMs};
+    for (int i = 0; i < 16; ++i) out[i] = v[i];
+    out[16] = P.groupSize;
+    for (int i = 0; i < pps::gpu::kNumGroupCandidates; ++i) out[17 + i] = P.tuneMs[i];
+    out[23] = P.fwdDecideMs;
+    out[24] = P.fwdChanceMs;
+    for (int k = 0; k < 6; ++k) out[25 + k] = P.kindBytes[k];
     return 0;
   });
 }

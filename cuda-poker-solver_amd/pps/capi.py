@@ -74,7 +74,7 @@ PPSG_SYMBOLS = [
     "ppsg_batch_set_active", "ppsg_batch_set_schedule", "ppsg_batch_set_counter",
     "ppsg_batch_get_counter", "ppsg_batch_reset", "ppsg_batch_step", "ppsg_batch_walk",
     "ppsg_batch_read", "ppsg_batch_write", "ppsg_batch_free_bytes", "ppsg_batch_compact",
-    "ppsg_device_memory",
+    "ppsg_device_memory", "ppsg_tune", "ppsg_autotune", "ppsg_kind_bytes",
 ]
 
 _lib = None
@@ -138,7 +138,8 @@ def lib():
                                 ctypes.c_char_p, P(ServeOptions), P(ServeResult), P(ServeReport)]
         L.pps_compile_footprint.argtypes = [P(Spot), P(BetConfig), P(ctypes.c_int64)]
         L.pps_batch_profile.argtypes = [ctypes.c_int, P(Spot), P(BetConfig), ctypes.c_char_p,
-                                        ctypes.c_int, ctypes.c_int64, P(ctypes.c_double)]
+                                        ctypes.c_int, ctypes.c_int64, ctypes.c_int,
+                                        P(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -152,6 +153,9 @@ def copy_probe(nbytes, reps):
     if L.ppsg_copy_probe(int(nbytes), int(reps), ctypes.byref(ms)) != 0:
         raise CapiError(L.ppsg_last_error().decode())
     return ms.value
+
+
+GROUP_CANDIDATES = [1, 2, 4, 8, 16, 32]  # terminal group sizes the autotuner tries
 
 
 class CapiError(RuntimeError):
@@ -397,14 +401,25 @@ def serve(spots, cfgs, algo="dcfr", arrival_ms=None, **options):
     return out, report
 
 
-def batch_profile(spots, cfgs, iters, algo="dcfr", arena_bytes=0):
+def batch_profile(spots, cfgs, iters, algo="dcfr", arena_bytes=0, detail=False):
     """Launch-level timing of the spots resident in one arena (pps_batch_profile)."""
     n = len(spots)
     S = (Spot * n)(*spots)
     C = (BetConfig * n)(*cfgs)
-    out = (ctypes.c_double * 12)()
-    _check(lib().pps_batch_profile(n, S, C, algo.encode(), int(iters), int(arena_bytes), out))
+    out = (ctypes.c_double * 31)()
+    _check(lib().pps_batch_profile(n, S, C, algo.encode(), int(iters), int(arena_bytes),
+                                   1 if detail else 0, out))
     keys = ["spots", "iters", "graph_ms_per_iter", "fwd_ms", "fwd_launches", "bwd_ms",
             "bwd_launches", "fwd_bytes", "bwd_bytes", "kernels_per_iter", "items_per_iter",
-            "arena_bytes"]
-    return dict(zip(keys, list(out)))
+            "arena_bytes", "decide_ms", "chance_ms", "fold_ms", "showdown_ms", "group_size"]
+    d = dict(zip(keys, list(out)))
+    d["group_size"] = int(d["group_size"])
+    d["autotune_ms"] = {g: round(out[17 + i], 4) for i, g in enumerate(GROUP_CANDIDATES)}
+    d["fwd_decide_ms"], d["fwd_chance_ms"] = out[23], out[24]
+    d["kind_bytes"] = dict(zip(["fwd_decide", "fwd_chance", "bwd_decide", "bwd_chance", "fold",
+                                "showdown"], list(out)[25:31]))
+    if not detail:
+        for k in ("decide_ms", "chance_ms", "fold_ms", "showdown_ms", "fwd_decide_ms",
+                  "fwd_chance_ms"):
+            del d[k]
+    return d

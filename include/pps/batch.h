@@ -83,9 +83,23 @@ struct BatchProfile {
   double fwdMs = 0.0, fwdLaunches = 0.0, bwdMs = 0.0, bwdLaunches = 0.0;
   double fwdBytes = 0.0, bwdBytes = 0.0;
   int64_t kernelsPerIter = 0, itemsPerIter = 0, arenaBytes = 0;
+  // detail: ms per iteration of each node kind (backward decide / chance
+  // and forward decide / chance re-run alone; fold and showdown launches)
+  double decideMs = 0.0, chanceMs = 0.0, foldMs = 0.0, showdownMs = 0.0;
+  double fwdDecideMs = 0.0, fwdChanceMs = 0.0;
+  // algorithmic bytes per batched iteration by kind (as ppsg_kind_bytes)
+  double kindBytes[6] = {0, 0, 0, 0, 0, 0};
+  // terminal group size chosen by the autotuner, and the EV-walk time (ms
+  // per both-traverser walk) of each candidate in kGroupCandidates order
+  int groupSize = 0;
+  std::vector<double> tuneMs;
 };
+
+// Terminal group sizes the autotuner tries.
+constexpr int kGroupCandidates[] = {1, 2, 4, 8, 16, 32};
+constexpr int kNumGroupCandidates = 6;
 BatchProfile profileBatch(const std::vector<SpotRequest>& requests, const std::string& algo,
-                          int iters, int64_t arenaBytes = 0);
+                          int iters, int64_t arenaBytes = 0, bool detail = false);
 
 }  // namespace gpu
 }  // namespace pps

@@ -160,9 +160,15 @@ int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const do
    once): out = [spots, iters, graph ms per batched iteration, forward ms,
    forward launches, backward ms, backward launches, forward bytes and
    backward bytes per batched iteration (algorithmic f32 rows), kernels
-   per iteration, work items per iteration, arena bytes]. */
+   per iteration, work items per iteration, arena bytes, then with
+   detail != 0 the ms per iteration of each node kind run alone: backward
+   decide, backward chance, fold, showdown; then the terminal group size
+   the autotuner chose and its EV-walk ms for group sizes 1, 2, 4, 8, 16,
+   32; then (detail) forward decide, forward chance; then the algorithmic
+   bytes per batched iteration of forward decide, forward chance, backward
+   decide, backward chance, fold, showdown]. */
 int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, const char* algo,
-                      int iters, int64_t arena_bytes, double out[12]);
+                      int iters, int64_t arena_bytes, int detail, double out[31]);
 /* Device bytes the compiled spot takes in an arena (host only). */
 int pps_compile_footprint(const pps_spot* spot, const pps_bet_config* cfg, int64_t* bytes);
 

@@ -28,6 +28,7 @@ extern "C" {
 enum { PPSG_DECIDE = 0, PPSG_SHOWDOWN = 1, PPSG_FOLD = 2, PPSG_CHANCE = 3 };
 enum { PPSG_MODE_CFR = 0, PPSG_MODE_EV = 1, PPSG_MODE_BR = 2 };
 enum { PPSG_REGRET = 0, PPSG_STRAT = 1, PPSG_REACH0 = 2, PPSG_REACH1 = 3, PPSG_CFV = 4 };
+enum { PPSG_TUNE_GROUP = 0 };
 
 typedef struct ppsg_tree_desc {
   int32_t n_base[2];               /* base list sizes (root class lists) */
@@ -102,20 +103,23 @@ int ppsg_read(ppsg_t* h, int buf, int64_t off, int64_t n, float* host);
 int ppsg_write(ppsg_t* h, int buf, int64_t off, int64_t n, const float* host);
 /* Device time (ms) of the last ppsg_iterate, from events on its stream. */
 int ppsg_last_iterate_ms(ppsg_t* h, double* ms);
-/* Engine facts: [0] kernels per iteration, [1] device bytes allocated,
+/* Engine facts: [0] kernels per iteration, [1] arena bytes,
    [2] work items per iteration, [3] graph captured (0/1), [4] forward and
    [5] backward algorithmic f32 row bytes per iteration (fold and showdown
-   rows count as forward: they are computed in the forward launches). */
+   rows count as forward: one terminal launch per half-step evaluates
+   them after the forward reach launches). */
 int ppsg_info(ppsg_t* h, int64_t* out, int cap);
 /* Runs `iters` CFR iterations launch by launch (no graph), timing every
    launch with events on the engine stream: out = [forward ms, forward
-   launches, backward ms, backward launches, iterations]. Advances the
-   counter like ppsg_iterate. Fold and showdown values are computed in the
-   forward launch of their depth. With cap >= 13 every launch is also re-run
-   restricted to one node kind: out[5..6] backward decide, [7..8] backward
-   chance, [9..10] fold, [11..12] showdown (ms, launches); the decide re-runs
-   apply the updates again, so use it for attribution only. (No reference counterpart; the reference's
-   GPU_CFR_PROFILE mode plays this role, cuda/gpu_cfr.cu:1895-1987.) */
+   launches, backward ms, backward launches, iterations]; the forward
+   figures include the terminal launch. Advances the counter like
+   ppsg_iterate. With cap >= 13 the terminal and backward launches are also
+   re-run restricted to one node kind: out[5..6] backward decide, [7..8]
+   backward chance, [9..10] fold, [11..12] showdown (ms, launches); with
+   cap >= 17 also [13..14] forward decide, [15..16] forward chance. The
+   decide re-runs apply the updates again, so use it for attribution only.
+   (No reference counterpart; the reference's GPU_CFR_PROFILE mode plays
+   this role, cuda/gpu_cfr.cu:1895-1987.) */
 int ppsg_profile(ppsg_t* h, int iters, double* out, int cap);
 /* Calibration for hardware byte counters (no reference counterpart): a
    plain device copy of `bytes` (multiple of 16), `reps` launches on the
@@ -168,6 +172,20 @@ int ppsg_batch_free_bytes(ppsg_t* h, int64_t* total, int64_t* largest);
    slots and arena blocks) when they exceed max_dead_fraction of all listed
    items; *compacted = 1 if it did (may be NULL). */
 int ppsg_batch_compact(ppsg_t* h, double max_dead_fraction, int* compacted);
+/* Algorithmic f32 row bytes of one iteration of the live spots by node
+   kind: forward decide, forward chance, backward decide, backward chance,
+   fold, showdown. */
+int ppsg_kind_bytes(ppsg_t* h, double out[6]);
+/* Launch shape (no reference counterpart; results do not depend on it).
+   PPSG_TUNE_GROUP: fold/showdown nodes of one board class evaluated by one
+   workgroup (1..1024; default 4). Larger groups load a class's gather list
+   and per-combo metadata once for more nodes and launch fewer workgroups;
+   smaller groups spread a small batch over more CUs. */
+int ppsg_tune(ppsg_t* h, int param, int value);
+/* Times n candidate group sizes on the live spots (reps EV walks of both
+   traversers each, which change no rows), keeps the fastest; *chosen and
+   ms[i] (per rep) may be NULL. */
+int ppsg_autotune(ppsg_t* h, const int* candidates, int n, int reps, int* chosen, double* ms);
 /* Free and total memory of the current device (for sizing an arena). */
 int ppsg_device_memory(int64_t* free_bytes, int64_t* total_bytes);
 
