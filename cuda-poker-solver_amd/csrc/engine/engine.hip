@@ -47,11 +47,15 @@
 namespace {
 
 constexpr int kBlock = 256;  // 4 wave64s
+constexpr int kWideBlock = 512;  // wide showdown groups: 8 wave64s
 constexpr int kTile = 256;   // combos per tile item
 constexpr int kMaxList = 1326;
 constexpr int kRowRegs = (kMaxList + kBlock - 1) / kBlock;  // 6
 constexpr int64_t kAlign = 256;
+constexpr int kLists = 5;  // per traverser: forward, backward, fold, showdown, wide showdown
 constexpr int kGatherHeader = 256;  // bytes: 53 int32 segment bounds + pad
+// fold: entries of a card's segment per lane (4 lanes per card)
+constexpr int kFoldPart = 12;
 // combos per lane (kTile apart) of decide and chance items
 constexpr int kRDecide = 2;
 constexpr int kRChance = 4;
@@ -105,10 +109,11 @@ enum ItemKind : uint8_t {
 //                traverser combos (parent), na branches
 //   kBwdFold /   a group of fold (or showdown) nodes of one board class:
 //   kBwdShowdown a the group's first TermNode, cnt nodes in the group, b the
-//                spot's chunk (TermNode offsets are relative to it), c
-//                showdown: the class's gather block (53 i32 segment bounds,
-//                then the u16 gather list); fold: the opponent's u32 cards
-//                per combo, d per traverser combo metadata (fold: u32 cards
+//                spot's chunk (TermNode offsets are relative to it), c the
+//                opponent's gather block of the class (53 i32 per-card
+//                segment bounds, then the u16 gather list: showdown, slots
+//                in strength order + the per-card segments; fold, the
+//                per-card segments), d per traverser combo metadata (fold: u32 cards
 //                + same; showdown: 4 u32), e gather entries per lane
 //                (showdown: 8, 12 or 16), na row registers per lane (3 or
 //                6), n traverser combos, m opponent combos
@@ -228,111 +233,124 @@ __device__ __forceinline__ void fwdItem(const Dev& d, const Item& it, int fromSt
 // ---------------------------------------------------------------------------
 // Fold and showdown terminals, one workgroup per group of nodes that share a
 // board class (and so the class's index tables, which stay in registers
-// across the group). The next node's reach row is loaded while the current
-// one is evaluated.
+// across the group).
 // ---------------------------------------------------------------------------
 
-// Fixed point for the fold card sums: reach (<= 1 per combo, <= 1326
-// combos) times 2^46 fits an int64 with room to spare; integer sums are
-// exact in any order, so LDS atomics stay deterministic.
-constexpr float kFixScale = 70368744177664.0f;  // 2^46
-constexpr double kFixInv = 1.0 / 70368744177664.0;
+// Showdown prefix sums (blocked scan over gathered f32 reach).
+using SdAcc = double;
 
-__device__ __forceinline__ long long toFix(float r) { return __float2ll_rn(r * kFixScale); }
+// Loads node g's opponent reach row (ROWS floats per lane, NT lanes) when
+// g < cnt.
+template <int ROWS, int NT = kBlock>
+__device__ __forceinline__ void loadReach(float (&rv)[ROWS], const char* base,
+                                          const TermNode* __restrict__ nodes, int g, int cnt,
+                                          int nO) {
+  if (g >= cnt) return;
+  const float* __restrict__ r = reinterpret_cast<const float*>(base + nodes[g].reach);
+#pragma unroll
+  for (int q = 0; q < ROWS; ++q) {
+    const int j = (int)threadIdx.x + q * NT;
+    rv[q] = j < nO ? r[j] : 0.0f;
+  }
+}
 
-// Fold group: mass(i) = total - cardSum[c1] - cardSum[c2] + r[same(i)],
-// with the 52 card sums accumulated by fixed-point LDS atomics (each
-// opponent combo adds its reach to its two cards; the total is half the
-// sum of the card sums).
+// Fold group: mass(i) = total - cardSum[c1] - cardSum[c2] + r[same(i)].
+// The class's per-card segments (opponent slots holding card c) are split
+// over 4 lanes per card (lanes 4c..4c+3, up to kFoldPart slots each, padded
+// with the zero sentinel slot); each lane sums its part in f64, the 4 parts
+// combine pairwise, and the total is half the sum of the card sums. The
+// next node's reach row is requested as soon as this one is staged.
 template <int ROWS>
 __device__ void foldGroup(const Dev& d, const Item& it, char* lds) {
-  __shared__ long long cs[53];
-  const int tid = threadIdx.x;
-  const int nO = it.m, nT = it.n;
+  __shared__ double cs[53];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int nO = it.m, nT = it.n, cnt = it.cnt;
   float* raw = reinterpret_cast<float*>(lds);
   const TermNode* __restrict__ nodes = at<TermNode>(d, it.a);
   const char* base = d.arena + it.b;
-  uint32_t oc[ROWS], tc[ROWS];
+  uint32_t tc[ROWS];
 #pragma unroll
   for (int q = 0; q < ROWS; ++q) {
-    const int j = tid + q * kBlock;
-    oc[q] = j < nO ? at<uint32_t>(d, it.c)[j] : 0u;
-    tc[q] = j < nT ? at<uint32_t>(d, it.d)[j] : 0u;
+    const int i = tid + q * kBlock;
+    tc[q] = i < nT ? at<uint32_t>(d, it.d)[i] : 0u;
   }
-  float rv[ROWS];
+  // this lane's part of its card's segment
+  uint16_t idx[kFoldPart];
   {
-    const float* __restrict__ r = reinterpret_cast<const float*>(base + nodes[0].reach);
-#pragma unroll
-    for (int q = 0; q < ROWS; ++q) {
-      const int j = tid + q * kBlock;
-      rv[q] = j < nO ? r[j] : 0.0f;
+    const int c = tid >> 2, part = tid & 3;
+    int lo = 0, len = 0;
+    if (c < 52) {
+      const int32_t* so = at<int32_t>(d, it.c);
+      const int s0 = so[c], sl = so[c + 1] - s0;
+      lo = s0 + (sl * part) / 4;
+      len = s0 + (sl * (part + 1)) / 4 - lo;
     }
+    const uint16_t* list = at<uint16_t>(d, it.c + kGatherHeader);
+#pragma unroll
+    for (int k = 0; k < kFoldPart; ++k) idx[k] = k < len ? list[lo + k] : (uint16_t)nO;
   }
-  for (int g = 0; g < it.cnt; ++g) {
+  auto node = [&](int g, float(&rv)[ROWS]) {
     const TermNode nd = nodes[g];
     if (g > 0) __syncthreads();  // the previous node's queries are done
-    if (tid < 53) cs[tid] = 0;
 #pragma unroll
     for (int q = 0; q < ROWS; ++q) {
       const int j = tid + q * kBlock;
       if (j < nO) raw[j] = rv[q];
     }
+    if (tid == 0) raw[nO] = 0.0f;
     __syncthreads();
+    loadReach<ROWS>(rv, base, nodes, g + 1, cnt, nO);  // the next node's reach
+    double part = 0.0;
 #pragma unroll
-    for (int q = 0; q < ROWS; ++q) {
-      const int j = tid + q * kBlock;
-      if (j < nO) {
-        const unsigned long long v = (unsigned long long)toFix(rv[q]);
-        atomicAdd(reinterpret_cast<unsigned long long*>(&cs[oc[q] & 0xff]), v);
-        atomicAdd(reinterpret_cast<unsigned long long*>(&cs[(oc[q] >> 8) & 0xff]), v);
-      }
-    }
-    if (g + 1 < it.cnt) {  // prefetch the next node's reach
-      const float* __restrict__ r = reinterpret_cast<const float*>(base + nodes[g + 1].reach);
-#pragma unroll
-      for (int q = 0; q < ROWS; ++q) {
-        const int j = tid + q * kBlock;
-        rv[q] = j < nO ? r[j] : 0.0f;
-      }
-    }
+    for (int k = 0; k < kFoldPart; ++k) part += (double)raw[idx[k]];
+    part += __shfl_xor(part, 1, 64);
+    part += __shfl_xor(part, 2, 64);
+    if ((tid & 3) == 0 && (tid >> 2) < 52) cs[tid >> 2] = part;
     __syncthreads();
-    if (tid < 64) {  // total = half the sum of the card sums (exact)
-      long long x = tid < 52 ? cs[tid] : 0;
+    if (tid < 64) {  // total = half the sum of the card sums
+      double x = tid < 52 ? cs[tid] : 0.0;
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-      if (tid == 0) cs[52] = x / 2;
+      if (lane == 0) cs[52] = 0.5 * x;
     }
     __syncthreads();
     float* __restrict__ out = reinterpret_cast<float*>(d.arena + it.b + nd.out);
-    const long long tot = cs[52];
-    const double v = (double)nd.v0 * kFixInv;
+    const double tot = cs[52];
+    const double v = nd.v0;
 #pragma unroll
     for (int q = 0; q < ROWS; ++q) {
       const int i = tid + q * kBlock;
       if (i >= nT) continue;
       const int c1 = tc[q] & 0xff, c2 = (tc[q] >> 8) & 0xff;
       const int same = (int)(tc[q] >> 16) - 1;
-      long long m = tot - cs[c1] - cs[c2];
-      if (same >= 0) m += toFix(raw[same]);
-      out[i] = (float)((double)m * v);
+      double m = tot - cs[c1] - cs[c2];
+      if (same >= 0) m += (double)raw[same];
+      if (m < 0.0) m = 0.0;
+      out[i] = (float)(m * v);
     }
-  }
+  };
+  float rv[ROWS];
+  loadReach<ROWS>(rv, base, nodes, 0, cnt, nO);
+  for (int g = 0; g < cnt; ++g) node(g, rv);
 }
 
 // Showdown group. Per node: opponent reach is staged in LDS, then each lane
 // gathers its PER consecutive entries of the class's gather list (slots in
 // strength order, then the per-card segments in strength order) and one
-// blocked f64 scan turns them into prefix sums X. Per traverser combo the
+// blocked scan turns them into prefix sums X. Per traverser combo the
 // weaker / weaker-or-equal / all masses come from X at le, te, n and at the
-// compile-time cut points of its two cards' segments.
-template <int PER, int ROWS>
+// compile-time cut points of its two cards' segments. The next node's reach
+// row is requested as soon as this one is staged.
+template <int NT, int PER, int ROWS>
 __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
-  __shared__ double waveTot[kBlock / 64];
+  using Acc = SdAcc;
+  __shared__ Acc waveTot[NT / 64];
+  __shared__ Acc zt[52][2];  // per card: prefix at its segment start, segment total
   __shared__ int so[53];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nO = it.m, nT = it.n;
+  const int nO = it.m, nT = it.n, cnt = it.cnt;
   float* raw = reinterpret_cast<float*>(lds);
-  double* X = reinterpret_cast<double*>(lds + (((nO + 1) * 4 + 15) & ~15));
+  Acc* X = reinterpret_cast<Acc*>(lds + (((nO + 1) * 4 + 15) & ~15));
   const TermNode* __restrict__ nodes = at<TermNode>(d, it.a);
   const char* base = d.arena + it.b;
 
@@ -344,78 +362,71 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
   uint4 meta[ROWS];
 #pragma unroll
   for (int q = 0; q < ROWS; ++q) {
-    const int i = tid + q * kBlock;
+    const int i = tid + q * NT;
     meta[q] = i < nT ? at<uint4>(d, it.d)[i] : make_uint4(0, 0, 0, 0);
   }
   if (tid < 53) so[tid] = at<int32_t>(d, it.c)[tid];
-  float rv[ROWS];
-  {
-    const float* __restrict__ r = reinterpret_cast<const float*>(base + nodes[0].reach);
-#pragma unroll
-    for (int q = 0; q < ROWS; ++q) {
-      const int j = tid + q * kBlock;
-      rv[q] = j < nO ? r[j] : 0.0f;
-    }
-  }
 
-  for (int g = 0; g < it.cnt; ++g) {
+  auto node = [&](int g, float(&rv)[ROWS]) {
     const TermNode nd = nodes[g];
     if (g > 0) __syncthreads();  // the previous node's queries are done with raw / X
     // 1. stage reach (raw[nO] = 0 is the padding sentinel)
 #pragma unroll
     for (int q = 0; q < ROWS; ++q) {
-      const int j = tid + q * kBlock;
+      const int j = tid + q * NT;
       if (j < nO) raw[j] = rv[q];
     }
     if (tid == 0) raw[nO] = 0.0f;
     __syncthreads();
-    if (g + 1 < it.cnt) {  // prefetch the next node's reach
-      const float* __restrict__ r = reinterpret_cast<const float*>(base + nodes[g + 1].reach);
-#pragma unroll
-      for (int q = 0; q < ROWS; ++q) {
-        const int j = tid + q * kBlock;
-        rv[q] = j < nO ? r[j] : 0.0f;
-      }
-    }
+    loadReach<ROWS, NT>(rv, base, nodes, g + 1, cnt, nO);  // the next node's reach
 
-    // 2. gather + blocked exclusive scan (f64); the lane's entries are summed
-    // once for the scan and re-read from LDS when the prefixes are written
-    double own = 0.0;
-#pragma unroll
-    for (int k = 0; k < PER / 4; ++k) {
-      const uint32_t w[2] = {gi[k].x, gi[k].y};
-#pragma unroll
-      for (int h = 0; h < 4; ++h) own += (double)raw[(w[h >> 1] >> ((h & 1) * 16)) & 0xffffu];
-    }
-    double inc = own;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const double t = __shfl_up(inc, off, 64);
-      if (lane >= off) inc += t;
-    }
-    if (lane == 63) waveTot[wave] = inc;
-    __syncthreads();
-    double run = inc - own;
-    for (int w = 0; w < wave; ++w) run += waveTot[w];
+    // 2. gather + blocked exclusive scan. X holds the exclusive prefix at
+    // every list position p (lane p / PER, entry p % PER), stored entry-major
+    // (X[(p % PER) * NT + p / PER]) so that a wave's writes of one entry are
+    // consecutive; the total sits at X[PER * NT].
+    float gv[PER];
+    Acc own = 0;
 #pragma unroll
     for (int k = 0; k < PER / 4; ++k) {
       const uint32_t w[2] = {gi[k].x, gi[k].y};
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
-        X[tid * PER + k * 4 + h] = run;
-        run += (double)raw[(w[h >> 1] >> ((h & 1) * 16)) & 0xffffu];
+        gv[k * 4 + h] = raw[(w[h >> 1] >> ((h & 1) * 16)) & 0xffffu];
+        own += (Acc)gv[k * 4 + h];
       }
     }
-    if (tid == kBlock - 1) X[kBlock * PER] = run;
+    Acc inc = own;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const Acc t = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += t;
+    }
+    if (lane == 63) waveTot[wave] = inc;
+    __syncthreads();
+    Acc run = inc - own;
+    for (int w = 0; w < wave; ++w) run += waveTot[w];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      X[k * NT + tid] = run;
+      run += (Acc)gv[k];
+    }
+    if (tid == NT - 1) X[PER * NT] = run;
+    __syncthreads();
+    auto at = [&](int p) -> Acc { return X[p == PER * NT ? p : (p % PER) * NT + p / PER]; };
+    if (tid < 52) {
+      const Acc z = at(nO + so[tid]);
+      zt[tid][0] = z;
+      zt[tid][1] = at(nO + so[tid + 1]) - z;
+    }
     __syncthreads();
 
     // 3. per traverser combo
     float* __restrict__ out = reinterpret_cast<float*>(d.arena + it.b + nd.out);
-    const double* Y = X + nO;  // per-card segment prefix (offset cancels)
-    const double vWin = nd.v0, vTie = nd.v1, vLose = nd.v2;
+    const Acc vWin = nd.v0, vTie = nd.v1, vLose = nd.v2;
+    const Acc total = at(nO);
 #pragma unroll
     for (int q = 0; q < ROWS; ++q) {
-      const int i = tid + q * kBlock;
+      const int i = tid + q * NT;
       if (i >= nT) continue;
       const uint4 m = meta[q];
       const int le = m.x & 0xffff, te = m.x >> 16;
@@ -423,19 +434,22 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
       const int a2 = m.z & 0xffff, b2 = m.z >> 16;
       const int c1 = m.w & 0xff, c2 = (m.w >> 8) & 0xff;
       const int same = (int)(m.w >> 16) - 1;
-      const double z1 = Y[so[c1]], z2 = Y[so[c2]];
-      const double rs = same >= 0 ? (double)raw[same] : 0.0;
-      double win = X[le] - (Y[a1] - z1) - (Y[a2] - z2);
-      const double leq = X[te] - (Y[b1] - z1) - (Y[b2] - z2) + rs;
-      const double all = X[nO] - (Y[so[c1 + 1]] - z1) - (Y[so[c2 + 1]] - z2) + rs;
-      if (win < 0.0) win = 0.0;
-      double tie = leq - win;
-      if (tie < 0.0) tie = 0.0;
-      double lose = all - leq;
-      if (lose < 0.0) lose = 0.0;
+      const Acc z1 = zt[c1][0], t1 = zt[c1][1], z2 = zt[c2][0], t2 = zt[c2][1];
+      const Acc rs = same >= 0 ? (Acc)raw[same] : (Acc)0;
+      Acc win = at(le) - (at(nO + a1) - z1) - (at(nO + a2) - z2);
+      const Acc leq = at(te) - (at(nO + b1) - z1) - (at(nO + b2) - z2) + rs;
+      const Acc all = total - t1 - t2 + rs;
+      if (win < 0) win = 0;
+      Acc tie = leq - win;
+      if (tie < 0) tie = 0;
+      Acc lose = all - leq;
+      if (lose < 0) lose = 0;
       out[i] = (float)(win * vWin + tie * vTie + lose * vLose);
     }
-  }
+  };
+  float rv[ROWS];
+  loadReach<ROWS, NT>(rv, base, nodes, 0, cnt, nO);
+  for (int g = 0; g < cnt; ++g) node(g, rv);
 }
 
 // Terminal launches of a half-step (after the forward pass has produced all
@@ -451,20 +465,26 @@ __global__ __launch_bounds__(kBlock) void foldKernel(Dev d, const Item* __restri
   else foldGroup<kRowRegs>(d, it, lds);
 }
 
+// Showdown groups of classes with at most 768 combos per side and 3072
+// gather entries: 256 lanes, 3 combos and 8 or 12 gather entries each.
 __global__ __launch_bounds__(kBlock) void showdownKernel(Dev d, const Item* __restrict__ items,
                                                          int masked) {
   extern __shared__ __align__(16) char lds[];
   const Item it = items[blockIdx.x];
   if (masked && !d.ctl[it.f].alive) return;
-  if (it.na == 3) {
-    if (it.e == 8) showdownGroup<8, 3>(d, it, lds);
-    else if (it.e == 12) showdownGroup<12, 3>(d, it, lds);
-    else showdownGroup<16, 3>(d, it, lds);
-  } else {
-    if (it.e == 8) showdownGroup<8, kRowRegs>(d, it, lds);
-    else if (it.e == 12) showdownGroup<12, kRowRegs>(d, it, lds);
-    else showdownGroup<16, kRowRegs>(d, it, lds);
-  }
+  if (it.e == 8) showdownGroup<kBlock, 8, 3>(d, it, lds);
+  else showdownGroup<kBlock, 12, 3>(d, it, lds);
+}
+
+// The larger classes (up to 1326 combos per side): 512 lanes, 3 combos and
+// 8 gather entries each.
+__global__ __launch_bounds__(kWideBlock) void showdownWideKernel(Dev d,
+                                                                 const Item* __restrict__ items,
+                                                                 int masked) {
+  extern __shared__ __align__(16) char lds[];
+  const Item it = items[blockIdx.x];
+  if (masked && !d.ctl[it.f].alive) return;
+  showdownGroup<kWideBlock, 8, 3>(d, it, lds);
 }
 
 // Forward launch of one depth: opponent reach of the next depth.
@@ -730,39 +750,49 @@ struct Layout {
 // four 8-byte index loads).
 int perLane(int len) { return len <= kBlock * 8 ? 8 : len <= kBlock * 12 ? 12 : 16; }
 
+// Showdown group shape: the 256-lane kernel takes classes with at most
+// 3 * kBlock combos per side and 12 gather entries per lane; the rest go
+// to the 512-lane kernel (3 combos, 8 gather entries per lane).
+bool wideShowdown(int nT, int nO) { return std::max(nT, nO) > 3 * kBlock || 3 * nO > 12 * kBlock; }
+
 // Dynamic LDS of a terminal item: reach (+ sentinel), and for showdown the
 // f64 prefixes.
-size_t terminalLdsOf(const Item& it) {
+size_t terminalLdsOf(const Item& it, int lanes) {
   const size_t reach = (size_t)(((it.m + 1) * 4 + 15) & ~15);
-  return it.kind == kBwdShowdown ? reach + (size_t)(kBlock * it.e + 1) * 8 : reach;
+  return it.kind == kBwdShowdown ? reach + (size_t)(lanes * it.e + 1) * sizeof(SdAcc) : reach;
 }
 
 // Host-side tables of one spot, built from its description.
 struct SpotTables {
   std::vector<int16_t> bmap[2];       // per parent class: [branch][parent slot]
   std::map<int, int64_t> bmapOff[2];  // class -> element offset
-  // per showdown class: [53 i32 segment bounds | pad to 256 B | u16 gather
-  // list (strength order, then per-card segments) padded to kBlock * per
-  // with the sentinel slot n]
-  std::vector<char> gather[2];
-  std::map<int, int64_t> sdOff[2];
+  std::vector<char> gather[2];  // per class and terminal kind, see gatherBlock
+  std::map<int, int64_t> sdOff[2], foldOff[2];
   std::map<int, int> sdPer[2];
   std::vector<uint32_t> cardSame[2];  // per list entry: c1 | c2 << 8 | (same+1) << 16
   std::vector<TermNode> termNodes;    // grouped: each group's nodes are contiguous
 
-  void gatherBlock(const ppsg_tree_desc* t, int p, int cl) {
-    auto& offs = sdOff[p];
+  // showdown: [53 i32 segment bounds | pad to 256 B | u16 list: slots in
+  // strength order, then the per-card segments, padded to kBlock * per with
+  // the sentinel slot n]; fold: [53 i32 segment bounds | pad | u16 per-card
+  // segments]
+  void gatherBlock(const ppsg_tree_desc* t, int p, int cl, bool sd) {
+    auto& offs = sd ? sdOff[p] : foldOff[p];
     if (offs.count(cl)) return;
     const int n = t->cls_n[p][cl];
     const int64_t lo = t->cls_list_off[p][cl];
-    std::vector<uint16_t> list(t->sorted[p] + lo, t->sorted[p] + lo + n);
+    std::vector<uint16_t> list;
+    if (sd) list.assign(t->sorted[p] + lo, t->sorted[p] + lo + n);
     list.insert(list.end(), t->seg_slot[p] + 2 * lo, t->seg_slot[p] + 2 * lo + 2 * n);
-    const int per = perLane((int)list.size());
-    list.resize((size_t)kBlock * per, (uint16_t)n);
+    if (sd) {
+      // padded for either showdown shape (kBlock * 8 or 12, kWideBlock * 8)
+      const int per = perLane((int)list.size());
+      list.resize(std::max<size_t>((size_t)kBlock * per, (size_t)kWideBlock * 8), (uint16_t)n);
+      sdPer[p][cl] = per;
+    }
     std::vector<char>& g = gather[p];
     g.resize((g.size() + 15) / 16 * 16, 0);
     offs[cl] = (int64_t)g.size();
-    sdPer[p][cl] = per;
     const size_t at0 = g.size();
     g.resize(at0 + kGatherHeader + list.size() * 2, 0);
     std::memcpy(g.data() + at0, t->seg_off[p] + (int64_t)cl * 53, 53 * 4);
@@ -777,6 +807,7 @@ enum SpotState { kFree = 0, kLive = 1, kParked = 2, kDoomed = 3 };
 struct TermGroup {
   uint8_t kind = 0;
   uint8_t rows = 0;
+  bool wide = false;  // showdown: the 512-lane kernel
   int per = 8, nT = 0, nO = 0, count = 0;
   int64_t first = 0;          // index of its first TermNode
   int64_t gather = 0, meta = 0;  // chunk-relative
@@ -830,8 +861,16 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
           T.bmap[p].insert(T.bmap[p].end(), fp, fp + np);
         }
       }
-    } else if (t->kind[u] == PPSG_SHOWDOWN) {
-      for (int p = 0; p < 2; ++p) T.gatherBlock(t, p, cl);
+    } else if (t->kind[u] == PPSG_SHOWDOWN || t->kind[u] == PPSG_FOLD) {
+      const bool sd = t->kind[u] == PPSG_SHOWDOWN;
+      for (int p = 0; p < 2; ++p) {
+        if (!sd) {  // fold: every card's segment fits 4 lanes x kFoldPart entries
+          const int32_t* so = t->seg_off[p] + (int64_t)cl * 53;
+          for (int c = 0; c < 52; ++c)
+            if (so[c + 1] - so[c] > 4 * kFoldPart) return fail("fold: card segment too long");
+        }
+        T.gatherBlock(t, p, cl, sd);
+      }
     }
   }
   for (int p = 0; p < 2; ++p) {
@@ -953,13 +992,14 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
         if (slot == 0) {
           TermGroup g;
           g.kind = sd ? kBwdShowdown : kBwdFold;
-          // showdown: the class's gather block and per-combo bounds; fold:
-          // both players' cards tables of the class
-          g.gather = sd ? L.gather[op] + T.sdOff[op][cl] : L.cardSame[op] + 4 * t->cls_list_off[op][cl];
+          // the opponent's gather block of the class; the traverser's
+          // per-combo bounds (showdown) or cards (fold)
+          g.gather = L.gather[op] + (sd ? T.sdOff[op][cl] : T.foldOff[op][cl]);
           g.meta = sd ? L.sdMeta[tr] + 16 * t->cls_list_off[tr][cl]
                       : L.cardSame[tr] + 4 * t->cls_list_off[tr][cl];
-          g.per = sd ? T.sdPer[op][cl] : 0;
-          g.rows = (uint8_t)(std::max(nT, nO) <= 3 * kBlock ? 3 : kRowRegs);
+          g.wide = sd && wideShowdown(nT, nO);
+          g.per = sd ? (g.wide ? 8 : T.sdPer[op][cl]) : 0;
+          g.rows = (uint8_t)(sd || std::max(nT, nO) <= 3 * kBlock ? 3 : kRowRegs);
           g.nT = nT;
           g.nO = nO;
           S.groups[tr].push_back(g);
@@ -1046,8 +1086,8 @@ struct ppsg_engine {
   int maxSpots = 0;
   std::vector<Spot> spots;
   // [traverser][0 forward / 1 backward per depth, 2 fold groups, 3 showdown
-  // groups (one list each)]
-  std::vector<DevList> lists[2][4];
+  // groups, 4 wide showdown groups (one list each)]
+  std::vector<DevList> lists[2][kLists];
   int groupSize = 4;  // terminal nodes per work item
   int64_t listedItems = 0, deadItems = 0;
   std::vector<int> live;
@@ -1101,7 +1141,7 @@ void ppsg_engine::dropGraph() {
 ppsg_engine::~ppsg_engine() {
   dropGraph();
   for (int tr = 0; tr < 2; ++tr)
-    for (int ps = 0; ps < 4; ++ps)
+    for (int ps = 0; ps < kLists; ++ps)
       for (DevList& l : lists[tr][ps])
         if (l.buf) (void)hipFree(l.buf);
   for (Spot& S : spots)
@@ -1140,8 +1180,8 @@ int ppsg_engine::setAlive(int slot, int v) {
 }
 
 // A spot's work items of one list with arena offsets: forward / backward
-// items of a depth, or (ps 2 / 3) its fold / showdown groups cut at the
-// group size.
+// items of a depth, or (ps 2 / 3 / 4) its fold / showdown / wide showdown
+// groups cut at the group size.
 std::vector<Item> ppsg_engine::spotItems(int slot, int tr, int ps, int dep) const {
   const Spot& S = spots[slot];
   std::vector<Item> v;
@@ -1159,7 +1199,7 @@ std::vector<Item> ppsg_engine::spotItems(int slot, int tr, int ps, int dep) cons
   }
   const uint8_t kind = ps == 2 ? kBwdFold : kBwdShowdown;
   for (const TermGroup& g : S.groups[tr]) {
-    if (g.kind != kind) continue;
+    if (g.kind != kind || (ps == 4) != g.wide) continue;
     for (int s0 = 0; s0 < g.count; s0 += groupSize) {
       Item it{};
       it.kind = g.kind;
@@ -1196,7 +1236,7 @@ int64_t ppsg_engine::countItems(int slot) const {
 
 namespace {
 
-int uploadTo(DevList& dl, const std::vector<Item>& v, bool append) {
+int uploadTo(DevList& dl, const std::vector<Item>& v, bool append, int lanes = kBlock) {
   const int64_t at0 = append ? dl.count : 0;
   const int64_t need = at0 + (int64_t)v.size();
   if (need > dl.cap) {
@@ -1212,7 +1252,7 @@ int uploadTo(DevList& dl, const std::vector<Item>& v, bool append) {
     HIP_TRY(hipMemcpy(dl.buf + at0, v.data(), v.size() * sizeof(Item), hipMemcpyHostToDevice));
   dl.count = need;
   for (const Item& it : v)
-    if (it.kind >= kBwdFold) dl.lds = std::max(dl.lds, terminalLdsOf(it));
+    if (it.kind >= kBwdFold) dl.lds = std::max(dl.lds, terminalLdsOf(it, lanes));
   return 0;
 }
 
@@ -1224,13 +1264,13 @@ int ppsg_engine::appendSpot(int slot) {
   const int D = S.maxDepth + 1;
   dropGraph();
   for (int tr = 0; tr < 2; ++tr)
-    for (int ps = 0; ps < 4; ++ps) {
+    for (int ps = 0; ps < kLists; ++ps) {
       std::vector<DevList>& L = lists[tr][ps];
       const int nl = ps < 2 ? D : 1;
       if ((int)L.size() < nl) L.resize(nl);
       for (int dep = 0; dep < nl; ++dep) {
         const std::vector<Item> v = spotItems(slot, tr, ps, dep);
-        if (!v.empty() && uploadTo(L[dep], v, true) != 0) return -1;
+        if (!v.empty() && uploadTo(L[dep], v, true, ps == 4 ? kWideBlock : kBlock) != 0) return -1;
       }
     }
   S.itemCount = countItems(slot);
@@ -1261,7 +1301,7 @@ int ppsg_engine::compact() {
     }
   }
   for (int tr = 0; tr < 2; ++tr)
-    for (int ps = 0; ps < 4; ++ps) {
+    for (int ps = 0; ps < kLists; ++ps) {
       std::vector<DevList>& L = lists[tr][ps];
       const int nl = ps < 2 ? D : (any ? 1 : 0);
       for (int dep = nl; dep < (int)L.size(); ++dep)
@@ -1279,7 +1319,7 @@ int ppsg_engine::compact() {
             return (int64_t)x.tile * x.m > (int64_t)y.tile * y.m;
           });
         L[dep].lds = 0;
-        if (uploadTo(L[dep], v, false) != 0) return -1;
+        if (uploadTo(L[dep], v, false, ps == 4 ? kWideBlock : kBlock) != 0) return -1;
       }
     }
   for (int s = 0; s < maxSpots; ++s)
@@ -1433,7 +1473,7 @@ int ppsg_engine::prepare() {
 int64_t ppsg_engine::kernelsPerIter() const {
   int64_t k = 1;  // counter bump
   for (int tr = 0; tr < 2; ++tr)
-    for (int ps = 0; ps < 4; ++ps)
+    for (int ps = 0; ps < kLists; ++ps)
       for (const DevList& l : lists[tr][ps]) k += l.count > 0;
   return k;
 }
@@ -1451,7 +1491,8 @@ int ppsg_engine::launchTerm(int tr, int ps, hipStream_t s) {
   const DevList& l = lists[tr][ps][0];
   if (l.count == 0) return 0;
   if (ps == 2) foldKernel<<<(unsigned)l.count, kBlock, l.lds, s>>>(dev(), l.buf, masked());
-  else showdownKernel<<<(unsigned)l.count, kBlock, l.lds, s>>>(dev(), l.buf, masked());
+  else if (ps == 3) showdownKernel<<<(unsigned)l.count, kBlock, l.lds, s>>>(dev(), l.buf, masked());
+  else showdownWideKernel<<<(unsigned)l.count, kWideBlock, l.lds, s>>>(dev(), l.buf, masked());
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1473,7 +1514,8 @@ int ppsg_engine::launchHalf(int tr, int mode, hipStream_t s) {
   const int D = depthCount();
   for (int dpt = 0; dpt < D; ++dpt)
     if (launchFwd(tr, dpt, fromStrat, -1, s) != 0) return -1;
-  if (launchTerm(tr, 2, s) != 0 || launchTerm(tr, 3, s) != 0) return -1;
+  for (int ps = 2; ps < kLists; ++ps)
+    if (launchTerm(tr, ps, s) != 0) return -1;
   for (int dpt = D - 1; dpt >= 0; --dpt)
     if (launchBwd(tr, dpt, mode, -1, s) != 0) return -1;
   return 0;
@@ -1781,13 +1823,13 @@ int ppsg_profile(ppsg_t* h, int iters, double* out, int cap) {
                      13 + 2 * k);
       }
       double lastFwd = acc[0];
-      for (int ps = 2; ps < 4 && rc == 0; ++ps) {
+      for (int ps = 2; ps < kLists && rc == 0; ++ps) {
         if (h->lists[tr][ps].empty() || !h->lists[tr][ps][0].count) continue;
         rc = timed([&] { return h->launchTerm(tr, ps, h->stream); }, 0);
         // attribution: fold and showdown have launches of their own
         if (rc == 0 && detail) {
-          acc[9 + 2 * (ps - 2)] += acc[0] - lastFwd;
-          acc[10 + 2 * (ps - 2)] += 1;
+          acc[ps == 2 ? 9 : 11] += acc[0] - lastFwd;
+          acc[ps == 2 ? 10 : 12] += 1;
         }
         lastFwd = acc[0];
       }
