@@ -18,10 +18,11 @@ and retire at exploitability <= 0.005 * pot.
   serving         a Poisson stream (1000 requests at 40/s, 70% turn spots as
                   above, 30% river spots): p50/p99 arrival-to-retirement
                   latency and aggregate iters/s over the stream's wall time
-  roofline        the dominant pass of the resident 256-spot batch, timed
-                  launch by launch with HIP events on the engine's stream:
-                  algorithmic f32 row bytes per launch / mean launch time,
-                  against MI355X's 8 TB/s HBM peak
+  roofline        the dominant kernel of the resident 256-spot batch (the
+                  one with the most time per iteration), timed launch by
+                  launch with HIP events on the engine's stream: algorithmic
+                  f32 row bytes per launch / mean launch time, against
+                  MI355X's 8 TB/s HBM peak; every kernel's figures beside it
   cpu_baseline    the CPU oracle (the tests' checker, a C++ OpenMP port of
                   the same algorithm) on the first spots of the stream, each
                   run for the iteration count the GPU needed, ~15 s
@@ -100,14 +101,46 @@ def cpu_baseline_single(spot, seconds):
                       f"(C++ CPU oracle, OpenMP over river deals), {dt:.1f} s"}
 
 
+def roofline_batch(prof):
+    """Per-kernel achieved bandwidth of the resident batch (algorithmic f32 row
+    bytes per launch / mean launch time, HIP events on the engine stream);
+    the dominant kernel (largest time per iteration) is the roofline entry."""
+    it = prof["iters"]
+    kb = prof["kind_bytes"]
+    term_ms = prof["fold_ms"] + prof["showdown_ms"]  # per iteration
+    kernels = {
+        # forward launches carry the reach items; the terminal launches follow
+        "fwdKernel": (kb["fwd_decide"] + kb["fwd_chance"], prof["fwd_ms"] / it - term_ms,
+                      prof["fwd_launches"] / it - prof["term_launches"] / it),
+        "bwdKernel": (kb["bwd_decide"] + kb["bwd_chance"], prof["bwd_ms"] / it,
+                      prof["bwd_launches"] / it),
+        "foldKernel": (kb["fold"], prof["fold_ms"], prof["fold_launches"] / it),
+        "showdownKernel": (kb["showdown"], prof["showdown_ms"], prof["showdown_launches"] / it),
+    }
+    per = {}
+    for name, (nbytes, ms, launches) in kernels.items():
+        per[name] = {"ms_per_iter": round(ms, 4), "launches_per_iter": round(launches, 2),
+                     "bytes_per_iter": round(nbytes),
+                     "gbs": round(nbytes / (ms * 1e-3) / 1e9, 1) if ms > 0 else None,
+                     "mean_launch_us": round(ms / launches * 1e3, 2) if launches else None}
+    dom = max(per, key=lambda k: per[k]["ms_per_iter"])
+    d = per[dom]
+    achieved = d["gbs"]
+    return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": round(d["bytes_per_iter"] / d["launches_per_iter"]),
+            "mean_launch_us": d["mean_launch_us"], "kernels": per}
+
+
 def roofline_of(prof, iters_key="iters"):
-    """Dominant pass: algorithmic bytes per launch / mean launch time."""
+    """Single spot: forward vs backward pass, algorithmic bytes per launch /
+    mean launch time."""
     it = prof[iters_key]
     fwd_per_iter, bwd_per_iter = prof["fwd_launches"] / it, prof["bwd_launches"] / it
     fwd_launch_ms = prof["fwd_ms"] / max(prof["fwd_launches"], 1)
     bwd_launch_ms = prof["bwd_ms"] / max(prof["bwd_launches"], 1)
     if prof["fwd_ms"] >= prof["bwd_ms"]:
-        kname, nbytes, lms = "fwdKernel", prof["fwd_bytes"] / fwd_per_iter, fwd_launch_ms
+        kname, nbytes, lms = "forward pass", prof["fwd_bytes"] / fwd_per_iter, fwd_launch_ms
     else:
         kname, nbytes, lms = "bwdKernel", prof["bwd_bytes"] / bwd_per_iter, bwd_launch_ms
     achieved = nbytes / (lms * 1e-3) / 1e9
@@ -117,8 +150,6 @@ def roofline_of(prof, iters_key="iters"):
         "bytes_per_launch": round(nbytes), "mean_launch_us": round(lms * 1e3, 2),
         "fwd_launch_us": round(fwd_launch_ms * 1e3, 2),
         "bwd_launch_us": round(bwd_launch_ms * 1e3, 2),
-        "fwd_gbs": round(prof["fwd_bytes"] / (prof["fwd_ms"] / it * 1e-3) / 1e9, 1),
-        "bwd_gbs": round(prof["bwd_bytes"] / (prof["bwd_ms"] / it * 1e-3) / 1e9, 1),
     }
 
 
@@ -130,6 +161,8 @@ def serve_stats(res, rep):
         "spots": len(res), "solved": len(lat), "converged": sum(r["converged"] for r in res),
         "iterations": rep["iterations"], "wall_s": round(rep["wall_ms"] / 1e3, 3),
         "busy_s": round(rep["busy_ms"] / 1e3, 3),
+        "device_s": round(rep["device_ms"] / 1e3, 3), "walk_s": round(rep["walk_ms"] / 1e3, 3),
+        "compact_s": round(rep["compact_ms"] / 1e3, 3), "admit_s": round(rep["admit_ms"] / 1e3, 3),
         "iters_per_s_wall": round(rep["iterations"] / (rep["wall_ms"] * 1e-3), 1),
         "iters_per_s_busy": round(rep["iterations"] / (rep["busy_ms"] * 1e-3), 1),
         "p50_ms": round(pick(0.50), 2) if lat else None,
@@ -154,7 +187,7 @@ def bench_batched(args, capi, configs):
     if args.profile_only:
         prof = capi.batch_profile(S, C, args.profile_iters)
         log(f"bench: profile {json.dumps(prof)}")
-        print(json.dumps({"profile": prof, "roofline": roofline_of(prof)}), flush=True)
+        print(json.dumps({"profile": prof, "roofline": roofline_batch(prof)}), flush=True)
         return 0
 
     res, rep = capi.serve(S, C, "dcfr", None, **opt)
@@ -190,7 +223,7 @@ def bench_batched(args, capi, configs):
 
     prof = capi.batch_profile(S, C, args.profile_iters)
     log(f"bench: profile {json.dumps(prof)}")
-    roof = roofline_of(prof)
+    roof = roofline_batch(prof)
 
     steps = rep["steps"] * CHECK_EVERY
     line = {

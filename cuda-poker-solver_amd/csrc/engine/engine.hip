@@ -673,6 +673,83 @@ __global__ void advanceCounters(Ctl* ctl, const int* __restrict__ slots, int n) 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Work-list compaction on the device: a stable filter keeping the items of
+// live spots (control block alive flag), in three steps: per-block counts,
+// one exclusive scan of the counts, scatter. Each block takes kCompactItems
+// consecutive items, every lane four of them in order.
+// ---------------------------------------------------------------------------
+
+constexpr int kCompactItems = 4 * kBlock;
+
+__device__ __forceinline__ int blockExclusiveScan(int v, int* total) {
+  __shared__ int wsum[kBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int inc = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += t;
+  }
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  int base = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    if (w < wave) base += wsum[w];
+    all += wsum[w];
+  }
+  __syncthreads();
+  if (total) *total = all;
+  return base + inc - v;
+}
+
+__global__ __launch_bounds__(kBlock) void compactCount(const Item* __restrict__ in, int64_t n,
+                                                       const Ctl* __restrict__ ctl,
+                                                       int* __restrict__ counts) {
+  const int64_t i0 = (int64_t)blockIdx.x * kCompactItems + 4 * (int64_t)threadIdx.x;
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (i0 + k < n && ctl[in[i0 + k].f].alive) ++c;
+  int total = 0;
+  (void)blockExclusiveScan(c, &total);
+  if (threadIdx.x == 0) counts[blockIdx.x] = total;
+}
+
+// One block: counts -> exclusive offsets (in place); *total = sum.
+__global__ __launch_bounds__(kBlock) void compactScan(int* __restrict__ counts, int nb,
+                                                      int* __restrict__ total) {
+  int carry = 0;
+  for (int b0 = 0; b0 < nb; b0 += kBlock) {
+    const int i = b0 + (int)threadIdx.x;
+    const int v = i < nb ? counts[i] : 0;
+    int sum = 0;
+    const int ex = blockExclusiveScan(v, &sum);
+    if (i < nb) counts[i] = carry + ex;
+    carry += sum;
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void compactScatter(const Item* __restrict__ in, int64_t n,
+                                                         const Ctl* __restrict__ ctl,
+                                                         const int* __restrict__ offs,
+                                                         Item* __restrict__ out) {
+  const int64_t i0 = (int64_t)blockIdx.x * kCompactItems + 4 * (int64_t)threadIdx.x;
+  bool keep[4];
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    keep[k] = i0 + k < n && ctl[in[i0 + k].f].alive;
+    c += keep[k];
+  }
+  int pos = offs[blockIdx.x] + blockExclusiveScan(c, nullptr);
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (keep[k]) out[pos++] = in[i0 + k];
+}
+
 // Plain streaming copy (16 B per lane per step): the calibration kernel for
 // hardware byte counters.
 __global__ __launch_bounds__(kBlock) void copyProbe(const float4* __restrict__ a,
@@ -1089,6 +1166,12 @@ struct ppsg_engine {
   // groups, 4 wide showdown groups (one list each)]
   std::vector<DevList> lists[2][kLists];
   int groupSize = 4;  // terminal nodes per work item
+  // device compaction scratch (grow-only)
+  Item* cscratch = nullptr;
+  int64_t cscratchCap = 0;
+  int* ccounts = nullptr;
+  int64_t ccountsCap = 0;
+  int* ctotal = nullptr;
   int64_t listedItems = 0, deadItems = 0;
   std::vector<int> live;
   int* liveDev = nullptr;
@@ -1117,6 +1200,8 @@ struct ppsg_engine {
   int64_t countItems(int slot) const;
   int appendSpot(int slot);
   int compact();
+  int compactList(DevList& l);
+  int rebuild();
   int prepare();
   void dropGraph();
   int launchFwd(int tr, int dpt, int fromStrat, int kindMask, hipStream_t s);
@@ -1147,6 +1232,9 @@ ppsg_engine::~ppsg_engine() {
   for (Spot& S : spots)
     if (S.coefDev) (void)hipFree(S.coefDev);
   if (liveDev) (void)hipFree(liveDev);
+  if (cscratch) (void)hipFree(cscratch);
+  if (ccounts) (void)hipFree(ccounts);
+  if (ctotal) (void)hipFree(ctotal);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (stream) (void)hipStreamDestroy(stream);
@@ -1218,7 +1306,7 @@ std::vector<Item> ppsg_engine::spotItems(int slot, int tr, int ps, int dep) cons
   }
   // largest groups first: they are the long ones
   std::stable_sort(v.begin(), v.end(), [](const Item& x, const Item& y) {
-    return (int64_t)x.tile * x.m > (int64_t)y.tile * y.m;
+    return (int64_t)x.cnt * x.m > (int64_t)y.cnt * y.m;
   });
   return v;
 }
@@ -1279,8 +1367,74 @@ int ppsg_engine::appendSpot(int slot) {
   return 0;
 }
 
-// Rebuilds the merged lists from the live spots and frees removed spots.
+// Drops the items of retired and parked spots from one list, on the device.
+int ppsg_engine::compactList(DevList& l) {
+  if (l.count == 0) return 0;
+  const int64_t nb = (l.count + kCompactItems - 1) / kCompactItems;
+  if (l.count > cscratchCap) {
+    if (cscratch) (void)hipFree(cscratch);
+    cscratch = nullptr;
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, (size_t)l.count * sizeof(Item)));
+    cscratch = static_cast<Item*>(p);
+    cscratchCap = l.count;
+  }
+  if (nb > ccountsCap) {
+    if (ccounts) (void)hipFree(ccounts);
+    ccounts = nullptr;
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, (size_t)nb * sizeof(int)));
+    ccounts = static_cast<int*>(p);
+    ccountsCap = nb;
+  }
+  if (!ctotal) {
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, sizeof(int)));
+    ctotal = static_cast<int*>(p);
+  }
+  compactCount<<<(unsigned)nb, kBlock, 0, stream>>>(l.buf, l.count, ctl, ccounts);
+  compactScan<<<1, kBlock, 0, stream>>>(ccounts, (int)nb, ctotal);
+  compactScatter<<<(unsigned)nb, kBlock, 0, stream>>>(l.buf, l.count, ctl, ccounts, cscratch);
+  HIP_TRY(hipGetLastError());
+  int kept = 0;
+  HIP_TRY(hipMemcpyAsync(&kept, ctotal, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  if (kept > 0)
+    HIP_TRY(hipMemcpyAsync(l.buf, cscratch, (size_t)kept * sizeof(Item), hipMemcpyDeviceToDevice,
+                           stream));
+  HIP_TRY(hipStreamSynchronize(stream));
+  l.count = kept;
+  return 0;
+}
+
+// Drops retired and parked spots' items from every list (on the device,
+// order kept) and frees removed spots.
 int ppsg_engine::compact() {
+  dropGraph();
+  for (int tr = 0; tr < 2; ++tr)
+    for (int ps = 0; ps < kLists; ++ps)
+      for (DevList& l : lists[tr][ps])
+        if (compactList(l) != 0) return -1;
+  listedItems = 0;
+  deadItems = 0;
+  for (int s = 0; s < maxSpots; ++s) {
+    Spot& S = spots[s];
+    if (S.state == kDoomed) {
+      arena.release(S.off, S.bytes);
+      if (S.coefDev) (void)hipFree(S.coefDev);
+      S = Spot{};
+    } else if (S.state == kParked) {
+      S.inLists = false;
+    } else if (S.state == kLive) {
+      listedItems += S.itemCount;
+    }
+  }
+  return 0;
+}
+
+// Rebuilds every list from the live spots' items (after a launch-shape
+// change) and frees removed spots.
+int ppsg_engine::rebuild() {
   dropGraph();
   listedItems = 0;
   deadItems = 0;
@@ -1316,7 +1470,7 @@ int ppsg_engine::compact() {
         }
         if (ps >= 2)  // largest groups of every spot first
           std::stable_sort(v.begin(), v.end(), [](const Item& x, const Item& y) {
-            return (int64_t)x.tile * x.m > (int64_t)y.tile * y.m;
+            return (int64_t)x.cnt * x.m > (int64_t)y.cnt * y.m;
           });
         L[dep].lds = 0;
         if (uploadTo(L[dep], v, false, ps == 4 ? kWideBlock : kBlock) != 0) return -1;
@@ -1808,7 +1962,9 @@ int ppsg_profile(ppsg_t* h, int iters, double* out, int cap) {
     acc[slot + 1] += 1;
     return 0;
   };
-  const bool detail = cap >= 13;
+  // cap 13: terminal launches attributed; cap 17: also decide / chance
+  // re-runs of the forward and backward launches
+  const bool detail = cap >= 17;
   const int D = h->depthCount();
   int rc = 0;
   for (int it = 0; it < iters && rc == 0; ++it) {
@@ -1818,7 +1974,7 @@ int ppsg_profile(ppsg_t* h, int iters, double* out, int cap) {
         rc = timed([&] { return h->launchFwd(tr, dpt, 0, -1, h->stream); }, 0);
         // attribution: forward decide / chance re-run alone (they rewrite the
         // same reach rows from the same inputs)
-        for (int k = 0; k < 2 && rc == 0 && cap >= 17; ++k)
+        for (int k = 0; k < 2 && rc == 0 && detail; ++k)
           rc = timed([&] { return h->launchFwd(tr, dpt, 0, 1 << (kFwdDecide + k), h->stream); },
                      13 + 2 * k);
       }
@@ -1826,8 +1982,8 @@ int ppsg_profile(ppsg_t* h, int iters, double* out, int cap) {
       for (int ps = 2; ps < kLists && rc == 0; ++ps) {
         if (h->lists[tr][ps].empty() || !h->lists[tr][ps][0].count) continue;
         rc = timed([&] { return h->launchTerm(tr, ps, h->stream); }, 0);
-        // attribution: fold and showdown have launches of their own
-        if (rc == 0 && detail) {
+        // fold and showdown have launches of their own: always attributed
+        if (rc == 0 && cap >= 13) {
           acc[ps == 2 ? 9 : 11] += acc[0] - lastFwd;
           acc[ps == 2 ? 10 : 12] += 1;
         }
@@ -1948,7 +2104,7 @@ int ppsg_tune(ppsg_t* h, int param, int value) {
   if (value < 1 || value > 1024) return fail("tune: group size must be 1..1024");
   if (value == h->groupSize) return 0;
   h->groupSize = value;
-  return h->compact();
+  return h->rebuild();
 }
 
 int ppsg_autotune(ppsg_t* h, const int* candidates, int n, int reps, int* chosen, double* ms) {

@@ -324,7 +324,9 @@ std::vector<SpotResult> serve(const std::vector<SpotRequest>& reqs, const ServeO
       }
       live.swap(keep);
       int c = 0;
+      const double c0 = msNow();
       check(ppsg_batch_compact(h, 0.25, &c), "compact");
+      rep.compactMs += msNow() - c0;
       rep.compactions += c;
     }
     rep.walkMs += msNow() - w0;
@@ -390,18 +392,20 @@ BatchProfile profileBatch(const std::vector<SpotRequest>& reqs, const std::strin
   check(ppsg_last_iterate_ms(E.h, &ms), "profile: replay time");
   P.graphMsPerIter = ms / iters;
   double o[17] = {0};
-  check(ppsg_profile(E.h, iters, o, 5), "profile: launches");
+  check(ppsg_profile(E.h, iters, o, 13), "profile: launches");
   P.fwdMs = o[0];
   P.fwdLaunches = o[1];
   P.bwdMs = o[2];
   P.bwdLaunches = o[3];
   P.iters = iters;
+  P.foldMs = o[9] / iters;
+  P.showdownMs = o[11] / iters;
+  P.foldLaunches = o[10];
+  P.showdownLaunches = o[12];
   if (detail) {
     check(ppsg_profile(E.h, iters, o, 17), "profile: attribution");
     P.decideMs = o[5] / iters;
     P.chanceMs = o[7] / iters;
-    P.foldMs = o[9] / iters;
-    P.showdownMs = o[11] / iters;
     P.fwdDecideMs = o[13] / iters;
     P.fwdChanceMs = o[15] / iters;
   }

@@ -408,6 +408,7 @@ int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const do
       report->device_ms = rep.deviceMs;
       report->walk_ms = rep.walkMs;
       report->admit_ms = rep.admitMs;
+      report->compact_ms = rep.compactMs;
       report->iterations = rep.iterations;
       report->steps = rep.steps;
       report->peak_live = rep.peakLive;
@@ -418,7 +419,7 @@ int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const do
 }
 
 int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, const char* algo,
-                      int iters, int64_t arena_bytes, int detail, double out[31]) {
+                      int iters, int64_t arena_bytes, int detail, double out[33]) {
   return guard([&] {
     if (n <= 0 || !spots || !cfgs || !out) throw std::runtime_error("null argument");
     std::vector<pps::gpu::SpotRequest> reqs(n);
@@ -438,6 +439,8 @@ int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, 
     out[23] = P.fwdDecideMs;
     out[24] = P.fwdChanceMs;
     for (int k = 0; k < 6; ++k) out[25 + k] = P.kindBytes[k];
+    out[31] = P.foldLaunches;
+    out[32] = P.showdownLaunches;
     return 0;
   });
 }

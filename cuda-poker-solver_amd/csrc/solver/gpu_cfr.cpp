@@ -559,7 +559,7 @@ std::vector<double> GpuPostflopSolver::profileIterations(int iters, bool detail)
   const int64_t t0 = totalIters_, tEnd = t0 + iters;
   const std::vector<float> tab = scheduleTable(t0, tEnd, (int)tEnd, I->lastAlgo);
   check(ppsg_set_schedule(I->eng, tab.data(), tEnd));
-  std::vector<double> out(detail ? 13 : 5, 0.0);
+  std::vector<double> out(detail ? 17 : 13, 0.0);
   check(ppsg_profile(I->eng, iters, out.data(), (int)out.size()));
   totalIters_ = tEnd;
   return out;

@@ -47,7 +47,8 @@ class ServeResult(ctypes.Structure):
 class ServeReport(ctypes.Structure):
     _fields_ = [("wall_ms", ctypes.c_double), ("busy_ms", ctypes.c_double),
                 ("device_ms", ctypes.c_double), ("walk_ms", ctypes.c_double),
-                ("admit_ms", ctypes.c_double), ("iterations", ctypes.c_int64),
+                ("admit_ms", ctypes.c_double), ("compact_ms", ctypes.c_double),
+                ("iterations", ctypes.c_int64),
                 ("steps", ctypes.c_int32), ("peak_live", ctypes.c_int32),
                 ("compactions", ctypes.c_int32)]
 
@@ -336,13 +337,15 @@ class Solver:
         return float(lib().pps_solver_iteration_bytes(self.h))
 
     def profile(self, iters, detail=False):
-        out = (ctypes.c_double * 13)()
+        out = (ctypes.c_double * 17)()
         _check(lib().pps_solver_profile(self.h, int(iters), int(detail), out))
         d = {"fwd_ms": out[0], "fwd_launches": int(out[1]), "bwd_ms": out[2],
-             "bwd_launches": int(out[3]), "iters": int(out[4])}
+             "bwd_launches": int(out[3]), "iters": int(out[4]), "fold_ms": out[9],
+             "fold_launches": int(out[10]), "showdown_ms": out[11],
+             "showdown_launches": int(out[12])}
         if detail:
-            for k, name in enumerate(["decide", "chance", "fold", "showdown"]):
-                d[f"{name}_only_ms"] = out[5 + 2 * k]
+            d.update(decide_only_ms=out[5], chance_only_ms=out[7], fwd_decide_only_ms=out[13],
+                     fwd_chance_only_ms=out[15])
         return d
 
     def engine_info(self):
@@ -406,7 +409,7 @@ def batch_profile(spots, cfgs, iters, algo="dcfr", arena_bytes=0, detail=False):
     n = len(spots)
     S = (Spot * n)(*spots)
     C = (BetConfig * n)(*cfgs)
-    out = (ctypes.c_double * 31)()
+    out = (ctypes.c_double * 33)()
     _check(lib().pps_batch_profile(n, S, C, algo.encode(), int(iters), int(arena_bytes),
                                    1 if detail else 0, out))
     keys = ["spots", "iters", "graph_ms_per_iter", "fwd_ms", "fwd_launches", "bwd_ms",
@@ -418,8 +421,9 @@ def batch_profile(spots, cfgs, iters, algo="dcfr", arena_bytes=0, detail=False):
     d["fwd_decide_ms"], d["fwd_chance_ms"] = out[23], out[24]
     d["kind_bytes"] = dict(zip(["fwd_decide", "fwd_chance", "bwd_decide", "bwd_chance", "fold",
                                 "showdown"], list(out)[25:31]))
+    d["fold_launches"], d["showdown_launches"] = out[31], out[32]
+    d["term_launches"] = out[31] + out[32]
     if not detail:
-        for k in ("decide_ms", "chance_ms", "fold_ms", "showdown_ms", "fwd_decide_ms",
-                  "fwd_chance_ms"):
+        for k in ("decide_ms", "chance_ms", "fwd_decide_ms", "fwd_chance_ms"):
             del d[k]
     return d

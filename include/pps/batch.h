@@ -60,6 +60,7 @@ struct ServeReport {
   double deviceMs = 0.0;  // DCFR iteration replays (events on the engine stream)
   double walkMs = 0.0;    // exploitability / EV walks incl. host reductions
   double admitMs = 0.0;   // uploads into the arena
+  double compactMs = 0.0; // dropping retired spots from the work lists
   int64_t iterations = 0; // sum of per-spot iterations
   int steps = 0, peakLive = 0, compactions = 0;
   double itersPerSec() const { return wallMs > 0.0 ? iterations / (wallMs * 1e-3) : 0.0; }
@@ -83,10 +84,12 @@ struct BatchProfile {
   double fwdMs = 0.0, fwdLaunches = 0.0, bwdMs = 0.0, bwdLaunches = 0.0;
   double fwdBytes = 0.0, bwdBytes = 0.0;
   int64_t kernelsPerIter = 0, itemsPerIter = 0, arenaBytes = 0;
-  // detail: ms per iteration of each node kind (backward decide / chance
-  // and forward decide / chance re-run alone; fold and showdown launches)
-  double decideMs = 0.0, chanceMs = 0.0, foldMs = 0.0, showdownMs = 0.0;
-  double fwdDecideMs = 0.0, fwdChanceMs = 0.0;
+  // ms per iteration of the fold and showdown launches, and their launch
+  // counts over the timed iterations
+  double foldMs = 0.0, showdownMs = 0.0, foldLaunches = 0.0, showdownLaunches = 0.0;
+  // detail: backward and forward decide / chance re-run alone (ms per
+  // iteration; an extra profiled run)
+  double decideMs = 0.0, chanceMs = 0.0, fwdDecideMs = 0.0, fwdChanceMs = 0.0;
   // algorithmic bytes per batched iteration by kind (as ppsg_kind_bytes)
   double kindBytes[6] = {0, 0, 0, 0, 0, 0};
   // terminal group size chosen by the autotuner, and the EV-walk time (ms

@@ -144,9 +144,10 @@ class GpuPostflopSolver {
   // root counterfactual row.
   double rootValue(int tr, int mode, std::vector<float>* rootCfv);
   // Continues a DCFR-family solve for `iters` iterations launch by launch,
-  // timing each launch: [forward ms, forward launches, backward ms,
-  // backward launches, iterations]; detail: also per-kind backward
-  // attribution (13 values; perturbs the rows — measurement only).
+  // timing each launch (ppsg_profile layout): [forward ms, forward launches,
+  // backward ms, backward launches, iterations, -, -, -, -, fold ms, fold
+  // launches, showdown ms, showdown launches]; detail: 17 values with the
+  // per-kind re-runs (perturbs the rows — measurement only).
   std::vector<double> profileIterations(int iters, bool detail = false);
   // Engine facts (include/ppsg.h ppsg_info).
   std::vector<int64_t> engineInfo() const;

@@ -100,8 +100,10 @@ int pps_solver_half_step(pps_solver* s, int tr, int t, const char* algo, int pla
 /* root value walk: mode 1 EV, 2 BR; fills the root cfv row; returns value */
 int pps_solver_root_value(pps_solver* s, int tr, int mode, float* root_cfv, double* value);
 double pps_solver_iteration_bytes(pps_solver* s);
-/* per-launch event timing of `iters` more iterations (profileIterations);
-   detail != 0: 13 values incl. per-kind backward attribution */
+/* per-launch event timing of `iters` more iterations (profileIterations),
+   as ppsg_profile: 13 values (forward incl. fold/showdown launches,
+   backward, per-terminal-kind launches); detail != 0: 17 values with the
+   per-kind re-runs of the forward and backward launches */
 int pps_solver_profile(pps_solver* s, int iters, int detail, double* out);
 /* ppsg_info of the solver's engine: 6 values */
 int pps_solver_engine_info(pps_solver* s, int64_t out6[6]);
@@ -143,7 +145,7 @@ typedef struct pps_serve_result {
 typedef struct pps_serve_report {
   double wall_ms;           /* stream start to last retirement */
   double busy_ms;           /* steps + checks + retirements: spots resident */
-  double device_ms, walk_ms, admit_ms;
+  double device_ms, walk_ms, admit_ms, compact_ms;
   int64_t iterations;       /* sum of per-spot iterations */
   int32_t steps, peak_live, compactions;
 } pps_serve_report;
@@ -160,15 +162,16 @@ int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const do
    once): out = [spots, iters, graph ms per batched iteration, forward ms,
    forward launches, backward ms, backward launches, forward bytes and
    backward bytes per batched iteration (algorithmic f32 rows), kernels
-   per iteration, work items per iteration, arena bytes, then with
-   detail != 0 the ms per iteration of each node kind run alone: backward
-   decide, backward chance, fold, showdown; then the terminal group size
-   the autotuner chose and its EV-walk ms for group sizes 1, 2, 4, 8, 16,
-   32; then (detail) forward decide, forward chance; then the algorithmic
-   bytes per batched iteration of forward decide, forward chance, backward
-   decide, backward chance, fold, showdown]. */
+   per iteration, work items per iteration, arena bytes, (detail != 0)
+   backward decide and backward chance ms per iteration run alone, fold
+   and showdown launch ms per iteration, the terminal group size the
+   autotuner chose and its EV-walk ms for group sizes 1, 2, 4, 8, 16, 32,
+   (detail) forward decide and forward chance ms per iteration run alone,
+   the algorithmic bytes per batched iteration of forward decide, forward
+   chance, backward decide, backward chance, fold, showdown, and the fold
+   and showdown launch counts over the timed iterations]. */
 int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, const char* algo,
-                      int iters, int64_t arena_bytes, int detail, double out[31]);
+                      int iters, int64_t arena_bytes, int detail, double out[33]);
 /* Device bytes the compiled spot takes in an arena (host only). */
 int pps_compile_footprint(const pps_spot* spot, const pps_bet_config* cfg, int64_t* bytes);
 

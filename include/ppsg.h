@@ -113,11 +113,12 @@ int ppsg_info(ppsg_t* h, int64_t* out, int cap);
    launch with events on the engine stream: out = [forward ms, forward
    launches, backward ms, backward launches, iterations]; the forward
    figures include the terminal launch. Advances the counter like
-   ppsg_iterate. With cap >= 13 the terminal and backward launches are also
-   re-run restricted to one node kind: out[5..6] backward decide, [7..8]
-   backward chance, [9..10] fold, [11..12] showdown (ms, launches); with
-   cap >= 17 also [13..14] forward decide, [15..16] forward chance. The
-   decide re-runs apply the updates again, so use it for attribution only.
+   ppsg_iterate. With cap >= 13: out[9..10] fold and [11..12] showdown
+   launches (ms, launches; part of the forward figures). With cap >= 17 the
+   forward and backward launches are also re-run restricted to one node
+   kind: out[5..6] backward decide, [7..8] backward chance, [13..14]
+   forward decide, [15..16] forward chance; the decide re-runs apply the
+   updates again, so use that mode for attribution only.
    (No reference counterpart; the reference's GPU_CFR_PROFILE mode plays
    this role, cuda/gpu_cfr.cu:1895-1987.) */
 int ppsg_profile(ppsg_t* h, int iters, double* out, int cap);
