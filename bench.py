@@ -27,8 +27,8 @@ and retire at exploitability <= 0.005 * pot.
                   the same algorithm) on the first spots of the stream, each
                   run for the iteration count the GPU needed, ~15 s
 
---workload large-turn / plain-turn measures one spot alone (2000 / 500
-iterations from the captured graph) instead.
+--workload large-turn / plain-turn / large-flop measures one spot alone
+(2000 / 500 / 1000 iterations from the captured graph) instead.
 """
 import argparse
 import json
@@ -90,8 +90,12 @@ def cpu_baseline_single(spot, seconds):
     threads = cpu_threads()
     o = oracle_of(spot, threads)
     t0 = time.perf_counter()
-    o.solve(2, spot["algo"])
-    per = max((time.perf_counter() - t0) / 2, 1e-4)
+    o.solve(1, spot["algo"])
+    per = max(time.perf_counter() - t0, 1e-4)
+    if per * 2 > seconds:  # a flop-size tree: the one timed iteration is the sample
+        return {"value": round(1.0 / per, 4), "unit": "iters/s", "cores": threads,
+                "kind": "port", "sample": f"1 DCFR iteration of the {spot['name']} spot "
+                f"(C++ CPU oracle, OpenMP over river deals), {per:.1f} s"}
     n = max(2, int(seconds / per))
     t0 = time.perf_counter()
     o.solve(n, spot["algo"])
@@ -268,7 +272,8 @@ def bench_batched(args, capi, configs):
 
 
 def bench_single(args, capi, configs):
-    spot = configs.large_turn() if args.workload == "large-turn" else configs.plain_turn()
+    spot = {"large-turn": configs.large_turn, "plain-turn": configs.plain_turn,
+            "large-flop": configs.large_flop}[args.workload]()
     sp, cf = spot_cfg(capi, spot)
     t0 = time.perf_counter()
     solver = capi.Solver(sp, cf)
@@ -302,6 +307,9 @@ def bench_single(args, capi, configs):
     solver.close()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline_single(spot, args.cpu_seconds)
+    if args.workload == "large-flop":
+        info_gb = info["device_bytes"] / 1e9
+        line["config"]["arena_gb"] = round(info_gb, 2)
     print(json.dumps(line), flush=True)
     return 0
 
@@ -313,7 +321,7 @@ def main():
                     help="per-spot iteration cap (batched) / timed iterations (single spot)")
     ap.add_argument("--warmup", type=int, default=64)
     ap.add_argument("--workload", default="batched-turn",
-                    choices=["batched-turn", "large-turn", "plain-turn"])
+                    choices=["batched-turn", "large-turn", "plain-turn", "large-flop"])
     ap.add_argument("--spots", type=int, default=256)
     ap.add_argument("--serve-spots", type=int, default=1000)
     ap.add_argument("--qps", type=float, default=40.0)
