@@ -172,7 +172,7 @@ def serve_stats(res, rep):
         "p50_ms": round(pick(0.50), 2) if lat else None,
         "p99_ms": round(pick(0.99), 2) if lat else None,
         "iters_per_spot_median": its[len(its) // 2] if its else None,
-        "peak_live": rep["peak_live"], "steps": rep["steps"],
+        "peak_live": rep["peak_live"], "steps": rep["steps"], "group_size": rep["group_size"],
     }
 
 
@@ -201,6 +201,10 @@ def bench_batched(args, capi, configs):
     batched = serve_stats(res, rep)
     value = rep["iterations"] / (rep["busy_ms"] * 1e-3)
     log(f"bench: batched {json.dumps(batched)}")
+    live_nodes = [sum(s["nodes"] for s, r in zip(spots, res) if r["iterations"] > k * CHECK_EVERY)
+                  for k in range(len(rep["trace"]))]
+    log(f"bench: batched steps (live, device ms, check ms, live nodes) "
+        f"{[t + (m,) for t, m in zip(rep['trace'], live_nodes)]}")
 
     single = None
     if not args.no_single:

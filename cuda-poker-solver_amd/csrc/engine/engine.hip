@@ -56,8 +56,12 @@ constexpr int kLists = 5;  // per traverser: forward, backward, fold, showdown, 
 constexpr int kGatherHeader = 256;  // bytes: 53 int32 segment bounds + pad
 // fold: entries of a card's segment per lane (4 lanes per card)
 constexpr int kFoldPart = 12;
-// combos per lane (kTile apart) of decide and chance items
+// combos per lane (kTile apart) of forward decide, backward decide and
+// forward chance items (chosen by measurement: more loads in flight per lane
+// help the read-mostly forward items; the read-modify-write backward tiles
+// do best with one combo per lane and more lanes busy)
 constexpr int kRDecide = 2;
+constexpr int kRBackward = 1;
 constexpr int kRChance = 4;
 
 thread_local std::string g_err;
@@ -412,18 +416,18 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
     }
     if (tid == NT - 1) X[PER * NT] = run;
     __syncthreads();
-    auto at = [&](int p) -> Acc { return X[p == PER * NT ? p : (p % PER) * NT + p / PER]; };
+    auto xAt = [&](int p) -> Acc { return X[p == PER * NT ? p : (p % PER) * NT + p / PER]; };
     if (tid < 52) {
-      const Acc z = at(nO + so[tid]);
+      const Acc z = xAt(nO + so[tid]);
       zt[tid][0] = z;
-      zt[tid][1] = at(nO + so[tid + 1]) - z;
+      zt[tid][1] = xAt(nO + so[tid + 1]) - z;
     }
     __syncthreads();
 
     // 3. per traverser combo
     float* __restrict__ out = reinterpret_cast<float*>(d.arena + it.b + nd.out);
     const Acc vWin = nd.v0, vTie = nd.v1, vLose = nd.v2;
-    const Acc total = at(nO);
+    const Acc total = xAt(nO);
 #pragma unroll
     for (int q = 0; q < ROWS; ++q) {
       const int i = tid + q * NT;
@@ -436,8 +440,8 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
       const int same = (int)(m.w >> 16) - 1;
       const Acc z1 = zt[c1][0], t1 = zt[c1][1], z2 = zt[c2][0], t2 = zt[c2][1];
       const Acc rs = same >= 0 ? (Acc)raw[same] : (Acc)0;
-      Acc win = at(le) - (at(nO + a1) - z1) - (at(nO + a2) - z2);
-      const Acc leq = at(te) - (at(nO + b1) - z1) - (at(nO + b2) - z2) + rs;
+      Acc win = xAt(le) - (xAt(nO + a1) - z1) - (xAt(nO + a2) - z2);
+      const Acc leq = xAt(te) - (xAt(nO + b1) - z1) - (xAt(nO + b2) - z2) + rs;
       const Acc all = total - t1 - t2 + rs;
       if (win < 0) win = 0;
       Acc tie = leq - win;
@@ -653,14 +657,13 @@ __global__ __launch_bounds__(kBlock) void bwdKernel(Dev d, const Item* __restric
     return;
   }
   switch (it.na) {
-    case 1: decideTile<MODE, 1, kRDecide>(d, it); break;
-    case 2: decideTile<MODE, 2, kRDecide>(d, it); break;
-    case 3: decideTile<MODE, 3, kRDecide>(d, it); break;
-    case 4: decideTile<MODE, 4, kRDecide>(d, it); break;
-    // 5 and 6 actions: one combo per lane keeps the kernel under 72 VGPRs
-    case 5: decideTile<MODE, 5, 1>(d, it); break;
-    case 6: decideTile<MODE, 6, 1>(d, it); break;
-    default: decideTile<MODE, 0, 1>(d, it); break;
+    case 1: decideTile<MODE, 1, kRBackward>(d, it); break;
+    case 2: decideTile<MODE, 2, kRBackward>(d, it); break;
+    case 3: decideTile<MODE, 3, kRBackward>(d, it); break;
+    case 4: decideTile<MODE, 4, kRBackward>(d, it); break;
+    case 5: decideTile<MODE, 5, kRBackward>(d, it); break;
+    case 6: decideTile<MODE, 6, kRBackward>(d, it); break;
+    default: decideTile<MODE, 0, kRBackward>(d, it); break;
   }
 }
 
@@ -671,6 +674,13 @@ __global__ void advanceCounters(Ctl* ctl, const int* __restrict__ slots, int n) 
     Ctl& c = ctl[slots[i]];
     c.counter = c.counter + 1;
   }
+}
+
+// Sets every control block's alive flag (walks of a subset of spots mask all
+// others, then restore the flags of the live set).
+__global__ void setAliveFlags(Ctl* ctl, const int* __restrict__ flags, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    ctl[i].alive = flags[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -1012,7 +1022,7 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
           }
         }
         const int64_t stride = na > 1 ? t->cfv_off[t->child_flat[kb + 1]] - t->cfv_off[first] : 0;
-        const int rb = na <= 4 ? kRDecide : 1;  // must match bwdKernel's dispatch
+        const int rb = kRBackward;  // must match bwdKernel's dispatch
         for (int q = 0; q < tiles(nT, rb); ++q) {
           Item it{};
           it.kind = kBwdDecide;
@@ -1182,6 +1192,8 @@ struct ppsg_engine {
   hipGraphExec_t exec = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double lastMs = -1.0;
+  int* flagsDev = nullptr;  // maxSpots alive flags staged for setAliveFlags
+  int forceMask = 0;        // kernels check alive flags even with no dead items
 
   Dev dev() const { return Dev{arena.base, ctl}; }
   bool valid(int slot) const {
@@ -1190,12 +1202,13 @@ struct ppsg_engine {
   }
   bool isLive(int slot) const { return valid(slot) && spots[slot].state == kLive; }
   int depthCount() const { return (int)lists[0][0].size(); }
-  int masked() const { return deadItems > 0 ? 1 : 0; }
+  int masked() const { return (forceMask || deadItems > 0) ? 1 : 0; }
   int init(int64_t bytes, int spotsCap);
   int add(const ppsg_tree_desc* t, int* slot);
   int remove(int slot);
   int setActive(int slot, bool on);
   int setAlive(int slot, int v);
+  int setAliveAll(const std::vector<int>& flags);
   std::vector<Item> spotItems(int slot, int tr, int ps, int dep) const;
   int64_t countItems(int slot) const;
   int appendSpot(int slot);
@@ -1232,6 +1245,7 @@ ppsg_engine::~ppsg_engine() {
   for (Spot& S : spots)
     if (S.coefDev) (void)hipFree(S.coefDev);
   if (liveDev) (void)hipFree(liveDev);
+  if (flagsDev) (void)hipFree(flagsDev);
   if (cscratch) (void)hipFree(cscratch);
   if (ccounts) (void)hipFree(ccounts);
   if (ctotal) (void)hipFree(ctotal);
@@ -1253,6 +1267,8 @@ int ppsg_engine::init(int64_t bytes, int spotsCap) {
   HIP_TRY(hipMalloc(&p, sizeof(Ctl) * (size_t)spotsCap));
   ctl = static_cast<Ctl*>(p);
   HIP_TRY(hipMemset(ctl, 0, sizeof(Ctl) * (size_t)spotsCap));
+  HIP_TRY(hipMalloc(&p, sizeof(int) * (size_t)spotsCap));
+  flagsDev = static_cast<int*>(p);
   maxSpots = spotsCap;
   spots.assign(spotsCap, Spot{});
   HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -1264,6 +1280,16 @@ int ppsg_engine::init(int64_t bytes, int spotsCap) {
 int ppsg_engine::setAlive(int slot, int v) {
   HIP_TRY(hipStreamSynchronize(stream));
   HIP_TRY(hipMemcpy(&ctl[slot].alive, &v, sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int ppsg_engine::setAliveAll(const std::vector<int>& flags) {
+  HIP_TRY(hipStreamSynchronize(stream));
+  HIP_TRY(hipMemcpy(flagsDev, flags.data(), sizeof(int) * (size_t)maxSpots, hipMemcpyHostToDevice));
+  const unsigned g = (unsigned)std::min<int64_t>(1024, (maxSpots + kBlock - 1) / kBlock);
+  setAliveFlags<<<g, kBlock, 0, stream>>>(ctl, flagsDev, maxSpots);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(stream));
   return 0;
 }
 
@@ -1804,6 +1830,49 @@ int walk(ppsg_t* h, int tr, int mode, int nSlots, const int* slots, float* const
   return 0;
 }
 
+// EV / BR half-step of the listed spots only (live or parked): every other
+// spot's items are masked for the walk, the live set's flags restored after.
+int walkOnly(ppsg_t* h, int tr, int mode, int nSlots, const int* slots, float* const* rootCfv) {
+  if (!h) return fail("null handle");
+  if (nSlots <= 0 || !slots || !rootCfv) return fail("null argument");
+  if ((tr != 0 && tr != 1) || (mode != PPSG_MODE_EV && mode != PPSG_MODE_BR))
+    return fail("walk: bad traverser or mode");
+  for (int i = 0; i < nSlots; ++i)
+    if (!h->valid(slots[i])) return fail("walk: no spot in slot " + std::to_string(slots[i]));
+  if (h->prepare() != 0) return -1;  // compacts (by the live flags) before they change
+  for (int i = 0; i < nSlots; ++i) {
+    Spot& S = h->spots[slots[i]];
+    if (S.inLists) continue;
+    // a parked spot dropped by a compaction: list its items again, masked
+    if (h->appendSpot(slots[i]) != 0) {
+      S.inLists = true;  // possibly listed in part: keep the masking on
+      h->listedItems += S.itemCount;
+      h->deadItems += S.itemCount;
+      return -1;
+    }
+    h->deadItems += S.itemCount;
+  }
+  std::vector<int> flags((size_t)h->maxSpots, 0);
+  for (int i = 0; i < nSlots; ++i) flags[slots[i]] = 1;
+  int rc = h->setAliveAll(flags);
+  if (rc == 0) {
+    h->forceMask = 1;
+    rc = h->launchHalf(tr, mode, h->stream);
+    h->forceMask = 0;
+  }
+  for (int s = 0; s < h->maxSpots; ++s) flags[s] = h->spots[s].state == kLive ? 1 : 0;
+  const int rc2 = h->setAliveAll(flags);
+  if (rc != 0) return rc;
+  if (rc2 != 0) return rc2;
+  for (int i = 0; i < nSlots; ++i) {
+    const Spot& S = h->spots[slots[i]];
+    if (rootCfv[i])
+      HIP_TRY(hipMemcpy(rootCfv[i], h->arena.base + S.off + S.rootCfv, sizeof(float) * S.n[tr],
+                        hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
 int64_t bufOf(const Spot& S, int buf, int64_t& total) {
   switch (buf) {
     case PPSG_REGRET: total = S.regTotal; return S.L.regret;
@@ -2064,6 +2133,11 @@ int ppsg_batch_step(ppsg_t* h, int n) { return iterate(h, n); }
 int ppsg_batch_walk(ppsg_t* h, int tr, int mode, int n_slots, const int* slots,
                     float* const* root_cfv) {
   return walk(h, tr, mode, n_slots, slots, root_cfv);
+}
+
+int ppsg_batch_walk_only(ppsg_t* h, int tr, int mode, int n_slots, const int* slots,
+                         float* const* root_cfv) {
+  return walkOnly(h, tr, mode, n_slots, slots, root_cfv);
 }
 
 int ppsg_batch_read(ppsg_t* h, int slot, int buf, int64_t off, int64_t n, float* host) {

@@ -1,24 +1,30 @@
 // pps_native: Python binding of pps::gpu::GpuPostflopSolver — drop-in for
 // the reference's python/pps_native.cpp:514-550 (same class, methods,
-// argument names and defaults, same result dicts). Engine calls release
-// the GIL; callers serialize use of one Solver.
+// argument names and defaults, same result dicts) — plus pps.Server, which
+// serves many spots on one GPU with their solves batched
+// (include/pps/server.h). Engine calls release the GIL; callers serialize
+// use of one Solver.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
 #include <memory>
+#include <optional>
 #include <stdexcept>
+#include <type_traits>
 #include <string>
 #include <vector>
 
 #include "pps/gpu_cfr.h"
 #include "pps/postflop_cfr.h"
 #include "pps/range.h"
+#include "pps/server.h"
 
 namespace py = pybind11;
 using pps::Card;
 using pps::Range;
 using pps::gpu::GpuPostflopSolver;
+using pps::gpu::SpotServer;
 
 namespace {
 
@@ -151,29 +157,62 @@ py::list intList(const std::vector<int>& v) {
   return out;
 }
 
+void readSpot(const py::object& board, const py::object& oop, const py::object& ip, int64_t pot,
+              int64_t stack, const py::object& bets, const py::object& raises, int maxRaises,
+              pps::gpu::PostflopSpot& spot, pps::pf::BetConfig& cfg) {
+  const std::vector<Card> bd = readBoard(board);
+  spot.nBoard = (int)bd.size();
+  for (int i = 0; i < spot.nBoard; ++i) spot.board[i] = bd[i];
+  spot.oop = readRange(oop);
+  spot.ip = readRange(ip);
+  spot.pot = pot;
+  spot.stack = stack;
+  spot.oop.filterBoard(spot.board, spot.nBoard);
+  spot.ip.filterBoard(spot.board, spot.nBoard);
+  cfg.betFracs.clear();
+  readBets(bets, cfg);
+  cfg.raiseMults = readDoubles(raises);
+  cfg.maxRaises = maxRaises;
+  if (cfg.betFracs.empty() && !cfg.betAllIn && !cfg.betGeometric)
+    throw std::runtime_error("empty bet abstraction: pass at least one size (e.g. \"0.75,a\")");
+}
+
+py::dict jobDict(const pps::gpu::JobInfo& j) {
+  py::dict d;
+  d["running"] = j.running;
+  d["status"] = j.status.empty() ? py::object(py::none()) : py::object(py::str(j.status));
+  d["kind"] = j.kind.empty() ? py::object(py::none()) : py::object(py::str(j.kind));
+  d["start"] = j.start;
+  d["done"] = j.done;
+  d["total"] = j.total;
+  d["target"] = j.target > 0.0 ? py::object(py::float_(j.target)) : py::object(py::none());
+  d["error"] = j.error.empty() ? py::object(py::none()) : py::object(py::str(j.error));
+  return d;
+}
+
+// A solver with its own engine, or a spot served by a SpotServer (then every
+// call runs between the server's batched steps, and solves are jobs of the
+// server, batched with the other spots' jobs).
 class Solver {
  public:
   Solver(const py::object& board, const py::object& oop, const py::object& ip, int64_t pot,
          int64_t stack, const py::object& bets, const py::object& raises, int maxRaises) {
-    const std::vector<Card> bd = readBoard(board);
     pps::gpu::PostflopSpot spot;
-    spot.nBoard = (int)bd.size();
-    for (int i = 0; i < spot.nBoard; ++i) spot.board[i] = bd[i];
-    spot.oop = readRange(oop);
-    spot.ip = readRange(ip);
-    spot.pot = pot;
-    spot.stack = stack;
-    spot.oop.filterBoard(spot.board, spot.nBoard);
-    spot.ip.filterBoard(spot.board, spot.nBoard);
     pps::pf::BetConfig cfg;
-    cfg.betFracs.clear();
-    readBets(bets, cfg);
-    cfg.raiseMults = readDoubles(raises);
-    cfg.maxRaises = maxRaises;
-    if (cfg.betFracs.empty() && !cfg.betAllIn && !cfg.betGeometric)
-      throw std::runtime_error("empty bet abstraction: pass at least one size (e.g. \"0.75,a\")");
+    readSpot(board, oop, ip, pot, stack, bets, raises, maxRaises, spot, cfg);
     py::gil_scoped_release nogil;
     s_ = std::make_unique<GpuPostflopSolver>(spot, cfg);
+  }
+
+  Solver(std::shared_ptr<SpotServer> srv, int id) : srv_(std::move(srv)), id_(id) {}
+
+  ~Solver() {
+    if (!srv_) return;
+    try {
+      py::gil_scoped_release nogil;
+      srv_->remove(id_);
+    } catch (...) {
+    }
   }
 
   static std::unique_ptr<Solver> load(const std::string& path) {
@@ -183,25 +222,82 @@ class Solver {
     return s;
   }
 
-  void solve(int maxIters, const std::string& algo, double target) {
+  // Runs f(solver) without the GIL: directly, or on the server between steps.
+  template <class F>
+  auto on(F&& f) const -> decltype(f(std::declval<GpuPostflopSolver&>())) {
+    using R = decltype(f(std::declval<GpuPostflopSolver&>()));
     py::gil_scoped_release nogil;
-    s_->solve(maxIters, algo, target);
-  }
-  void continueSolve(int maxIters, double target) {
-    py::gil_scoped_release nogil;
-    s_->continueSolve(maxIters, target);
-  }
-  void reset() {
-    py::gil_scoped_release nogil;
-    s_->reset();
+    if (!srv_) return f(*s_);
+    if constexpr (std::is_void_v<R>) {
+      srv_->with(id_, [&](GpuPostflopSolver& g) { f(g); });
+    } else {
+      std::optional<R> r;
+      srv_->with(id_, [&](GpuPostflopSolver& g) { r.emplace(f(g)); });
+      return std::move(*r);
+    }
   }
 
-  py::dict stats() {
-    pps::pf::NodeStats st;
+  void solve(int maxIters, const std::string& algo, double target) {
+    if (!srv_) return on([&](GpuPostflopSolver& g) { g.solve(maxIters, algo, target); });
+    startSolve(maxIters, algo, target);
+    finishJob();
+  }
+  void continueSolve(int maxIters, double target) {
+    if (!srv_) return on([&](GpuPostflopSolver& g) { g.continueSolve(maxIters, target); });
+    startContinue(maxIters, target);
+    finishJob();
+  }
+  void reset() {
+    if (!srv_) return on([](GpuPostflopSolver& g) { g.reset(); });
+    py::gil_scoped_release nogil;
+    srv_->reset(id_);
+  }
+
+  // ---- jobs of a served spot (no reference counterpart in the module; the
+  // reference's HTTP layer runs them on threads, python/pps_api.py:350-403)
+  py::dict startSolve(int maxIters, const std::string& algo, double target) {
+    served("start_solve");
     {
       py::gil_scoped_release nogil;
-      st = s_->stats();
+      srv_->solve(id_, maxIters, algo, target);
     }
+    return job();
+  }
+  py::dict startContinue(int maxIters, double target) {
+    served("start_continue");
+    {
+      py::gil_scoped_release nogil;
+      srv_->continueSolve(id_, maxIters, target);
+    }
+    return job();
+  }
+  py::dict job() {
+    served("job");
+    pps::gpu::JobInfo j;
+    {
+      py::gil_scoped_release nogil;
+      j = srv_->job(id_);
+    }
+    return jobDict(j);
+  }
+  bool cancel() {
+    served("cancel");
+    py::gil_scoped_release nogil;
+    return srv_->cancel(id_);
+  }
+  py::dict wait() {
+    served("wait");
+    pps::gpu::JobInfo j;
+    {
+      py::gil_scoped_release nogil;
+      j = srv_->wait(id_);
+    }
+    return jobDict(j);
+  }
+  int id() const { return srv_ ? id_ : -1; }
+
+  py::dict stats() {
+    const pps::pf::NodeStats st = on([](GpuPostflopSolver& g) { return g.stats(); });
     py::dict d;
     d["ev_oop"] = st.ev0;
     d["ev_ip"] = st.ev1;
@@ -210,20 +306,23 @@ class Solver {
     return d;
   }
 
-  int numNodes() const { return s_->numNodes(); }
-  int maxDepth() const { return s_->maxDepth(); }
-  int numDecideNodes() const { return s_->numDecideNodes(); }
-  int iterationsRun() const { return s_->iterationsRun(); }
-  int64_t totalIterations() const { return s_->totalIterations(); }
-  double iterationBytes() const { return s_->iterationBytes(); }
-  double lastSolveDeviceMs() const { return s_->lastSolveDeviceMs(); }
+  int numNodes() const { return on([](GpuPostflopSolver& g) { return g.numNodes(); }); }
+  int maxDepth() const { return on([](GpuPostflopSolver& g) { return g.maxDepth(); }); }
+  int numDecideNodes() const { return on([](GpuPostflopSolver& g) { return g.numDecideNodes(); }); }
+  int iterationsRun() const { return on([](GpuPostflopSolver& g) { return g.iterationsRun(); }); }
+  int64_t totalIterations() const {
+    return on([](GpuPostflopSolver& g) { return g.totalIterations(); });
+  }
+  double iterationBytes() const { return on([](GpuPostflopSolver& g) { return g.iterationBytes(); }); }
+  double lastSolveDeviceMs() const {
+    return on([](GpuPostflopSolver& g) { return g.lastSolveDeviceMs(); });
+  }
 
   py::list decideNodes() const {
     std::vector<pps::gpu::DecideNodeInfo> ns;
-    {
-      py::gil_scoped_release nogil;
-      for (int i = 0; i < s_->numDecideNodes(); ++i) ns.push_back(s_->decideNode(i));
-    }
+    on([&](GpuPostflopSolver& g) {
+      for (int i = 0; i < g.numDecideNodes(); ++i) ns.push_back(g.decideNode(i));
+    });
     py::list out;
     for (size_t i = 0; i < ns.size(); ++i) {
       py::dict d;
@@ -247,13 +346,12 @@ class Solver {
     pps::gpu::DecideNodeInfo n;
     std::vector<uint8_t> cards;
     std::vector<double> agg;
-    {
-      py::gil_scoped_release nogil;
-      cs = s_->comboStrategy(idx);
-      n = s_->decideNode(idx);
-      cards = s_->playerCards(n.player);
-      agg = s_->nodeStrategy(idx);
-    }
+    on([&](GpuPostflopSolver& g) {
+      cs = g.comboStrategy(idx);
+      n = g.decideNode(idx);
+      cards = g.playerCards(n.player);
+      agg = g.nodeStrategy(idx);
+    });
     const py::ssize_t na = (py::ssize_t)cs.actions.size(), nc = (py::ssize_t)cs.combos.size();
     py::dict d;
     d["index"] = idx;
@@ -276,11 +374,10 @@ class Solver {
   py::dict rootEv() {
     pps::gpu::ComboEv ev;
     std::vector<uint8_t> cards[2];
-    {
-      py::gil_scoped_release nogil;
-      ev = s_->rootEvPerCombo();
-      for (int p = 0; p < 2; ++p) cards[p] = s_->playerCards(p);
-    }
+    on([&](GpuPostflopSolver& g) {
+      ev = g.rootEvPerCombo();
+      for (int p = 0; p < 2; ++p) cards[p] = g.playerCards(p);
+    });
     py::dict out;
     const char* names[2] = {"oop", "ip"};
     for (int p = 0; p < 2; ++p) {
@@ -299,11 +396,10 @@ class Solver {
   py::dict nodeEv(int idx) {
     pps::gpu::NodeEv ev;
     std::vector<uint8_t> cards[2];
-    {
-      py::gil_scoped_release nogil;
-      ev = s_->nodeEv(idx);
-      for (int p = 0; p < 2; ++p) cards[p] = s_->playerCards(p);
-    }
+    on([&](GpuPostflopSolver& g) {
+      ev = g.nodeEv(idx);
+      for (int p = 0; p < 2; ++p) cards[p] = g.playerCards(p);
+    });
     py::dict out;
     out["decider"] = ev.decider;
     py::list acts, aev;
@@ -334,11 +430,8 @@ class Solver {
   }
 
   py::dict runoutSummary(int idx, int action) {
-    pps::gpu::RunoutSummary rs;
-    {
-      py::gil_scoped_release nogil;
-      rs = s_->runoutSummary(idx, action);
-    }
+    const pps::gpu::RunoutSummary rs =
+        on([&](GpuPostflopSolver& g) { return g.runoutSummary(idx, action); });
     py::dict out;
     out["decider"] = rs.decider;
     py::list acts, brs;
@@ -359,24 +452,21 @@ class Solver {
   }
 
   void save(const std::string& path) const {
-    py::gil_scoped_release nogil;
-    s_->save(path);
+    on([&](GpuPostflopSolver& g) { g.save(path); });
   }
 
   py::list playerCards(int p) const {
     py::list out;
-    for (uint8_t c : s_->playerCards(p)) out.append((int)c);
+    for (uint8_t c : on([&](GpuPostflopSolver& g) { return g.playerCards(p); })) out.append((int)c);
     return out;
   }
-  py::array_t<double> playerWeights(int p) const { return arr(s_->playerWeights(p)); }
-  int numBaseCombos(int p) const { return s_->numBaseCombos(p); }
+  py::array_t<double> playerWeights(int p) const {
+    return arr(on([&](GpuPostflopSolver& g) { return g.playerWeights(p); }));
+  }
+  int numBaseCombos(int p) const { return on([&](GpuPostflopSolver& g) { return g.numBaseCombos(p); }); }
 
   py::dict treeStructure() const {
-    pps::gpu::TreeStructure t;
-    {
-      py::gil_scoped_release nogil;
-      t = s_->treeStructure();
-    }
+    const pps::gpu::TreeStructure t = on([](GpuPostflopSolver& g) { return g.treeStructure(); });
     py::list kinds, base, kids, boards;
     for (uint8_t k : t.kinds) kinds.append((int)k);
     for (int c : t.childBase) base.append(c);
@@ -393,6 +483,75 @@ class Solver {
  private:
   Solver() = default;
   std::unique_ptr<GpuPostflopSolver> s_;
+  std::shared_ptr<SpotServer> srv_;
+  int id_ = -1;
+
+  void served(const char* what) const {
+    if (!srv_) throw std::runtime_error(std::string(what) + ": this solver is not served (use pps.Server.add)");
+  }
+  // blocks until the spot's job ends; a job error is raised here
+  void finishJob() {
+    const py::dict j = wait();
+    if (!j["error"].is_none()) throw std::runtime_error(j["error"].cast<std::string>());
+  }
+};
+
+// pps.Server: owns a SpotServer; hands out served Solvers.
+class Server {
+ public:
+  Server(int64_t arenaBytes, int maxSpots, int maxStep) {
+    pps::gpu::ServerOptions o;
+    o.arenaBytes = arenaBytes;
+    o.maxSpots = maxSpots;
+    o.maxStep = maxStep;
+    py::gil_scoped_release nogil;
+    srv_ = std::make_shared<SpotServer>(o);
+  }
+
+  std::unique_ptr<Solver> add(const py::object& board, const py::object& oop, const py::object& ip,
+                              int64_t pot, int64_t stack, const py::object& bets,
+                              const py::object& raises, int maxRaises) {
+    pps::gpu::PostflopSpot spot;
+    pps::pf::BetConfig cfg;
+    readSpot(board, oop, ip, pot, stack, bets, raises, maxRaises, spot, cfg);
+    int id = -1;
+    {
+      py::gil_scoped_release nogil;
+      id = srv_->add(spot, cfg);
+    }
+    return std::make_unique<Solver>(srv_, id);
+  }
+
+  std::unique_ptr<Solver> load(const std::string& path) {
+    int id = -1;
+    {
+      py::gil_scoped_release nogil;
+      id = srv_->load(path);
+    }
+    return std::make_unique<Solver>(srv_, id);
+  }
+
+  py::dict stats() {
+    pps::gpu::ServerStats s;
+    {
+      py::gil_scoped_release nogil;
+      s = srv_->stats();
+    }
+    py::dict d;
+    d["steps"] = s.steps;
+    d["iterations"] = s.iterations;
+    d["device_ms"] = s.deviceMs;
+    d["check_ms"] = s.checkMs;
+    d["resident"] = s.resident;
+    d["running"] = s.running;
+    d["peak_running"] = s.peakRunning;
+    d["arena_bytes"] = s.arenaBytes;
+    d["free_bytes"] = s.freeBytes;
+    return d;
+  }
+
+ private:
+  std::shared_ptr<SpotServer> srv_;
 };
 
 }  // namespace
@@ -429,5 +588,21 @@ PYBIND11_MODULE(pps_native, m) {
       .def("player_cards", &Solver::playerCards, py::arg("player"))
       .def("player_weights", &Solver::playerWeights, py::arg("player"))
       .def("num_base_combos", &Solver::numBaseCombos, py::arg("player"))
-      .def("tree_structure", &Solver::treeStructure);
+      .def("tree_structure", &Solver::treeStructure)
+      .def("start_solve", &Solver::startSolve, py::arg("max_iters"), py::arg("algo") = "dcfr",
+           py::arg("target") = -1.0)
+      .def("start_continue", &Solver::startContinue, py::arg("max_iters"), py::arg("target") = -1.0)
+      .def("job", &Solver::job)
+      .def("cancel", &Solver::cancel)
+      .def("wait", &Solver::wait)
+      .def_property_readonly("spot_id", &Solver::id);
+  py::class_<Server>(m, "Server")
+      .def(py::init<int64_t, int, int>(), py::arg("arena_bytes") = 0, py::arg("max_spots") = 256,
+           py::arg("max_step") = 32)
+      .def("add", &Server::add, py::arg("board"), py::arg("oop"), py::arg("ip"), py::arg("pot"),
+           py::arg("stack"), py::arg("bets") = "0.75,a", py::arg("raises") = "2.5,3",
+           py::arg("max_raises") = 0)
+      .def("load", &Server::load, py::arg("path"))
+      .def("stats", &Server::stats);
+  py::register_exception<pps::gpu::ArenaFull>(m, "ArenaFull", PyExc_MemoryError);
 }

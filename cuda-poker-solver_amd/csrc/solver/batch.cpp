@@ -112,6 +112,16 @@ std::vector<SpotResult> serve(const std::vector<SpotRequest>& reqs, const ServeO
   Engine E;
   check(ppsg_batch_create(arena, opt.maxSpots, &E.h), "batch create");
   ppsg_t* h = E.h;
+  // Terminal group size: a batch admitted up front shares each launch among
+  // many spots, so larger groups (fewer, longer workgroups) pay off; a
+  // stream keeps few spots live and wants the parallelism of small groups.
+  int group = 4;
+  if (opt.prefill) {
+    const int together = std::min<int>(n, opt.maxLive > 0 ? opt.maxLive : opt.maxSpots);
+    while (group < 32 && group * 8 < together) group *= 2;
+  }
+  check(ppsg_tune(h, PPSG_TUNE_GROUP, group), "tune");
+  rep.groupSize = group;
 
   // ---- compile workers ----
   std::vector<Compiled> compiled(n);
@@ -277,6 +287,8 @@ std::vector<SpotResult> serve(const std::vector<SpotRequest>& reqs, const ServeO
     double ms = 0.0;
     check(ppsg_last_iterate_ms(h, &ms), "step time");
     rep.deviceMs += ms;
+    rep.stepLive.push_back((double)live.size());
+    rep.stepDeviceMs.push_back(ms);
     rep.steps += 1;
     rep.iterations += (int64_t)chunk * (int64_t)live.size();
     for (Live& L : live) L.iterations += chunk;
@@ -325,11 +337,13 @@ std::vector<SpotResult> serve(const std::vector<SpotRequest>& reqs, const ServeO
       live.swap(keep);
       int c = 0;
       const double c0 = msNow();
-      check(ppsg_batch_compact(h, 0.25, &c), "compact");
+      // device-side compaction costs a few ms: drop retired spots at once
+      check(ppsg_batch_compact(h, 0.0, &c), "compact");
       rep.compactMs += msNow() - c0;
       rep.compactions += c;
     }
     rep.walkMs += msNow() - w0;
+    rep.stepCheckMs.push_back(msNow() - w0);
     rep.busyMs += msNow() - b0;
   }
   rep.wallMs = msNow();

@@ -383,6 +383,7 @@ int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const do
     }
     pps::gpu::ServeReport rep;
     const std::vector<pps::gpu::SpotResult> res = pps::gpu::serve(reqs, o, &rep);
+    if (report) std::memset(report, 0, sizeof(*report));
     for (int i = 0; i < n; ++i) {
       const pps::gpu::SpotResult& r = res[i];
       pps_serve_result& d = out[i];
@@ -413,6 +414,12 @@ int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const do
       report->steps = rep.steps;
       report->peak_live = rep.peakLive;
       report->compactions = rep.compactions;
+      report->group_size = rep.groupSize;
+      for (size_t k = 0; k < rep.stepLive.size() && k < PPS_SERVE_TRACE; ++k) {
+        report->step_live[k] = rep.stepLive[k];
+        report->step_device_ms[k] = rep.stepDeviceMs[k];
+        report->step_check_ms[k] = rep.stepCheckMs[k];
+      }
     }
     return 0;
   });

@@ -47,24 +47,58 @@ std::vector<float> scheduleTable(int64_t from, int64_t to, int planned, const st
 struct GpuPostflopSolver::Impl {
   CompiledTree T;
   ppsg_t* eng = nullptr;
+  int slot = 0;           // the spot's slot in eng (a one-spot engine: 0)
+  bool attached = false;  // eng is a SpotServer's arena, shared with other spots
   std::string lastAlgo;
   double Z = 0.0;
   double lastMs = -1.0;
+  // rows read from a solution file before the spot is attached to an arena
+  std::vector<float> pendingReg, pendingStr;
+  int64_t pendingIters = -1;
 
   ~Impl() {
-    if (eng) ppsg_destroy(eng);
+    if (!eng) return;
+    if (attached) (void)ppsg_batch_remove(eng, slot);
+    else ppsg_destroy(eng);
   }
 
   std::vector<float> read(int buf, int64_t off, int64_t n) const {
     std::vector<float> v((size_t)n);
-    check(ppsg_read(eng, buf, off, n, v.data()));
+    check(ppsg_batch_read(eng, slot, buf, off, n, v.data()));
     return v;
+  }
+  void write(int buf, int64_t off, int64_t n, const float* v) const {
+    check(ppsg_batch_write(eng, slot, buf, off, n, v));
+  }
+  // EV / BR walk of this spot alone; cfv (n[tr] floats) may be null.
+  void walk(int tr, int mode, float* cfv) const {
+    if (!attached) {
+      check(ppsg_walk(eng, tr, mode, cfv));
+      return;
+    }
+    float* outs[1] = {cfv};
+    check(ppsg_batch_walk_only(eng, tr, mode, 1, &slot, outs));
+  }
+  void setSchedule(const std::vector<float>& tab, int64_t n) const {
+    check(ppsg_batch_set_schedule(eng, slot, tab.data(), n));
+  }
+  void setCounter(int64_t t) const { check(ppsg_batch_set_counter(eng, slot, t)); }
+  int64_t counter() const {
+    int64_t t = 0;
+    check(ppsg_batch_get_counter(eng, slot, &t));
+    return t;
+  }
+  // Calls that iterate or time a whole engine belong to one-spot engines.
+  void ownEngine(const char* what) const {
+    if (attached)
+      throw std::runtime_error(std::string(what) +
+                               ": this spot is served by a SpotServer; run it through the server");
   }
 
   // Range-weighted root value of traverser tr (EV or BR walk) / pair mass.
   double rootValue(int tr, int mode) {
     std::vector<float> cfv(T.n[tr]);
-    check(ppsg_walk(eng, tr, mode, cfv.data()));
+    walk(tr, mode, cfv.data());
     double v = 0.0;
     for (int i = 0; i < T.n[tr]; ++i) v += T.w[tr][i] * (double)cfv[i];
     return Z > 0.0 ? v / Z : 0.0;
@@ -150,29 +184,120 @@ GpuPostflopSolver::GpuPostflopSolver(const PostflopSpot& spot, const pf::BetConf
   }
 }
 
+GpuPostflopSolver::GpuPostflopSolver(Deferred, const PostflopSpot& spot, const pf::BetConfig& cfg)
+    : impl_(new Impl()) {
+  try {
+    init(spot, cfg, false);
+  } catch (...) {
+    delete impl_;
+    impl_ = nullptr;
+    throw;
+  }
+}
+
 GpuPostflopSolver::~GpuPostflopSolver() { delete impl_; }
 
-void GpuPostflopSolver::init(const PostflopSpot& spot, const pf::BetConfig& cfg) {
+// ---- server hooks ----
+
+bool GpuPostflopSolver::beginSolve(int iterations, const std::string& algo) {
+  Impl* I = impl_;
+  iterationsRun_ = 0;
+  if (I->T.empty) return false;
+  if (!knownAlgo(algo))
+    throw std::runtime_error("unknown algo '" + algo + "' (expected dcfr, dcfr-g2 or hs30)");
+  reset();
+  if (iterations <= 0) return false;
+  I->lastAlgo = algo;
+  I->lastMs = 0.0;
+  I->setSchedule(scheduleTable(0, iterations, iterations, algo), iterations);
+  return true;
+}
+
+bool GpuPostflopSolver::beginContinue(int iterations) {
+  Impl* I = impl_;
+  iterationsRun_ = 0;
+  if (I->T.empty) return false;
+  if (totalIters_ == 0) throw std::runtime_error("continueSolve: no prior solve() to continue");
+  if (I->lastAlgo == "hs30" || !knownAlgo(I->lastAlgo))
+    throw std::runtime_error("continueSolve: the last solve used '" + I->lastAlgo +
+                             "', whose schedule depends on the planned iteration count; "
+                             "continuation is defined for DCFR only");
+  if (iterations <= 0) return false;
+  const int64_t t0 = totalIters_;
+  if (t0 + iterations > (int64_t)std::numeric_limits<int>::max())
+    throw std::runtime_error("continueSolve: schedule index overflow");
+  const int64_t tEnd = t0 + iterations;
+  I->lastMs = 0.0;
+  I->setSchedule(scheduleTable(t0, tEnd, (int)tEnd, I->lastAlgo), tEnd);
+  return true;
+}
+
+void GpuPostflopSolver::advanced(int n, double deviceMs) {
+  iterationsRun_ += n;
+  totalIters_ += n;
+  impl_->lastMs += deviceMs;
+}
+
+double GpuPostflopSolver::explFrom(const float* br0, const float* br1) const {
+  const Impl* I = impl_;
+  double v[2] = {0.0, 0.0};
+  const float* rows[2] = {br0, br1};
+  for (int tr = 0; tr < 2; ++tr) {
+    for (int i = 0; i < I->T.n[tr]; ++i) v[tr] += I->T.w[tr][i] * (double)rows[tr][i];
+    v[tr] = I->Z > 0.0 ? v[tr] / I->Z : 0.0;
+  }
+  return (v[0] + v[1] - (double)I->T.spot.pot) / 2.0;
+}
+
+int GpuPostflopSolver::slot() const { return impl_->T.empty ? -1 : impl_->slot; }
+
+int GpuPostflopSolver::rootWidth(int tr) const { return impl_->T.empty ? 0 : impl_->T.n[tr]; }
+
+void GpuPostflopSolver::init(const PostflopSpot& spot, const pf::BetConfig& cfg, bool device) {
   Impl* I = impl_;
   I->T = compileSpot(spot, cfg);
   if (I->T.empty) return;
   numNodes_ = (int)I->T.numNodes;
   maxDepth_ = I->T.maxDepth;
   I->Z = pairMass(I->T);
+  if (!device) return;
   const ppsg_tree_desc d = I->T.desc();
   check(ppsg_create(&d, &I->eng));
+}
+
+void GpuPostflopSolver::attach(ppsg_t* arena) {
+  Impl* I = impl_;
+  if (I->T.empty || I->eng) return;
+  const ppsg_tree_desc d = I->T.desc();
+  int slot = -1;
+  const int rc = ppsg_batch_add(arena, &d, &slot);
+  if (rc == -2) throw ArenaFull("no room for this spot in the server's device arena");
+  check(rc);
+  I->eng = arena;
+  I->slot = slot;
+  I->attached = true;
+  check(ppsg_batch_set_active(arena, slot, 0));  // parked until a job runs it
+  if (I->pendingIters >= 0) {  // a loaded solution's rows
+    I->write(PPSG_REGRET, 0, I->T.regTotal, I->pendingReg.data());
+    I->write(PPSG_STRAT, 0, I->T.regTotal, I->pendingStr.data());
+    I->setCounter(I->pendingIters);
+    I->pendingReg = std::vector<float>();
+    I->pendingStr = std::vector<float>();
+    I->pendingIters = -1;
+  }
 }
 
 void GpuPostflopSolver::reset() {
   iterationsRun_ = 0;
   totalIters_ = 0;
   if (impl_->T.empty) return;
-  check(ppsg_reset(impl_->eng));
+  check(ppsg_batch_reset(impl_->eng, impl_->slot));
 }
 
 void GpuPostflopSolver::replayChunks(int iterations, double target) {
   Impl* I = impl_;
-  const int chunk = 128;
+  I->ownEngine("solve");
+  const int chunk = kCheckEvery;
   int done = 0;
   double ms = 0.0;
   while (done < iterations) {
@@ -198,8 +323,7 @@ void GpuPostflopSolver::solve(int iterations, const std::string& algo, double ta
   reset();
   if (iterations <= 0) return;
   I->lastAlgo = algo;
-  const std::vector<float> tab = scheduleTable(0, iterations, iterations, algo);
-  check(ppsg_set_schedule(I->eng, tab.data(), iterations));
+  I->setSchedule(scheduleTable(0, iterations, iterations, algo), iterations);
   replayChunks(iterations, target);
 }
 
@@ -217,8 +341,7 @@ void GpuPostflopSolver::continueSolve(int iterations, double target) {
   if (t0 + iterations > (int64_t)std::numeric_limits<int>::max())
     throw std::runtime_error("continueSolve: schedule index overflow");
   const int64_t tEnd = t0 + iterations;
-  const std::vector<float> tab = scheduleTable(t0, tEnd, (int)tEnd, I->lastAlgo);
-  check(ppsg_set_schedule(I->eng, tab.data(), tEnd));
+  I->setSchedule(scheduleTable(t0, tEnd, (int)tEnd, I->lastAlgo), tEnd);
   replayChunks(iterations, target);
 }
 
@@ -254,7 +377,7 @@ pf::NodeStats GpuPostflopSolver::stats() {
         rows[I->T.regOff[u] + nC + c] = 1.0f - p0;
       }
     }
-    if (!rows.empty()) check(ppsg_write(I->eng, PPSG_STRAT, 0, (int64_t)rows.size(), rows.data()));
+    if (!rows.empty()) I->write(PPSG_STRAT, 0, (int64_t)rows.size(), rows.data());
     std::fprintf(stderr, "seeded strat rows (mode %d)\n", mode);
   }
   st.ev0 = I->rootValue(0, PPSG_MODE_EV);
@@ -324,7 +447,7 @@ ComboEv GpuPostflopSolver::rootEvPerCombo() {
   if (I->T.empty) return out;
   for (int tr = 0; tr < 2; ++tr) {
     std::vector<float> cfv(I->T.n[tr]);
-    check(ppsg_walk(I->eng, tr, PPSG_MODE_EV, cfv.data()));
+    I->walk(tr, PPSG_MODE_EV, cfv.data());
     const int op = 1 - tr;
     double perCard[52] = {0.0}, tot = 0.0;
     for (int j = 0; j < I->T.n[op]; ++j) {
@@ -353,7 +476,7 @@ NodeEv GpuPostflopSolver::nodeEv(int decideIdx) {
   out.actions = I->labels(u);
   out.actionEv.assign(na, {});
   for (int tr = 0; tr < 2; ++tr) {
-    check(ppsg_walk(I->eng, tr, PPSG_MODE_EV, nullptr));
+    I->walk(tr, PPSG_MODE_EV, nullptr);
     NodeEv::Side& S = out.side[tr];
     const int cl = I->T.cls[u];
     const int nC = I->T.clsN[tr][cl];
@@ -454,7 +577,7 @@ RunoutSummary GpuPostflopSolver::runoutSummary(int decideIdx, int actionIdx) {
   out.decider = I->T.owner[fu];
   out.actions = I->labels(fu);
   for (int tr = 0; tr < 2; ++tr) {
-    check(ppsg_walk(I->eng, tr, PPSG_MODE_EV, nullptr));
+    I->walk(tr, PPSG_MODE_EV, nullptr);
     const std::vector<double>& wSelf = I->T.w[tr];
     for (int b = 0; b < nb; ++b) {
       const int bi = out.branches[b].nextDecide;
@@ -534,7 +657,7 @@ std::vector<float> GpuPostflopSolver::rows(int which) const {
 void GpuPostflopSolver::setRows(int which, const std::vector<float>& v) {
   if (impl_->T.empty) return;
   if ((int64_t)v.size() != impl_->T.regTotal) throw std::runtime_error("row size mismatch");
-  check(ppsg_write(impl_->eng, which == 0 ? PPSG_REGRET : PPSG_STRAT, 0, (int64_t)v.size(), v.data()));
+  impl_->write(which == 0 ? PPSG_REGRET : PPSG_STRAT, 0, (int64_t)v.size(), v.data());
 }
 
 void GpuPostflopSolver::halfStep(int tr, int t, const std::string& algo, int planned) {
@@ -542,13 +665,12 @@ void GpuPostflopSolver::halfStep(int tr, int t, const std::string& algo, int pla
   if (I->T.empty) return;
   if (!knownAlgo(algo)) throw std::runtime_error("unknown algo '" + algo + "'");
   if (tr != 0 && tr != 1) throw std::runtime_error("traverser must be 0 or 1");
-  int64_t saved = 0;
-  check(ppsg_get_counter(I->eng, &saved));
-  const std::vector<float> tab = scheduleTable(t, (int64_t)t + 1, planned, algo);
-  check(ppsg_set_schedule(I->eng, tab.data(), (int64_t)t + 1));
-  check(ppsg_set_counter(I->eng, t));
+  I->ownEngine("halfStep");
+  const int64_t saved = I->counter();
+  I->setSchedule(scheduleTable(t, (int64_t)t + 1, planned, algo), (int64_t)t + 1);
+  I->setCounter(t);
   check(ppsg_half_step(I->eng, tr));
-  check(ppsg_set_counter(I->eng, saved));
+  I->setCounter(saved);
 }
 
 std::vector<double> GpuPostflopSolver::profileIterations(int iters, bool detail) {
@@ -556,9 +678,9 @@ std::vector<double> GpuPostflopSolver::profileIterations(int iters, bool detail)
   if (I->T.empty || iters <= 0) return {};
   if (totalIters_ == 0 || I->lastAlgo == "hs30")
     throw std::runtime_error("profileIterations: needs a prior dcfr / dcfr-g2 solve");
+  I->ownEngine("profileIterations");
   const int64_t t0 = totalIters_, tEnd = t0 + iters;
-  const std::vector<float> tab = scheduleTable(t0, tEnd, (int)tEnd, I->lastAlgo);
-  check(ppsg_set_schedule(I->eng, tab.data(), tEnd));
+  I->setSchedule(scheduleTable(t0, tEnd, (int)tEnd, I->lastAlgo), tEnd);
   std::vector<double> out(detail ? 17 : 13, 0.0);
   check(ppsg_profile(I->eng, iters, out.data(), (int)out.size()));
   totalIters_ = tEnd;
@@ -568,6 +690,7 @@ std::vector<double> GpuPostflopSolver::profileIterations(int iters, bool detail)
 std::vector<int64_t> GpuPostflopSolver::engineInfo() const {
   std::vector<int64_t> out(6, 0);
   if (impl_->T.empty) return out;
+  impl_->ownEngine("engineInfo");
   check(ppsg_info(impl_->eng, out.data(), 6));
   return out;
 }
@@ -577,7 +700,7 @@ double GpuPostflopSolver::rootValue(int tr, int mode, std::vector<float>* rootCf
   if (I->T.empty) return 0.0;
   if (tr != 0 && tr != 1) throw std::runtime_error("traverser must be 0 or 1");
   std::vector<float> cfv(I->T.n[tr]);
-  check(ppsg_walk(I->eng, tr, mode, cfv.data()));
+  I->walk(tr, mode, cfv.data());
   double v = 0.0;
   for (int i = 0; i < I->T.n[tr]; ++i) v += I->T.w[tr][i] * (double)cfv[i];
   if (rootCfv) *rootCfv = cfv;
@@ -685,6 +808,26 @@ void GpuPostflopSolver::save(const std::string& path) const {
 
 GpuPostflopSolver::GpuPostflopSolver(const std::string& solutionPath) : impl_(new Impl()) {
   try {
+    loadFrom(solutionPath, true);
+  } catch (...) {
+    delete impl_;
+    impl_ = nullptr;
+    throw;
+  }
+}
+
+GpuPostflopSolver::GpuPostflopSolver(Deferred, const std::string& solutionPath) : impl_(new Impl()) {
+  try {
+    loadFrom(solutionPath, false);
+  } catch (...) {
+    delete impl_;
+    impl_ = nullptr;
+    throw;
+  }
+}
+
+void GpuPostflopSolver::loadFrom(const std::string& solutionPath, bool device) {
+  {
     Reader r(solutionPath);
     if (r.val<uint32_t>() != kMagic) throw std::runtime_error("not a solution file: " + solutionPath);
     const uint32_t ver = r.val<uint32_t>();
@@ -717,7 +860,7 @@ GpuPostflopSolver::GpuPostflopSolver(const std::string& solutionPath) : impl_(ne
     std::string algo((size_t)algoLen, '\0');
     r.raw(algo.data(), (size_t)algoLen);
     if (!knownAlgo(algo)) throw std::runtime_error("solution file corrupt: unknown algo '" + algo + "'");
-    init(spot, cfg);
+    init(spot, cfg, device);
     Impl* I = impl_;
     if (I->T.empty)
       throw std::runtime_error(
@@ -732,15 +875,17 @@ GpuPostflopSolver::GpuPostflopSolver(const std::string& solutionPath) : impl_(ne
     std::vector<float> reg((size_t)regTotal), str((size_t)regTotal);
     r.raw(reg.data(), reg.size() * 4);
     r.raw(str.data(), str.size() * 4);
-    check(ppsg_write(I->eng, PPSG_REGRET, 0, regTotal, reg.data()));
-    check(ppsg_write(I->eng, PPSG_STRAT, 0, regTotal, str.data()));
-    check(ppsg_set_counter(I->eng, savedIters));
+    if (device) {
+      I->write(PPSG_REGRET, 0, regTotal, reg.data());
+      I->write(PPSG_STRAT, 0, regTotal, str.data());
+      I->setCounter(savedIters);
+    } else {
+      I->pendingReg = std::move(reg);
+      I->pendingStr = std::move(str);
+      I->pendingIters = savedIters;
+    }
     I->lastAlgo = algo;
     totalIters_ = savedIters;
-  } catch (...) {
-    delete impl_;
-    impl_ = nullptr;
-    throw;
   }
 }
 

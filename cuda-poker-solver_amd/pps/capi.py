@@ -50,7 +50,9 @@ class ServeReport(ctypes.Structure):
                 ("admit_ms", ctypes.c_double), ("compact_ms", ctypes.c_double),
                 ("iterations", ctypes.c_int64),
                 ("steps", ctypes.c_int32), ("peak_live", ctypes.c_int32),
-                ("compactions", ctypes.c_int32)]
+                ("compactions", ctypes.c_int32), ("group_size", ctypes.c_int32),
+                ("step_live", ctypes.c_double * 256), ("step_device_ms", ctypes.c_double * 256),
+                ("step_check_ms", ctypes.c_double * 256)]
 
 
 # every function include/pps_capi.h and include/ppsg.h declare
@@ -73,7 +75,7 @@ PPSG_SYMBOLS = [
     "ppsg_last_iterate_ms", "ppsg_info", "ppsg_profile", "ppsg_copy_probe", "ppsg_last_error",
     "ppsg_footprint", "ppsg_batch_create", "ppsg_batch_add", "ppsg_batch_remove",
     "ppsg_batch_set_active", "ppsg_batch_set_schedule", "ppsg_batch_set_counter",
-    "ppsg_batch_get_counter", "ppsg_batch_reset", "ppsg_batch_step", "ppsg_batch_walk",
+    "ppsg_batch_get_counter", "ppsg_batch_reset", "ppsg_batch_step", "ppsg_batch_walk", "ppsg_batch_walk_only",
     "ppsg_batch_read", "ppsg_batch_write", "ppsg_batch_free_bytes", "ppsg_batch_compact",
     "ppsg_device_memory", "ppsg_tune", "ppsg_autotune", "ppsg_kind_bytes",
 ]
@@ -400,7 +402,10 @@ def serve(spots, cfgs, algo="dcfr", arrival_ms=None, **options):
             "nodes": r.nodes, "footprint": r.footprint, "arrival_ms": r.arrival_ms,
             "admit_ms": r.admit_ms, "retire_ms": r.retire_ms,
         })
-    report = {k: getattr(rep, k) for k, _ in ServeReport._fields_}
+    report = {k: getattr(rep, k) for k, _ in ServeReport._fields_ if not k.startswith("step_")}
+    nt = min(rep.steps, 256)
+    report["trace"] = [(int(rep.step_live[k]), round(rep.step_device_ms[k], 3),
+                        round(rep.step_check_ms[k], 3)) for k in range(nt)]
     return out, report
 
 

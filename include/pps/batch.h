@@ -63,6 +63,10 @@ struct ServeReport {
   double compactMs = 0.0; // dropping retired spots from the work lists
   int64_t iterations = 0; // sum of per-spot iterations
   int steps = 0, peakLive = 0, compactions = 0;
+  int groupSize = 0;  // terminal nodes per workgroup used
+  // per step: live spots, device ms of the step's replays, host ms of the
+  // step's checks / retirements
+  std::vector<double> stepLive, stepDeviceMs, stepCheckMs;
   double itersPerSec() const { return wallMs > 0.0 ? iterations / (wallMs * 1e-3) : 0.0; }
   double busyItersPerSec() const { return busyMs > 0.0 ? iterations / (busyMs * 1e-3) : 0.0; }
 };

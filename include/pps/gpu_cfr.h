@@ -13,14 +13,27 @@
 // Errors throw std::runtime_error (the reference exits the process on
 // device errors; this library reports them instead).
 #include <cstdint>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "postflop_cfr.h"
 #include "range.h"
 
+struct ppsg_engine;  // device engine handle (include/ppsg.h)
+
 namespace pps {
 namespace gpu {
+
+// Iterations between exploitability checks of a solve with a target.
+constexpr int kCheckEvery = 128;
+
+// Thrown when a SpotServer's device arena has no room for another spot.
+struct ArenaFull : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+class SpotServer;
 
 struct PostflopSpot {
   Card board[5] = {0, 0, 0, 0, 0};
@@ -153,6 +166,7 @@ class GpuPostflopSolver {
   std::vector<int64_t> engineInfo() const;
 
  private:
+  friend class SpotServer;
   struct Impl;
   Impl* impl_ = nullptr;
   int numNodes_ = 0;
@@ -160,8 +174,28 @@ class GpuPostflopSolver {
   int iterationsRun_ = 0;
   int64_t totalIters_ = 0;
 
-  void init(const PostflopSpot& spot, const pf::BetConfig& cfg);
+  // A spot for a SpotServer's arena: compiled (and a solution file read) by
+  // the constructor, uploaded by attach (queries here; iterations run by the
+  // server's batched steps).
+  struct Deferred {};
+  GpuPostflopSolver(Deferred, const PostflopSpot& spot, const pf::BetConfig& cfg);
+  GpuPostflopSolver(Deferred, const std::string& solutionPath);
+  void attach(ppsg_engine* arena);  // throws ArenaFull when there is no room
+  void init(const PostflopSpot& spot, const pf::BetConfig& cfg, bool device = true);
+  void loadFrom(const std::string& path, bool device);
   void replayChunks(int iterations, double target);
+
+  // Server hooks. beginSolve / beginContinue check arguments as solve() /
+  // continueSolve() do and set the spot's rows and schedule; advanced(n)
+  // books n iterations run by the server; explFrom turns the two root BR
+  // rows of this spot into its exploitability. slot() is -1 for an empty
+  // tree.
+  bool beginSolve(int iterations, const std::string& algo);
+  bool beginContinue(int iterations);
+  void advanced(int n, double deviceMs);
+  double explFrom(const float* br0, const float* br1) const;
+  int slot() const;
+  int rootWidth(int tr) const;
 };
 
 }  // namespace gpu

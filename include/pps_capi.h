@@ -147,8 +147,11 @@ typedef struct pps_serve_report {
   double busy_ms;           /* steps + checks + retirements: spots resident */
   double device_ms, walk_ms, admit_ms, compact_ms;
   int64_t iterations;       /* sum of per-spot iterations */
-  int32_t steps, peak_live, compactions;
+  int32_t steps, peak_live, compactions, group_size;
+  /* the first PPS_SERVE_TRACE steps: live spots, device ms, check ms */
+  double step_live[256], step_device_ms[256], step_check_ms[256];
 } pps_serve_report;
+#define PPS_SERVE_TRACE 256
 
 /* Fills options with the defaults above. */
 void pps_serve_defaults(pps_serve_options* opt);

@@ -165,6 +165,11 @@ int ppsg_batch_step(ppsg_t* h, int n);
    listed (active) spot to root_cfv[i] (n_base[tr] floats; NULL skips). */
 int ppsg_batch_walk(ppsg_t* h, int tr, int mode, int n_slots, const int* slots,
                     float* const* root_cfv);
+/* EV / BR half-step of the listed spots only, live or parked (the others'
+   rows are untouched); copies each listed spot's root cfv row to
+   root_cfv[i] (NULL entries skip). For queries on one spot of a batch. */
+int ppsg_batch_walk_only(ppsg_t* h, int tr, int mode, int n_slots, const int* slots,
+                         float* const* root_cfv);
 int ppsg_batch_read(ppsg_t* h, int slot, int buf, int64_t off, int64_t n, float* host);
 int ppsg_batch_write(ppsg_t* h, int slot, int buf, int64_t off, int64_t n, const float* host);
 /* Free arena bytes in total and in the largest block (either may be NULL). */
