@@ -5,7 +5,9 @@
 
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <memory>
+#include <optional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -16,9 +18,26 @@
 #include "pps/gpu_cfr.h"
 #include "pps/postflop_cfr.h"
 #include "pps/range.h"
+#include "pps/server.h"
 
+// A solver of its own, or a spot served by a SpotServer (then queries run
+// between the server's steps and solves are batched jobs of the server).
 struct pps_solver {
   std::unique_ptr<pps::gpu::GpuPostflopSolver> s;
+  std::shared_ptr<pps::gpu::SpotServer> srv;
+  int id = -1;
+  ~pps_solver() {
+    if (srv) {
+      try {
+        srv->remove(id);
+      } catch (...) {
+      }
+    }
+  }
+};
+
+struct pps_server {
+  std::shared_ptr<pps::gpu::SpotServer> srv;
 };
 
 namespace {
@@ -71,9 +90,26 @@ pps::pf::BetConfig toCfg(const pps_bet_config* c) {
   return cfg;
 }
 
-pps::gpu::GpuPostflopSolver& S(pps_solver* h) {
-  if (!h || !h->s) throw std::runtime_error("null solver");
-  return *h->s;
+// Runs f(solver): directly, or on the server between its steps.
+template <typename F>
+auto on(pps_solver* h, F&& f) -> decltype(f(std::declval<pps::gpu::GpuPostflopSolver&>())) {
+  using R = decltype(f(std::declval<pps::gpu::GpuPostflopSolver&>()));
+  if (!h || (!h->s && !h->srv)) throw std::runtime_error("null solver");
+  if (!h->srv) return f(*h->s);
+  std::optional<R> r;
+  h->srv->with(h->id, [&](pps::gpu::GpuPostflopSolver& g) { r.emplace(f(g)); });
+  return std::move(*r);
+}
+
+pps_solver* served(pps_solver* h, const char* what) {
+  if (!h || !h->srv) throw std::runtime_error(std::string(what) + ": not a served solver");
+  return h;
+}
+
+// Blocks until the served spot's job ends; a job error becomes the call's.
+void finishJob(pps_solver* h) {
+  const pps::gpu::JobInfo j = h->srv->wait(h->id);
+  if (!j.error.empty()) throw std::runtime_error(j.error);
 }
 
 }  // namespace
@@ -168,46 +204,73 @@ void pps_solver_destroy(pps_solver* s) { delete s; }
 
 int pps_solver_solve(pps_solver* s, int iters, const char* algo, double target) {
   return guard([&] {
-    S(s).solve(iters, algo ? algo : "dcfr", target);
-    return 0;
+    const std::string a = algo ? algo : "dcfr";
+    if (s && s->srv) {
+      s->srv->solve(s->id, iters, a, target);
+      finishJob(s);
+      return 0;
+    }
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) {
+      g.solve(iters, a, target);
+      return 0;
+    });
   });
 }
 
 int pps_solver_continue(pps_solver* s, int iters, double target) {
   return guard([&] {
-    S(s).continueSolve(iters, target);
-    return 0;
+    if (s && s->srv) {
+      s->srv->continueSolve(s->id, iters, target);
+      finishJob(s);
+      return 0;
+    }
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) {
+      g.continueSolve(iters, target);
+      return 0;
+    });
   });
 }
 
 int pps_solver_reset(pps_solver* s) {
   return guard([&] {
-    S(s).reset();
-    return 0;
+    if (s && s->srv) {
+      s->srv->reset(s->id);
+      return 0;
+    }
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) {
+      g.reset();
+      return 0;
+    });
   });
 }
 
 int pps_solver_iterations_run(pps_solver* s) {
-  return guard([&] { return S(s).iterationsRun(); });
+  return guard([&] {
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) { return g.iterationsRun(); });
+  });
 }
 
 int64_t pps_solver_total_iterations(pps_solver* s) {
   int64_t v = -1;
   guard([&] {
-    v = S(s).totalIterations();
-    return 0;
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) {
+      v = g.totalIterations();
+      return 0;
+    });
   });
   return v;
 }
 
 int pps_solver_stats(pps_solver* s, double out4[4]) {
   return guard([&] {
-    const pps::pf::NodeStats st = S(s).stats();
-    out4[0] = st.ev0;
-    out4[1] = st.ev1;
-    out4[2] = st.expl;
-    out4[3] = (double)st.pairMass;
-    return 0;
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) {
+      const pps::pf::NodeStats st = g.stats();
+      out4[0] = st.ev0;
+      out4[1] = st.ev1;
+      out4[2] = st.expl;
+      out4[3] = (double)st.pairMass;
+      return 0;
+    });
   });
 }
 
@@ -218,128 +281,160 @@ static int copyOut(const std::vector<double>& v, double* out, int cap) {
 }
 
 int pps_solver_root_strategy(pps_solver* s, double* out, int cap) {
-  return guard([&] { return copyOut(S(s).rootStrategy(), out, cap); });
+  return guard([&] {
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) { return copyOut(g.rootStrategy(), out, cap); });
+  });
 }
 
 int pps_solver_node_strategy(pps_solver* s, int idx, double* out, int cap) {
-  return guard([&] { return copyOut(S(s).nodeStrategy(idx), out, cap); });
+  return guard([&] {
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) { return copyOut(g.nodeStrategy(idx), out, cap); });
+  });
 }
 
 int pps_solver_num_decide_nodes(pps_solver* s) {
-  return guard([&] { return S(s).numDecideNodes(); });
+  return guard([&] {
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) { return g.numDecideNodes(); });
+  });
 }
 
 int pps_solver_decide_node(pps_solver* s, int idx, int32_t info6[6], int64_t* actions, int cap) {
   return guard([&] {
-    const pps::gpu::DecideNodeInfo d = S(s).decideNode(idx);
-    info6[0] = d.nodeId;
-    info6[1] = d.player;
-    info6[2] = d.depth;
-    info6[3] = d.nBoard;
-    info6[4] = d.nCombos;
-    info6[5] = (int32_t)d.actions.size();
-    if ((int)d.actions.size() * 3 > cap) throw std::runtime_error("output too small");
-    for (size_t a = 0; a < d.actions.size(); ++a) {
-      actions[3 * a] = (int64_t)d.actions[a].kind;
-      actions[3 * a + 1] = d.actions[a].amount;
-      actions[3 * a + 2] = d.children[a];
-    }
-    return 0;
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) {
+      const pps::gpu::DecideNodeInfo d = g.decideNode(idx);
+      info6[0] = d.nodeId;
+      info6[1] = d.player;
+      info6[2] = d.depth;
+      info6[3] = d.nBoard;
+      info6[4] = d.nCombos;
+      info6[5] = (int32_t)d.actions.size();
+      if ((int)d.actions.size() * 3 > cap) throw std::runtime_error("output too small");
+      for (size_t a = 0; a < d.actions.size(); ++a) {
+        actions[3 * a] = (int64_t)d.actions[a].kind;
+        actions[3 * a + 1] = d.actions[a].amount;
+        actions[3 * a + 2] = d.children[a];
+      }
+      return 0;
+    });
   });
 }
 
 int pps_solver_combo_strategy(pps_solver* s, int idx, int32_t* combos, double* freqs, int cap) {
   return guard([&] {
-    const pps::gpu::ComboStrategy cs = S(s).comboStrategy(idx);
-    if (cs.combos.size() > (size_t)cap || cs.freqs.size() > (size_t)cap * 32)
-      throw std::runtime_error("output too small");
-    std::copy(cs.combos.begin(), cs.combos.end(), combos);
-    std::copy(cs.freqs.begin(), cs.freqs.end(), freqs);
-    return (int)cs.combos.size();
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) {
+      const pps::gpu::ComboStrategy cs = g.comboStrategy(idx);
+      if (cs.combos.size() > (size_t)cap || cs.freqs.size() > (size_t)cap * 32)
+        throw std::runtime_error("output too small");
+      std::copy(cs.combos.begin(), cs.combos.end(), combos);
+      std::copy(cs.freqs.begin(), cs.freqs.end(), freqs);
+      return (int)cs.combos.size();
+    });
   });
 }
 
 int pps_solver_num_nodes(pps_solver* s) {
-  return guard([&] { return S(s).numNodes(); });
+  return guard([&] {
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) { return g.numNodes(); });
+  });
 }
 
 int pps_solver_max_depth(pps_solver* s) {
-  return guard([&] { return S(s).maxDepth(); });
+  return guard([&] {
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) { return g.maxDepth(); });
+  });
 }
 
 int pps_solver_num_base_combos(pps_solver* s, int player) {
-  return guard([&] { return S(s).numBaseCombos(player); });
+  return guard([&] {
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) { return g.numBaseCombos(player); });
+  });
 }
 
 int pps_solver_save(pps_solver* s, const char* path) {
   return guard([&] {
-    S(s).save(path ? path : "");
-    return 0;
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) {
+      g.save(path ? path : "");
+      return 0;
+    });
   });
 }
 
 int64_t pps_solver_read_rows(pps_solver* s, int which, float* out, int64_t cap) {
   int64_t n = -1;
   guard([&] {
-    const std::vector<float> r = S(s).rows(which);
-    if (!out) {  // size query
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) {
+      const std::vector<float> r = g.rows(which);
+      if (!out) {  // size query
+        n = (int64_t)r.size();
+        return 0;
+      }
+      if ((int64_t)r.size() > cap) throw std::runtime_error("output too small");
+      std::copy(r.begin(), r.end(), out);
       n = (int64_t)r.size();
       return 0;
-    }
-    if ((int64_t)r.size() > cap) throw std::runtime_error("output too small");
-    std::copy(r.begin(), r.end(), out);
-    n = (int64_t)r.size();
-    return 0;
+    });
   });
   return n;
 }
 
 int pps_solver_write_rows(pps_solver* s, int which, const float* in, int64_t n) {
   return guard([&] {
-    S(s).setRows(which, std::vector<float>(in, in + n));
-    return 0;
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) {
+      g.setRows(which, std::vector<float>(in, in + n));
+      return 0;
+    });
   });
 }
 
 int pps_solver_half_step(pps_solver* s, int tr, int t, const char* algo, int planned) {
   return guard([&] {
-    S(s).halfStep(tr, t, algo ? algo : "dcfr", planned);
-    return 0;
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) {
+      g.halfStep(tr, t, algo ? algo : "dcfr", planned);
+      return 0;
+    });
   });
 }
 
 int pps_solver_root_value(pps_solver* s, int tr, int mode, float* root_cfv, double* value) {
   return guard([&] {
-    std::vector<float> cfv;
-    const double v = S(s).rootValue(tr, mode, &cfv);
-    if (root_cfv) std::copy(cfv.begin(), cfv.end(), root_cfv);
-    if (value) *value = v;
-    return 0;
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) {
+      std::vector<float> cfv;
+      const double v = g.rootValue(tr, mode, &cfv);
+      if (root_cfv) std::copy(cfv.begin(), cfv.end(), root_cfv);
+      if (value) *value = v;
+      return 0;
+    });
   });
 }
 
 double pps_solver_iteration_bytes(pps_solver* s) {
   double v = -1.0;
   guard([&] {
-    v = S(s).iterationBytes();
-    return 0;
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) {
+      v = g.iterationBytes();
+      return 0;
+    });
   });
   return v;
 }
 
 int pps_solver_profile(pps_solver* s, int iters, int detail, double* out) {
   return guard([&] {
-    const std::vector<double> v = S(s).profileIterations(iters, detail != 0);
-    for (size_t i = 0; i < v.size(); ++i) out[i] = v[i];
-    return 0;
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) {
+      const std::vector<double> v = g.profileIterations(iters, detail != 0);
+      for (size_t i = 0; i < v.size(); ++i) out[i] = v[i];
+      return 0;
+    });
   });
 }
 
 int pps_solver_engine_info(pps_solver* s, int64_t out6[6]) {
   return guard([&] {
-    const std::vector<int64_t> v = S(s).engineInfo();
-    for (size_t i = 0; i < 6; ++i) out6[i] = v[i];
-    return 0;
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) {
+      const std::vector<int64_t> v = g.engineInfo();
+      for (size_t i = 0; i < 6; ++i) out6[i] = v[i];
+      return 0;
+    });
   });
 }
 
@@ -463,11 +558,122 @@ int pps_compile_footprint(const pps_spot* spot, const pps_bet_config* cfg, int64
   });
 }
 
+int pps_server_create(const pps_server_options* opt, pps_server** out) {
+  return guard([&] {
+    if (!out) throw std::runtime_error("null output");
+    pps::gpu::ServerOptions o;
+    if (opt) {
+      o.arenaBytes = opt->arena_bytes;
+      if (opt->max_spots > 0) o.maxSpots = opt->max_spots;
+      if (opt->max_step > 0) o.maxStep = opt->max_step;
+    }
+    auto h = std::make_unique<pps_server>();
+    h->srv = std::make_shared<pps::gpu::SpotServer>(o);
+    *out = h.release();
+    return 0;
+  });
+}
+
+void pps_server_destroy(pps_server* srv) { delete srv; }
+
+static int admitted(pps_server* srv, pps_solver** out, const std::function<int()>& add) {
+  if (!srv || !srv->srv || !out) throw std::runtime_error("null argument");
+  int id = -1;
+  try {
+    id = add();
+  } catch (const pps::gpu::ArenaFull& e) {
+    g_err = e.what();
+    return PPS_ARENA_FULL;
+  }
+  auto h = std::make_unique<pps_solver>();
+  h->srv = srv->srv;
+  h->id = id;
+  *out = h.release();
+  return 0;
+}
+
+int pps_server_add(pps_server* srv, const pps_spot* spot, const pps_bet_config* cfg,
+                   pps_solver** out) {
+  return guard([&] {
+    const pps::gpu::PostflopSpot sp = toSpot(spot);
+    const pps::pf::BetConfig c = toCfg(cfg);
+    return admitted(srv, out, [&] { return srv->srv->add(sp, c); });
+  });
+}
+
+int pps_server_load(pps_server* srv, const char* path, pps_solver** out) {
+  return guard([&] {
+    if (!path) throw std::runtime_error("null path");
+    const std::string p(path);
+    return admitted(srv, out, [&] { return srv->srv->load(p); });
+  });
+}
+
+int pps_server_stats(pps_server* srv, pps_server_counters* out) {
+  return guard([&] {
+    if (!srv || !srv->srv || !out) throw std::runtime_error("null argument");
+    const pps::gpu::ServerStats s = srv->srv->stats();
+    std::memset(out, 0, sizeof(*out));
+    out->steps = s.steps;
+    out->iterations = s.iterations;
+    out->device_ms = s.deviceMs;
+    out->check_ms = s.checkMs;
+    out->resident = s.resident;
+    out->running = s.running;
+    out->peak_running = s.peakRunning;
+    out->arena_bytes = s.arenaBytes;
+    out->free_bytes = s.freeBytes;
+    return 0;
+  });
+}
+
+int pps_solver_start_solve(pps_solver* s, int iters, const char* algo, double target) {
+  return guard([&] {
+    served(s, "start_solve")->srv->solve(s->id, iters, algo ? algo : "dcfr", target);
+    return 0;
+  });
+}
+
+int pps_solver_start_continue(pps_solver* s, int iters, double target) {
+  return guard([&] {
+    served(s, "start_continue")->srv->continueSolve(s->id, iters, target);
+    return 0;
+  });
+}
+
+int pps_solver_job(pps_solver* s, int wait, pps_job* out) {
+  return guard([&] {
+    if (!out) throw std::runtime_error("null output");
+    served(s, "job");
+    const pps::gpu::JobInfo j = wait ? s->srv->wait(s->id) : s->srv->job(s->id);
+    std::memset(out, 0, sizeof(*out));
+    out->running = j.running ? 1 : 0;
+    out->status = j.status == "running"     ? PPS_JOB_RUNNING
+                  : j.status == "done"      ? PPS_JOB_DONE
+                  : j.status == "cancelled" ? PPS_JOB_CANCELLED
+                  : j.status == "error"     ? PPS_JOB_ERROR
+                                            : PPS_JOB_NONE;
+    out->kind = j.kind == "solve" ? 1 : j.kind == "continue" ? 2 : 0;
+    out->start = j.start;
+    out->done = j.done;
+    out->total = j.total;
+    out->target = j.target;
+    if (!j.error.empty()) throw std::runtime_error(j.error);
+    return 0;
+  });
+}
+
+int pps_solver_cancel(pps_solver* s) {
+  return guard([&] { return served(s, "cancel")->srv->cancel(s->id) ? 1 : 0; });
+}
+
 double pps_solver_last_solve_ms(pps_solver* s) {
   double v = -1.0;
   guard([&] {
-    v = S(s).lastSolveDeviceMs();
-    return 0;
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) {
+      v = g.lastSolveDeviceMs();
+      return 0;
+    });
   });
   return v;
 }

@@ -45,6 +45,8 @@ class SpotServer::Turn {
     s_.wake_.notify_all();
     lk_ = std::unique_lock<std::mutex>(s_.mu_);
     s_.waiting_.fetch_sub(1);
+    // the driver re-checks its wait for callers only when signalled
+    s_.callers_.notify_all();
   }
   ~Turn() {
     lk_.unlock();
@@ -148,10 +150,10 @@ void SpotServer::start(Entry& e, const std::string& kind, int64_t t0, int total,
   const int slot = e.s->slot();
   if (slot < 0 || run <= 0) {  // nothing to run (empty tree, no iterations)
     e.job.status = "done";
-    ended_.notify_all();
     return;
   }
   check(ppsg_batch_set_active(eng_, slot, 1));
+  e.done = std::make_shared<Done>();
   e.job.running = true;
   e.job.status = "running";
   wake_.notify_all();
@@ -193,12 +195,16 @@ JobInfo SpotServer::job(int id) {
 }
 
 JobInfo SpotServer::wait(int id) {
-  Turn t(*this);
-  ended_.wait(t.lock(), [&] {
-    auto it = spots_.find(id);
-    return it == spots_.end() || !it->second.job.running;
-  });
-  return entry(id).job;
+  std::shared_ptr<Done> d;
+  {
+    Turn t(*this);
+    Entry& e = entry(id);
+    if (!e.job.running) return e.job;
+    d = e.done;
+  }
+  std::unique_lock<std::mutex> l(d->mu);
+  d->cv.wait(l, [&] { return d->ended; });
+  return d->final;
 }
 
 void SpotServer::reset(int id) {
@@ -232,7 +238,15 @@ void SpotServer::finish(Entry& e, const std::string& status, const std::string& 
   const int slot = e.s->slot();
   if (slot >= 0 && ppsg_batch_set_active(eng_, slot, 0) != 0 && e.job.error.empty())
     e.job.error = std::string("park failed: ") + ppsg_last_error();
-  ended_.notify_all();
+  if (e.done) {
+    {
+      std::lock_guard<std::mutex> l(e.done->mu);
+      e.done->final = e.job;
+      e.done->ended = true;
+    }
+    e.done->cv.notify_all();
+    e.done.reset();
+  }
 }
 
 void SpotServer::run() {

@@ -223,10 +223,11 @@ def _job(e: Entry) -> Optional[Dict[str, Any]]:
 
 def _meta(sid: str, e: Entry) -> Dict[str, Any]:
     s = e.solver
+    job = _job(e)  # first: once it reads ended, the counters below are final
     return {"id": sid, "board": e.board, "num_nodes": s.num_nodes,
             "num_decide_nodes": s.num_decide_nodes, "max_depth": s.max_depth,
             "total_iterations": s.total_iterations, "iterations_run": s.iterations_run,
-            "job": _job(e)}
+            "job": job}
 
 
 def _target(t: Optional[float]) -> float:

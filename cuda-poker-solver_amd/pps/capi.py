@@ -67,6 +67,9 @@ CAPI_SYMBOLS = [
     "pps_solver_write_rows", "pps_solver_half_step", "pps_solver_root_value",
     "pps_solver_iteration_bytes", "pps_solver_last_solve_ms", "pps_solver_profile",
     "pps_solver_engine_info", "pps_serve_defaults", "pps_serve", "pps_compile_footprint",
+    "pps_server_create", "pps_server_destroy", "pps_server_add", "pps_server_load",
+    "pps_server_stats", "pps_solver_start_solve", "pps_solver_start_continue", "pps_solver_job",
+    "pps_solver_cancel",
     "pps_batch_profile",
 ]
 PPSG_SYMBOLS = [
@@ -143,8 +146,83 @@ def lib():
         L.pps_batch_profile.argtypes = [ctypes.c_int, P(Spot), P(BetConfig), ctypes.c_char_p,
                                         ctypes.c_int, ctypes.c_int64, ctypes.c_int,
                                         P(ctypes.c_double)]
+        L.pps_server_create.argtypes = [P(ServerOptions), P(ctypes.c_void_p)]
+        L.pps_server_destroy.restype = None
+        L.pps_server_destroy.argtypes = [ctypes.c_void_p]
+        L.pps_server_add.argtypes = [ctypes.c_void_p, P(Spot), P(BetConfig), P(ctypes.c_void_p)]
+        L.pps_server_load.argtypes = [ctypes.c_void_p, ctypes.c_char_p, P(ctypes.c_void_p)]
+        L.pps_server_stats.argtypes = [ctypes.c_void_p, P(ServerCounters)]
+        L.pps_solver_start_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p,
+                                             ctypes.c_double]
+        L.pps_solver_start_continue.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        L.pps_solver_job.argtypes = [ctypes.c_void_p, ctypes.c_int, P(Job)]
+        L.pps_solver_cancel.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
+
+
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("arena_bytes", ctypes.c_int64), ("max_spots", ctypes.c_int32),
+                ("max_step", ctypes.c_int32)]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("running", ctypes.c_int32), ("status", ctypes.c_int32), ("kind", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("start", ctypes.c_int64), ("done", ctypes.c_int64),
+                ("total", ctypes.c_int64), ("target", ctypes.c_double)]
+
+
+class ServerCounters(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int64), ("iterations", ctypes.c_int64),
+                ("device_ms", ctypes.c_double), ("check_ms", ctypes.c_double),
+                ("resident", ctypes.c_int32), ("running", ctypes.c_int32),
+                ("peak_running", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("arena_bytes", ctypes.c_int64), ("free_bytes", ctypes.c_int64)]
+
+
+JOB_STATUS = {0: None, 1: "running", 2: "done", 3: "cancelled", 4: "error"}
+ARENA_FULL = 2
+
+
+class Server:
+    """The spot server through the C ABI (pps_server_*); add / load give
+    served Solver handles."""
+
+    def __init__(self, arena_bytes=0, max_spots=256, max_step=32):
+        o = ServerOptions(int(arena_bytes), int(max_spots), int(max_step))
+        h = ctypes.c_void_p()
+        _check(lib().pps_server_create(ctypes.byref(o), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pps_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _served(self, rc, h):
+        if rc == ARENA_FULL:
+            raise MemoryError(lib().pps_last_error().decode())
+        _check(rc)
+        s = Solver.__new__(Solver)
+        s.h = h
+        return s
+
+    def add(self, spot, cfg):
+        h = ctypes.c_void_p()
+        return self._served(lib().pps_server_add(self.h, ctypes.byref(spot), ctypes.byref(cfg),
+                                                 ctypes.byref(h)), h)
+
+    def load(self, path):
+        h = ctypes.c_void_p()
+        return self._served(lib().pps_server_load(self.h, path.encode(), ctypes.byref(h)), h)
+
+    def stats(self):
+        c = ServerCounters()
+        _check(lib().pps_server_stats(self.h, ctypes.byref(c)))
+        return {k: getattr(c, k) for k, _ in ServerCounters._fields_ if k != "pad"}
 
 
 def copy_probe(nbytes, reps):
@@ -263,6 +341,23 @@ class Solver:
 
     def reset(self):
         _check(lib().pps_solver_reset(self.h))
+
+    # jobs of a served solver (Server.add)
+    def start_solve(self, iters, algo="dcfr", target=-1.0):
+        _check(lib().pps_solver_start_solve(self.h, int(iters), algo.encode(), float(target)))
+
+    def start_continue(self, iters, target=-1.0):
+        _check(lib().pps_solver_start_continue(self.h, int(iters), float(target)))
+
+    def job(self, wait=False):
+        j = Job()
+        _check(lib().pps_solver_job(self.h, int(bool(wait)), ctypes.byref(j)))
+        return {"running": bool(j.running), "status": JOB_STATUS[j.status],
+                "kind": {0: None, 1: "solve", 2: "continue"}[j.kind], "start": j.start,
+                "done": j.done, "total": j.total, "target": j.target}
+
+    def cancel(self):
+        return bool(_check(lib().pps_solver_cancel(self.h)))
 
     @property
     def iterations_run(self):

@@ -96,12 +96,21 @@ class SpotServer {
   ServerStats stats();
 
  private:
+  // End-of-job signal with its own lock: waiters sleep on it, never on the
+  // server lock that the driver holds across consecutive steps.
+  struct Done {
+    std::mutex mu;
+    std::condition_variable cv;
+    bool ended = false;
+    JobInfo final;  // the job as it ended
+  };
   struct Entry {
     std::unique_ptr<GpuPostflopSolver> s;
     JobInfo job;
     bool cancel = false;
     int64_t nextStop = 0;  // job iterations at which it next checks or ends
     double target = -1.0;
+    std::shared_ptr<Done> done;  // the running job's signal
   };
 
   class Turn;  // holds mu_ for a caller; the driver yields to waiting callers
@@ -110,7 +119,6 @@ class SpotServer {
   ppsg_engine* eng_ = nullptr;
   std::mutex mu_;
   std::condition_variable wake_;     // driver: a job started or a caller waits
-  std::condition_variable ended_;    // waiters: a job ended
   std::condition_variable callers_;  // driver: no caller waits any more
   std::atomic<int> waiting_{0};
   int group_ = 4;                    // terminal group size of the engine

@@ -178,6 +178,62 @@ int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, 
 /* Device bytes the compiled spot takes in an arena (host only). */
 int pps_compile_footprint(const pps_spot* spot, const pps_bet_config* cfg, int64_t* bytes);
 
+/* ---- spot server (no reference counterpart as a call; it replaces the
+   global solve lock of the reference's HTTP layer, python/pps_api.py:165,
+   by batching) ----
+   Spots registered with a server stay resident in its device arena; a
+   driver thread advances every spot with a running job together. A served
+   spot is a pps_solver: every pps_solver_* query works on it (between the
+   server's steps), pps_solver_solve / pps_solver_continue run as batched
+   jobs of the server and return when the job ends, with results equal bit
+   for bit to a solver of its own. pps_solver_destroy removes it from the
+   server. Calls from many threads at once are safe; jobs started from
+   different threads share the GPU's launches. */
+typedef struct pps_server pps_server;
+typedef struct pps_server_options {
+  int64_t arena_bytes;  /* 0: 80 % of free device memory */
+  int32_t max_spots;    /* resident spots at most (256) */
+  int32_t max_step;     /* iterations per batched step at most (32) */
+} pps_server_options;
+typedef struct pps_job {
+  int32_t running;
+  int32_t status;   /* PPS_JOB_* */
+  int32_t kind;     /* 0 none, 1 solve, 2 continue */
+  int32_t pad;
+  int64_t start;    /* total iterations when the job began */
+  int64_t done;     /* iterations run by the job */
+  int64_t total;    /* iterations asked for */
+  double target;    /* exploitability target in chips, <= 0: none */
+} pps_job;
+enum { PPS_JOB_NONE = 0, PPS_JOB_RUNNING = 1, PPS_JOB_DONE = 2, PPS_JOB_CANCELLED = 3,
+       PPS_JOB_ERROR = 4 };
+typedef struct pps_server_counters {
+  int64_t steps, iterations;  /* batched steps; spot-iterations run */
+  double device_ms, check_ms;
+  int32_t resident, running, peak_running, pad;
+  int64_t arena_bytes, free_bytes;
+} pps_server_counters;
+/* pps_server_add / pps_server_load return this when the arena is full */
+#define PPS_ARENA_FULL 2
+
+int pps_server_create(const pps_server_options* opt, pps_server** out); /* opt may be NULL */
+/* Releases the handle; the server stops when its last served solver is
+   destroyed too (running jobs are cancelled then). */
+void pps_server_destroy(pps_server* srv);
+int pps_server_add(pps_server* srv, const pps_spot* spot, const pps_bet_config* cfg,
+                   pps_solver** out);
+int pps_server_load(pps_server* srv, const char* path, pps_solver** out);
+int pps_server_stats(pps_server* srv, pps_server_counters* out);
+/* Jobs of a served solver: start and return at once (-1 with a running
+   job); pps_solver_job reads the job (wait != 0: after it ends); cancel
+   returns 1 if a running job was asked to stop, 0 if none ran. A job's
+   error message is left in pps_last_error by pps_solver_job, which then
+   returns -1. */
+int pps_solver_start_solve(pps_solver* s, int iters, const char* algo, double target);
+int pps_solver_start_continue(pps_solver* s, int iters, double target);
+int pps_solver_job(pps_solver* s, int wait, pps_job* out);
+int pps_solver_cancel(pps_solver* s);
+
 #ifdef __cplusplus
 }
 #endif

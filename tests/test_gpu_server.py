@@ -63,13 +63,15 @@ def test_jobs_started_together_match_own_solvers(expected):
     served = [srv.add(**spec(i)) for i, _, _, _ in JOBS]
     # a long companion job keeps the batch wide while the others start
     companion = srv.add(**spec(0))
-    companion.start_solve(50000)
+    companion.start_solve(2000000)
     for s, (i, it, algo, target) in zip(served, JOBS):
         j = s.start_solve(it, algo, target)
         assert j["kind"] == "solve" and j["total"] == it
     for s in served:
         j = s.wait()
         assert j["status"] == "done" and not j["running"] and j["error"] is None
+    j = companion.job()
+    assert j["running"] and j["status"] == "running", j
     assert companion.cancel()
     assert companion.wait()["status"] == "cancelled"
     got = [result(s) for s in served]
@@ -177,3 +179,40 @@ def test_arena_full_and_removal():
     assert srv.stats()["resident"] == n - 1
     held.append(srv.add(**spec(1)))
     assert srv.stats()["resident"] == n
+
+
+def test_c_abi_server_matches_own_solvers():
+    """The same server through the C ABI (pps_server_*): served handles
+    answer every pps_solver_* query, and their solves equal solvers of
+    their own."""
+    from pps import capi, configs
+    spots = [configs.batch_turn(sd) for sd in (1001, 1013, 1002)] + [configs.river_spot(3003)]
+
+    def pair(s):
+        return (capi.make_spot(s["board"], s["w0"], s["w1"], s["pot"], s["stack"]),
+                capi.make_config(s["bets"], s["raises"], s["allin"], False, s["max_raises"]))
+
+    want = []
+    for s in spots:
+        own = capi.Solver(*pair(s))
+        own.solve(256, "dcfr", -1.0)
+        want.append((own.total_iterations, own.stats(), own.root_strategy()))
+        own.close()
+    srv = capi.Server(arena_bytes=ARENA)
+    served = [srv.add(*pair(s)) for s in spots]
+    for h in served:
+        h.start_solve(256)
+        with pytest.raises(capi.CapiError):
+            h.start_solve(10)  # one job at a time
+    for h in served:
+        j = h.job(wait=True)
+        assert j["status"] == "done" and j["done"] == 256 and not j["running"], j
+    got = [(h.total_iterations, h.stats(), h.root_strategy()) for h in served]
+    assert got == want
+    assert srv.stats()["resident"] == len(spots) and srv.stats()["peak_running"] >= 1
+    served[0].solve(100)  # blocking solve through the same call as a solver of its own
+    assert served[0].total_iterations == 100
+    for h in served:
+        h.close()
+    assert srv.stats()["resident"] == 0
+    srv.close()
