@@ -229,6 +229,10 @@ def bench_batched(args, capi, configs):
         serving["mix"] = "70% batched-turn spots, 30% river spots"
         log(f"bench: serving {json.dumps(serving)}")
 
+    http = None
+    if args.http_spots > 0:
+        http = bench_http(spots[:args.http_spots], args.http_clients, args.steps)
+
     prof = capi.batch_profile(S, C, args.profile_iters)
     log(f"bench: profile {json.dumps(prof)}")
     roof = roofline_batch(prof)
@@ -255,6 +259,7 @@ def bench_batched(args, capi, configs):
             "single": single,
             "batched_vs_single": round(value / single["iters_per_s_busy"], 3) if single else None,
             "serving": serving,
+            "http": http,
             "resident_batch": {
                 "graph_ms_per_batched_iter": round(prof["graph_ms_per_iter"], 4),
                 "iters_per_s": round(prof["spots"] / (prof["graph_ms_per_iter"] * 1e-3), 1),
@@ -273,6 +278,74 @@ def bench_batched(args, capi, configs):
                                                   args.cpu_seconds)
     print(json.dumps(line), flush=True)
     return 0
+
+
+def card_text(c):
+    return "23456789TJQKA"[c % 13] + "cdhs"[c // 13]
+
+
+def http_body(spot):
+    """A stream spot as the HTTP API's SpotSpec (ranges as 1326 weights)."""
+    return {"board": "".join(card_text(c) for c in spot["board"]),
+            "oop": [float(x) for x in spot["w0"]], "ip": [float(x) for x in spot["w1"]],
+            "pot": int(spot["pot"]), "stack": int(spot["stack"]),
+            "bets": list(spot["bets"]) + (["a"] if spot["allin"] else []),
+            "raises": list(spot["raises"]), "max_raises": int(spot["max_raises"])}
+
+
+def bench_http(spots, clients, max_iters):
+    """HTTP front end, in process (FastAPI TestClient): each spot is one
+    client session (create, solve to 0.5 % pot with wait, delete) on
+    `clients` threads. Concurrent: solves batched by the spot server.
+    Serialized: the same requests with every solve behind one lock, as the
+    reference's API runs them (python/pps_api.py:165,183-186)."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+
+    from fastapi.testclient import TestClient
+
+    from pps import api
+    api.ARENA_GB = 16.0
+    client = TestClient(api.app)
+    bodies = [http_body(s) for s in spots]
+
+    def session(k, lock):
+        t0 = time.perf_counter()
+        r = client.post("/solvers", json=bodies[k])
+        if r.status_code != 201:
+            raise RuntimeError(f"create: {r.status_code} {r.text[:200]}")
+        sid = r.json()["id"]
+        req = {"max_iters": max_iters, "target": TARGET * bodies[k]["pot"]}
+        if lock is None:
+            r = client.post(f"/solvers/{sid}/solve", json=req)
+        else:
+            with lock:
+                r = client.post(f"/solvers/{sid}/solve", json=req)
+        if r.status_code != 200:
+            raise RuntimeError(f"solve: {r.status_code} {r.text[:200]}")
+        its = r.json()["iterations_run"]
+        client.delete(f"/solvers/{sid}")
+        return its, (time.perf_counter() - t0) * 1e3
+
+    out = {"spots": len(spots), "clients": clients, "max_iters": max_iters}
+    for mode, lock in (("concurrent", None), ("serialized", threading.Lock())):
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=clients) as ex:
+            got = list(ex.map(lambda k: session(k, lock), range(len(spots))))
+        wall = time.perf_counter() - t0
+        lat = sorted(g[1] for g in got)
+        its = sum(g[0] for g in got)
+        out[mode] = {"iterations": its, "wall_s": round(wall, 3),
+                     "iters_per_s": round(its / wall, 1),
+                     "p50_ms": round(lat[len(lat) // 2], 2),
+                     "p99_ms": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))], 2)}
+        log(f"bench: http {mode} {json.dumps(out[mode])}")
+    out["concurrent_vs_serialized"] = round(out["concurrent"]["iters_per_s"]
+                                            / out["serialized"]["iters_per_s"], 3)
+    srv = api.server().stats()
+    out["server"] = {"steps": srv["steps"], "peak_running": srv["peak_running"]}
+    api._SERVER = None
+    return out
 
 
 def bench_single(args, capi, configs):
@@ -330,6 +403,9 @@ def main():
     ap.add_argument("--serve-spots", type=int, default=1000)
     ap.add_argument("--qps", type=float, default=40.0)
     ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--http-spots", type=int, default=64,
+                    help="spots of the batched stream sent through the HTTP API (0: skip)")
+    ap.add_argument("--http-clients", type=int, default=16)
     ap.add_argument("--profile-iters", type=int, default=8)
     ap.add_argument("--profile-only", action="store_true",
                     help="only the launch-level profile of the resident batch")
