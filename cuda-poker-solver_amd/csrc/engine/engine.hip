@@ -56,6 +56,7 @@ constexpr int kLists = 5;  // per traverser: forward, backward, fold, showdown, 
 constexpr int kGatherHeader = 256;  // bytes: 53 int32 segment bounds + pad
 // fold: entries of a card's segment per lane (4 lanes per card)
 constexpr int kFoldPart = 12;
+constexpr int kMaxGroup = 64;  // terminal nodes per work item at most
 // combos per lane (kTile apart) of forward decide, backward decide and
 // forward chance items (chosen by measurement: more loads in flight per lane
 // help the read-mostly forward items; the read-modify-write backward tiles
@@ -240,21 +241,44 @@ __device__ __forceinline__ void fwdItem(const Dev& d, const Item& it, int fromSt
 // across the group).
 // ---------------------------------------------------------------------------
 
-// Showdown prefix sums (blocked scan over gathered f32 reach).
-using SdAcc = double;
-
 // Loads node g's opponent reach row (ROWS floats per lane, NT lanes) when
 // g < cnt.
 template <int ROWS, int NT = kBlock>
 __device__ __forceinline__ void loadReach(float (&rv)[ROWS], const char* base,
-                                          const TermNode* __restrict__ nodes, int g, int cnt,
-                                          int nO) {
+                                          const TermNode* nodes, int g, int cnt, int nO) {
   if (g >= cnt) return;
   const float* __restrict__ r = reinterpret_cast<const float*>(base + nodes[g].reach);
+  // unconditional loads (index clamped into the row; callers stage only
+  // j < nO): no register is written while a load into it is in flight
+#pragma unroll
+  for (int q = 0; q < ROWS; ++q) rv[q] = r[min((int)threadIdx.x + q * NT, nO - 1)];
+}
+
+// A group's node descriptors, copied into LDS once: reading them from the
+// arena inside the node loop would be vector loads ordered behind every
+// output store in flight (the arena is also written), so the next node's
+// reach request would wait for all of them.
+__device__ __forceinline__ const TermNode* stageNodes(TermNode* grp, const TermNode* nodes,
+                                                      int cnt) {
+  const int tid = threadIdx.x;
+  if (tid < 2 * cnt) {
+    const int4* src = reinterpret_cast<const int4*>(nodes);
+    reinterpret_cast<int4*>(grp)[tid] = src[tid];
+  }
+  __syncthreads();
+  return grp;
+}
+
+// A node's results, stored once the next node's reach is staged: its
+// stores are then older than the next reach request, so the wait for that
+// reach (which cannot count conditional stores and waits for all vector
+// memory operations) never waits for them.
+template <int ROWS, int NT>
+__device__ __forceinline__ void storeRows(float* out, const float (&res)[ROWS], int nT) {
 #pragma unroll
   for (int q = 0; q < ROWS; ++q) {
-    const int j = (int)threadIdx.x + q * NT;
-    rv[q] = j < nO ? r[j] : 0.0f;
+    const int i = (int)threadIdx.x + q * NT;
+    if (i < nT) out[i] = res[q];
   }
 }
 
@@ -267,10 +291,11 @@ __device__ __forceinline__ void loadReach(float (&rv)[ROWS], const char* base,
 template <int ROWS>
 __device__ void foldGroup(const Dev& d, const Item& it, char* lds) {
   __shared__ double cs[53];
+  __shared__ TermNode grp[kMaxGroup];
   const int tid = threadIdx.x, lane = tid & 63;
   const int nO = it.m, nT = it.n, cnt = it.cnt;
   float* raw = reinterpret_cast<float*>(lds);
-  const TermNode* __restrict__ nodes = at<TermNode>(d, it.a);
+  const TermNode* nodes = stageNodes(grp, at<TermNode>(d, it.a), cnt);
   const char* base = d.arena + it.b;
   uint32_t tc[ROWS];
 #pragma unroll
@@ -293,6 +318,8 @@ __device__ void foldGroup(const Dev& d, const Item& it, char* lds) {
 #pragma unroll
     for (int k = 0; k < kFoldPart; ++k) idx[k] = k < len ? list[lo + k] : (uint16_t)nO;
   }
+  float res[ROWS];
+  float* pend = nullptr;  // the previous node's output row
   auto node = [&](int g, float(&rv)[ROWS]) {
     const TermNode nd = nodes[g];
     if (g > 0) __syncthreads();  // the previous node's queries are done
@@ -302,6 +329,7 @@ __device__ void foldGroup(const Dev& d, const Item& it, char* lds) {
       if (j < nO) raw[j] = rv[q];
     }
     if (tid == 0) raw[nO] = 0.0f;
+    if (g > 0) storeRows<ROWS, kBlock>(pend, res, nT);
     __syncthreads();
     loadReach<ROWS>(rv, base, nodes, g + 1, cnt, nO);  // the next node's reach
     double part = 0.0;
@@ -318,62 +346,116 @@ __device__ void foldGroup(const Dev& d, const Item& it, char* lds) {
       if (lane == 0) cs[52] = 0.5 * x;
     }
     __syncthreads();
-    float* __restrict__ out = reinterpret_cast<float*>(d.arena + it.b + nd.out);
+    pend = reinterpret_cast<float*>(d.arena + it.b + nd.out);
     const double tot = cs[52];
     const double v = nd.v0;
 #pragma unroll
     for (int q = 0; q < ROWS; ++q) {
-      const int i = tid + q * kBlock;
-      if (i >= nT) continue;
       const int c1 = tc[q] & 0xff, c2 = (tc[q] >> 8) & 0xff;
       const int same = (int)(tc[q] >> 16) - 1;
       double m = tot - cs[c1] - cs[c2];
       if (same >= 0) m += (double)raw[same];
       if (m < 0.0) m = 0.0;
-      out[i] = (float)(m * v);
+      res[q] = (float)(m * v);
     }
   };
   float rv[ROWS];
   loadReach<ROWS>(rv, base, nodes, 0, cnt, nO);
   for (int g = 0; g < cnt; ++g) node(g, rv);
+  storeRows<ROWS, kBlock>(pend, res, nT);
 }
 
-// Showdown group. Per node: opponent reach is staged in LDS, then each lane
-// gathers its PER consecutive entries of the class's gather list (slots in
-// strength order, then the per-card segments in strength order) and one
-// blocked scan turns them into prefix sums X. Per traverser combo the
-// weaker / weaker-or-equal / all masses come from X at le, te, n and at the
-// compile-time cut points of its two cards' segments. The next node's reach
-// row is requested as soon as this one is staged.
-template <int NT, int PER, int ROWS>
+// Lane-permuted copy of a double within a wave through DPP moves (VALU, no
+// LDS traffic). CTRL is the DPP control: quad_perm (0x00-0xff), row_shr:n
+// (0x110 + n), row_bcast:15 (0x142), row_bcast:31 (0x143); lanes without a
+// source, or in rows outside ROW_MASK, read 0.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dppD(double x) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave (f64, fixed order):
+// within each row of 16 lanes by shifts of 1, 2, 4, 8, then row 0's total
+// into row 1 and row 2's into row 3, then lane 31's into rows 2 and 3.
+__device__ __forceinline__ double waveInclusiveSum(double x) {
+  x += dppD<0x111>(x);
+  x += dppD<0x112>(x);
+  x += dppD<0x114>(x);
+  x += dppD<0x118>(x);
+  x += dppD<0x142, 0xa>(x);
+  x += dppD<0x143, 0xc>(x);
+  return x;
+}
+
+// LDS of a showdown group: reach (+ sentinel) | strength-order prefixes
+// (f64, 4 per lane) | per-card segment prefixes (f32, one extra slot per
+// card holding its total).
+__host__ __device__ constexpr size_t sdRawBytes(int nO) { return (size_t)(((nO + 1) * 4 + 15) & ~15); }
+__host__ __device__ constexpr size_t sdLdsBytes(int nO, int lanes) {
+  return sdRawBytes(nO) + (size_t)lanes * 4 * sizeof(double) + (size_t)(2 * nO + 52) * 4;
+}
+
+// Showdown group. Per node: opponent reach is staged in LDS, then
+//  - the class's opponent slots in strength order (4 per lane) are gathered
+//    and scanned in f64 into the prefixes Xs (entry-major: position p at
+//    Xs[(p % 4) * NT + p / 4], so a wave's stores of one entry are
+//    consecutive);
+//  - each card's segment (its opponent slots in strength order) is gathered
+//    by 4 lanes, and the segment-relative prefixes are stored in f32 (Y,
+//    segment c at offset c past its position so each segment has a slot
+//    for its total); segments hold at most 48 slots, so their f32 prefixes
+//    keep about 2^-24 of a segment total, not of the whole range.
+// Per traverser combo the weaker / weaker-or-equal / all masses are
+//   win = Xs[le] - Y1[a1] - Y2[a2], leq = Xs[te] - Y1[b1] - Y2[b2] + same,
+//   all = total - t1 - t2 + same
+// with the compile-time cut points le, te (strength list) and a, b (card
+// segments). The next node's reach row is requested as soon as this one is
+// staged.
+template <int NT, int ROWS>
 __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
-  using Acc = SdAcc;
-  __shared__ Acc waveTot[NT / 64];
-  __shared__ Acc zt[52][2];  // per card: prefix at its segment start, segment total
-  __shared__ int so[53];
+  __shared__ double waveTot[NT / 64];
+  __shared__ float segTot[52];
+  __shared__ TermNode grp[kMaxGroup];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nO = it.m, nT = it.n, cnt = it.cnt;
   float* raw = reinterpret_cast<float*>(lds);
-  Acc* X = reinterpret_cast<Acc*>(lds + (((nO + 1) * 4 + 15) & ~15));
-  const TermNode* __restrict__ nodes = at<TermNode>(d, it.a);
+  double* Xs = reinterpret_cast<double*>(lds + sdRawBytes(nO));
+  float* Y = reinterpret_cast<float*>(lds + sdRawBytes(nO) + (size_t)NT * 4 * sizeof(double));
+  const TermNode* nodes = stageNodes(grp, at<TermNode>(d, it.a), cnt);
   const char* base = d.arena + it.b;
 
-  // class data for the whole group: 4 gather indices per 8-byte load
-  uint2 gi[PER / 4];
-  const uint2* gsrc = at<uint2>(d, it.c + kGatherHeader) + tid * (PER / 4);
+  // class data for the whole group: 4 strength-order slots, this lane's part
+  // of one card segment, the per-combo bounds
+  const uint2 gs = at<uint2>(d, it.c + kGatherHeader)[tid];
+  const int sc = tid >> 2, part = tid & 3;
+  int segLo = 0, segLen = 0;
+  if (sc < 52) {
+    const int32_t* so = at<int32_t>(d, it.c);
+    const int s0 = so[sc], sl = so[sc + 1] - s0;
+    segLo = s0 + (sl * part) / 4;
+    segLen = s0 + (sl * (part + 1)) / 4 - segLo;
+  }
+  uint16_t idx[kFoldPart];
+  {
+    const uint16_t* seg = at<uint16_t>(d, it.c + kGatherHeader + (int64_t)NT * 4 * 2);
 #pragma unroll
-  for (int k = 0; k < PER / 4; ++k) gi[k] = gsrc[k];
+    for (int k = 0; k < kFoldPart; ++k) idx[k] = k < segLen ? seg[segLo + k] : (uint16_t)nO;
+  }
   uint4 meta[ROWS];
 #pragma unroll
   for (int q = 0; q < ROWS; ++q) {
     const int i = tid + q * NT;
     meta[q] = i < nT ? at<uint4>(d, it.d)[i] : make_uint4(0, 0, 0, 0);
   }
-  if (tid < 53) so[tid] = at<int32_t>(d, it.c)[tid];
 
+  float res[ROWS];
+  float* pend = nullptr;  // the previous node's output row
   auto node = [&](int g, float(&rv)[ROWS]) {
     const TermNode nd = nodes[g];
-    if (g > 0) __syncthreads();  // the previous node's queries are done with raw / X
+    if (g > 0) __syncthreads();  // the previous node's queries are done with raw / Xs / Y
     // 1. stage reach (raw[nO] = 0 is the padding sentinel)
 #pragma unroll
     for (int q = 0; q < ROWS; ++q) {
@@ -381,79 +463,82 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
       if (j < nO) raw[j] = rv[q];
     }
     if (tid == 0) raw[nO] = 0.0f;
+    if (g > 0) storeRows<ROWS, NT>(pend, res, nT);
     __syncthreads();
     loadReach<ROWS, NT>(rv, base, nodes, g + 1, cnt, nO);  // the next node's reach
 
-    // 2. gather + blocked exclusive scan. X holds the exclusive prefix at
-    // every list position p (lane p / PER, entry p % PER), stored entry-major
-    // (X[(p % PER) * NT + p / PER]) so that a wave's writes of one entry are
-    // consecutive; the total sits at X[PER * NT].
-    float gv[PER];
-    Acc own = 0;
-#pragma unroll
-    for (int k = 0; k < PER / 4; ++k) {
-      const uint32_t w[2] = {gi[k].x, gi[k].y};
-#pragma unroll
-      for (int h = 0; h < 4; ++h) {
-        gv[k * 4 + h] = raw[(w[h >> 1] >> ((h & 1) * 16)) & 0xffffu];
-        own += (Acc)gv[k * 4 + h];
-      }
-    }
-    Acc inc = own;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const Acc t = __shfl_up(inc, off, 64);
-      if (lane >= off) inc += t;
-    }
+    // 2. gathers; strength-order scan across the block
+    float gv[4];
+    gv[0] = raw[gs.x & 0xffffu];
+    gv[1] = raw[gs.x >> 16];
+    gv[2] = raw[gs.y & 0xffffu];
+    gv[3] = raw[gs.y >> 16];
+    const double own = (((double)gv[0] + (double)gv[1]) + (double)gv[2]) + (double)gv[3];
+    const double inc = waveInclusiveSum(own);
     if (lane == 63) waveTot[wave] = inc;
+    float sv[kFoldPart];
+    double segSum = 0.0;
+#pragma unroll
+    for (int k = 0; k < kFoldPart; ++k) {
+      sv[k] = raw[idx[k]];
+      segSum += (double)sv[k];
+    }
     __syncthreads();
-    Acc run = inc - own;
+    double run = inc - own;
     for (int w = 0; w < wave; ++w) run += waveTot[w];
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      X[k * NT + tid] = run;
-      run += (Acc)gv[k];
+    for (int k = 0; k < 4; ++k) {
+      Xs[k * NT + tid] = run;
+      run += (double)gv[k];
     }
-    if (tid == NT - 1) X[PER * NT] = run;
-    __syncthreads();
-    auto xAt = [&](int p) -> Acc { return X[p == PER * NT ? p : (p % PER) * NT + p / PER]; };
-    if (tid < 52) {
-      const Acc z = xAt(nO + so[tid]);
-      zt[tid][0] = z;
-      zt[tid][1] = xAt(nO + so[tid + 1]) - z;
+    // 3. segment-relative prefixes: the quad's four parts, in order
+    const double p0 = dppD<0x00>(segSum), p1 = dppD<0x55>(segSum);
+    const double p2 = dppD<0xaa>(segSum), p3 = dppD<0xff>(segSum);
+    if (sc < 52) {
+      double y = part == 0 ? 0.0 : part == 1 ? p0 : part == 2 ? p0 + p1 : (p0 + p1) + p2;
+      float* yo = Y + segLo + sc;
+#pragma unroll
+      for (int k = 0; k < kFoldPart; ++k) {
+        if (k < segLen) yo[k] = (float)y;
+        y += (double)sv[k];
+      }
+      if (part == 3) {
+        const float t = (float)(((p0 + p1) + p2) + p3);
+        yo[segLen] = t;
+        segTot[sc] = t;
+      }
     }
     __syncthreads();
 
-    // 3. per traverser combo
-    float* __restrict__ out = reinterpret_cast<float*>(d.arena + it.b + nd.out);
-    const Acc vWin = nd.v0, vTie = nd.v1, vLose = nd.v2;
-    const Acc total = xAt(nO);
+    // 4. per traverser combo
+    auto xs = [&](int p) -> double { return Xs[(p & 3) * NT + (p >> 2)]; };
+    pend = reinterpret_cast<float*>(d.arena + it.b + nd.out);
+    const double vWin = nd.v0, vTie = nd.v1, vLose = nd.v2;
+    const double total = xs(nO);
 #pragma unroll
     for (int q = 0; q < ROWS; ++q) {
-      const int i = tid + q * NT;
-      if (i >= nT) continue;
       const uint4 m = meta[q];
       const int le = m.x & 0xffff, te = m.x >> 16;
-      const int a1 = m.y & 0xffff, b1 = m.y >> 16;
-      const int a2 = m.z & 0xffff, b2 = m.z >> 16;
       const int c1 = m.w & 0xff, c2 = (m.w >> 8) & 0xff;
+      const int a1 = (m.y & 0xffff) + c1, b1 = (m.y >> 16) + c1;
+      const int a2 = (m.z & 0xffff) + c2, b2 = (m.z >> 16) + c2;
       const int same = (int)(m.w >> 16) - 1;
-      const Acc z1 = zt[c1][0], t1 = zt[c1][1], z2 = zt[c2][0], t2 = zt[c2][1];
-      const Acc rs = same >= 0 ? (Acc)raw[same] : (Acc)0;
-      Acc win = xAt(le) - (xAt(nO + a1) - z1) - (xAt(nO + a2) - z2);
-      const Acc leq = xAt(te) - (xAt(nO + b1) - z1) - (xAt(nO + b2) - z2) + rs;
-      const Acc all = total - t1 - t2 + rs;
+      const double rs = same >= 0 ? (double)raw[same] : 0.0;
+      double win = xs(le) - (double)Y[a1] - (double)Y[a2];
+      const double leq = xs(te) - (double)Y[b1] - (double)Y[b2] + rs;
+      const double all = total - (double)segTot[c1] - (double)segTot[c2] + rs;
       if (win < 0) win = 0;
-      Acc tie = leq - win;
+      double tie = leq - win;
       if (tie < 0) tie = 0;
-      Acc lose = all - leq;
+      double lose = all - leq;
       if (lose < 0) lose = 0;
-      out[i] = (float)(win * vWin + tie * vTie + lose * vLose);
+      res[q] = (float)(win * vWin + tie * vTie + lose * vLose);
     }
   };
   float rv[ROWS];
   loadReach<ROWS, NT>(rv, base, nodes, 0, cnt, nO);
   for (int g = 0; g < cnt; ++g) node(g, rv);
+  storeRows<ROWS, NT>(pend, res, nT);
 }
 
 // Terminal launches of a half-step (after the forward pass has produced all
@@ -469,26 +554,24 @@ __global__ __launch_bounds__(kBlock) void foldKernel(Dev d, const Item* __restri
   else foldGroup<kRowRegs>(d, it, lds);
 }
 
-// Showdown groups of classes with at most 768 combos per side and 3072
-// gather entries: 256 lanes, 3 combos and 8 or 12 gather entries each.
+// Showdown groups of classes with at most 768 combos per side: 256 lanes,
+// 3 combos and 4 strength-order slots each.
 __global__ __launch_bounds__(kBlock) void showdownKernel(Dev d, const Item* __restrict__ items,
                                                          int masked) {
   extern __shared__ __align__(16) char lds[];
   const Item it = items[blockIdx.x];
   if (masked && !d.ctl[it.f].alive) return;
-  if (it.e == 8) showdownGroup<kBlock, 8, 3>(d, it, lds);
-  else showdownGroup<kBlock, 12, 3>(d, it, lds);
+  showdownGroup<kBlock, 3>(d, it, lds);
 }
 
-// The larger classes (up to 1326 combos per side): 512 lanes, 3 combos and
-// 8 gather entries each.
+// The larger classes (up to 1326 combos per side): 512 lanes.
 __global__ __launch_bounds__(kWideBlock) void showdownWideKernel(Dev d,
                                                                  const Item* __restrict__ items,
                                                                  int masked) {
   extern __shared__ __align__(16) char lds[];
   const Item it = items[blockIdx.x];
   if (masked && !d.ctl[it.f].alive) return;
-  showdownGroup<kWideBlock, 8, 3>(d, it, lds);
+  showdownGroup<kWideBlock, 3>(d, it, lds);
 }
 
 // Forward launch of one depth: opponent reach of the next depth.
@@ -833,20 +916,14 @@ struct Layout {
   }
 };
 
-// Gather entries per lane for a list of `len` entries: 8, 12 or 16 (two to
-// four 8-byte index loads).
-int perLane(int len) { return len <= kBlock * 8 ? 8 : len <= kBlock * 12 ? 12 : 16; }
-
 // Showdown group shape: the 256-lane kernel takes classes with at most
-// 3 * kBlock combos per side and 12 gather entries per lane; the rest go
-// to the 512-lane kernel (3 combos, 8 gather entries per lane).
-bool wideShowdown(int nT, int nO) { return std::max(nT, nO) > 3 * kBlock || 3 * nO > 12 * kBlock; }
+// 3 * kBlock combos per side; the rest go to the 512-lane kernel.
+bool wideShowdown(int nT, int nO) { return std::max(nT, nO) > 3 * kBlock; }
 
 // Dynamic LDS of a terminal item: reach (+ sentinel), and for showdown the
-// f64 prefixes.
+// prefixes.
 size_t terminalLdsOf(const Item& it, int lanes) {
-  const size_t reach = (size_t)(((it.m + 1) * 4 + 15) & ~15);
-  return it.kind == kBwdShowdown ? reach + (size_t)(lanes * it.e + 1) * sizeof(SdAcc) : reach;
+  return it.kind == kBwdShowdown ? sdLdsBytes(it.m, lanes) : sdRawBytes(it.m);
 }
 
 // Host-side tables of one spot, built from its description.
@@ -855,28 +932,25 @@ struct SpotTables {
   std::map<int, int64_t> bmapOff[2];  // class -> element offset
   std::vector<char> gather[2];  // per class and terminal kind, see gatherBlock
   std::map<int, int64_t> sdOff[2], foldOff[2];
-  std::map<int, int> sdPer[2];
   std::vector<uint32_t> cardSame[2];  // per list entry: c1 | c2 << 8 | (same+1) << 16
   std::vector<TermNode> termNodes;    // grouped: each group's nodes are contiguous
 
-  // showdown: [53 i32 segment bounds | pad to 256 B | u16 list: slots in
-  // strength order, then the per-card segments, padded to kBlock * per with
-  // the sentinel slot n]; fold: [53 i32 segment bounds | pad | u16 per-card
-  // segments]
+  // showdown: [53 i32 segment bounds | pad to 256 B | u16 slots in strength
+  // order, padded with the sentinel slot n to 4 per lane of the class's
+  // kernel | u16 per-card segments]; fold: [53 i32 segment bounds | pad |
+  // u16 per-card segments]
   void gatherBlock(const ppsg_tree_desc* t, int p, int cl, bool sd) {
     auto& offs = sd ? sdOff[p] : foldOff[p];
     if (offs.count(cl)) return;
     const int n = t->cls_n[p][cl];
     const int64_t lo = t->cls_list_off[p][cl];
     std::vector<uint16_t> list;
-    if (sd) list.assign(t->sorted[p] + lo, t->sorted[p] + lo + n);
-    list.insert(list.end(), t->seg_slot[p] + 2 * lo, t->seg_slot[p] + 2 * lo + 2 * n);
     if (sd) {
-      // padded for either showdown shape (kBlock * 8 or 12, kWideBlock * 8)
-      const int per = perLane((int)list.size());
-      list.resize(std::max<size_t>((size_t)kBlock * per, (size_t)kWideBlock * 8), (uint16_t)n);
-      sdPer[p][cl] = per;
+      const bool wide = wideShowdown(t->cls_n[0][cl], t->cls_n[1][cl]);
+      list.assign(t->sorted[p] + lo, t->sorted[p] + lo + n);
+      list.resize((size_t)4 * (wide ? kWideBlock : kBlock), (uint16_t)n);
     }
+    list.insert(list.end(), t->seg_slot[p] + 2 * lo, t->seg_slot[p] + 2 * lo + 2 * n);
     std::vector<char>& g = gather[p];
     g.resize((g.size() + 15) / 16 * 16, 0);
     offs[cl] = (int64_t)g.size();
@@ -895,7 +969,7 @@ struct TermGroup {
   uint8_t kind = 0;
   uint8_t rows = 0;
   bool wide = false;  // showdown: the 512-lane kernel
-  int per = 8, nT = 0, nO = 0, count = 0;
+  int nT = 0, nO = 0, count = 0;
   int64_t first = 0;          // index of its first TermNode
   int64_t gather = 0, meta = 0;  // chunk-relative
 };
@@ -951,11 +1025,10 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
     } else if (t->kind[u] == PPSG_SHOWDOWN || t->kind[u] == PPSG_FOLD) {
       const bool sd = t->kind[u] == PPSG_SHOWDOWN;
       for (int p = 0; p < 2; ++p) {
-        if (!sd) {  // fold: every card's segment fits 4 lanes x kFoldPart entries
-          const int32_t* so = t->seg_off[p] + (int64_t)cl * 53;
-          for (int c = 0; c < 52; ++c)
-            if (so[c + 1] - so[c] > 4 * kFoldPart) return fail("fold: card segment too long");
-        }
+        // every card's segment fits 4 lanes x kFoldPart entries
+        const int32_t* so = t->seg_off[p] + (int64_t)cl * 53;
+        for (int c = 0; c < 52; ++c)
+          if (so[c + 1] - so[c] > 4 * kFoldPart) return fail("terminal: card segment too long");
         T.gatherBlock(t, p, cl, sd);
       }
     }
@@ -1085,7 +1158,6 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
           g.meta = sd ? L.sdMeta[tr] + 16 * t->cls_list_off[tr][cl]
                       : L.cardSame[tr] + 4 * t->cls_list_off[tr][cl];
           g.wide = sd && wideShowdown(nT, nO);
-          g.per = sd ? (g.wide ? 8 : T.sdPer[op][cl]) : 0;
           g.rows = (uint8_t)(sd || std::max(nT, nO) <= 3 * kBlock ? 3 : kRowRegs);
           g.nT = nT;
           g.nO = nO;
@@ -1323,7 +1395,6 @@ std::vector<Item> ppsg_engine::spotItems(int slot, int tr, int ps, int dep) cons
       it.d = S.off + g.meta;
       it.n = g.nT;
       it.m = g.nO;
-      it.e = g.per;
       it.na = g.rows;
       it.cnt = std::min(groupSize, g.count - s0);
       it.f = slot;
@@ -2175,7 +2246,7 @@ int ppsg_kind_bytes(ppsg_t* h, double out[6]) {
 int ppsg_tune(ppsg_t* h, int param, int value) {
   if (!h) return fail("null handle");
   if (param != PPSG_TUNE_GROUP) return fail("tune: unknown parameter");
-  if (value < 1 || value > 1024) return fail("tune: group size must be 1..1024");
+  if (value < 1 || value > kMaxGroup) return fail("tune: group size must be 1..64");
   if (value == h->groupSize) return 0;
   h->groupSize = value;
   return h->rebuild();

@@ -184,7 +184,7 @@ int ppsg_batch_compact(ppsg_t* h, double max_dead_fraction, int* compacted);
 int ppsg_kind_bytes(ppsg_t* h, double out[6]);
 /* Launch shape (no reference counterpart; results do not depend on it).
    PPSG_TUNE_GROUP: fold/showdown nodes of one board class evaluated by one
-   workgroup (1..1024; default 4). Larger groups load a class's gather list
+   workgroup (1..64; default 4). Larger groups load a class's gather list
    and per-combo metadata once for more nodes and launch fewer workgroups;
    smaller groups spread a small batch over more CUs. */
 int ppsg_tune(ppsg_t* h, int param, int value);
