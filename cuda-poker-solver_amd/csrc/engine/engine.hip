@@ -929,9 +929,9 @@ size_t terminalLdsOf(const Item& it, int lanes) {
 // Host-side tables of one spot, built from its description.
 struct SpotTables {
   std::vector<int16_t> bmap[2];       // per parent class: [branch][parent slot]
-  std::map<int, int64_t> bmapOff[2];  // class -> element offset
+  std::vector<int64_t> bmapOff[2];  // class -> element offset, -1: none
   std::vector<char> gather[2];  // per class and terminal kind, see gatherBlock
-  std::map<int, int64_t> sdOff[2], foldOff[2];
+  std::vector<int64_t> sdOff[2], foldOff[2];  // class -> byte offset, -1: none
   std::vector<uint32_t> cardSame[2];  // per list entry: c1 | c2 << 8 | (same+1) << 16
   std::vector<TermNode> termNodes;    // grouped: each group's nodes are contiguous
 
@@ -941,7 +941,7 @@ struct SpotTables {
   // u16 per-card segments]
   void gatherBlock(const ppsg_tree_desc* t, int p, int cl, bool sd) {
     auto& offs = sd ? sdOff[p] : foldOff[p];
-    if (offs.count(cl)) return;
+    if (offs[cl] >= 0) return;
     const int n = t->cls_n[p][cl];
     const int64_t lo = t->cls_list_off[p][cl];
     std::vector<uint16_t> list;
@@ -1006,6 +1006,23 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
     b = t->child_base[u];
     e = u + 1 < N ? t->child_base[u + 1] : (int)t->n_child;
   };
+  for (int p = 0; p < 2; ++p) {
+    T.bmapOff[p].assign(nc, -1);
+    T.sdOff[p].assign(nc, -1);
+    T.foldOff[p].assign(nc, -1);
+  }
+  // every card segment of a class with terminals fits 4 lanes x kFoldPart
+  std::vector<uint8_t> termCls(nc, 0);
+  for (int64_t u = 0; u < N; ++u)
+    if (t->kind[u] == PPSG_SHOWDOWN || t->kind[u] == PPSG_FOLD) termCls[t->cls[u]] = 1;
+  for (int c = 0; c < nc; ++c) {
+    if (!termCls[c]) continue;
+    for (int p = 0; p < 2; ++p) {
+      const int32_t* so = t->seg_off[p] + (int64_t)c * 53;
+      for (int k = 0; k < 52; ++k)
+        if (so[k + 1] - so[k] > 4 * kFoldPart) return fail("terminal: card segment too long");
+    }
+  }
   for (int64_t u = 0; u < N; ++u) {
     const int cl = t->cls[u];
     if (t->kind[u] == PPSG_CHANCE) {
@@ -1013,7 +1030,7 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
       kidsOf(u, b, e);
       if (e - b > 255) return fail("more than 255 chance branches");
       for (int p = 0; p < 2; ++p) {
-        if (T.bmapOff[p].count(cl)) continue;
+        if (T.bmapOff[p][cl] >= 0) continue;
         T.bmapOff[p][cl] = (int64_t)T.bmap[p].size();
         const int np = t->cls_n[p][cl];
         for (int k = b; k < e; ++k) {
@@ -1024,13 +1041,7 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
       }
     } else if (t->kind[u] == PPSG_SHOWDOWN || t->kind[u] == PPSG_FOLD) {
       const bool sd = t->kind[u] == PPSG_SHOWDOWN;
-      for (int p = 0; p < 2; ++p) {
-        // every card's segment fits 4 lanes x kFoldPart entries
-        const int32_t* so = t->seg_off[p] + (int64_t)cl * 53;
-        for (int c = 0; c < 52; ++c)
-          if (so[c + 1] - so[c] > 4 * kFoldPart) return fail("terminal: card segment too long");
-        T.gatherBlock(t, p, cl, sd);
-      }
+      for (int p = 0; p < 2; ++p) T.gatherBlock(t, p, cl, sd);
     }
   }
   for (int p = 0; p < 2; ++p) {
@@ -1065,7 +1076,7 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
   for (int tr = 0; tr < 2; ++tr)
     for (int ps = 0; ps < 2; ++ps) S.items[tr][ps].assign(D, {});
   auto tiles = [](int n, int r) { return (n + kTile * r - 1) / (kTile * r); };
-  std::map<int, int> groupOf[2];  // (class, kind) -> group index + 1
+  std::vector<int> groupOf[2] = {std::vector<int>((size_t)2 * nc, 0), std::vector<int>((size_t)2 * nc, 0)};  // (class, kind) -> group index + 1
   std::vector<std::vector<TermNode>> members[2];
   for (int64_t u = 0; u < N; ++u) {
     const int dep = t->depth[u], cl = t->cls[u], k = t->kind[u];

@@ -2,6 +2,8 @@
 #include "compile.h"
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <cmath>
 #include <map>
 #include <stdexcept>
@@ -21,7 +23,6 @@ struct Build {
   const pf::BetConfig& cfg;
   CompiledTree& T;
   std::map<std::vector<uint8_t>, int> classIds;
-  std::vector<std::vector<int32_t>> kids;
 
   Build(const PostflopSpot& s, const pf::BetConfig& c, CompiledTree& t) : spot(s), cfg(c), T(t) {}
 
@@ -71,9 +72,13 @@ struct Build {
     return id;
   }
 
-  int newNode(int depth, int cl, int64_t potBase, int64_t sc0, int64_t sc1, int64_t pc0,
-              int64_t pc1) {
+  // Nodes are created in pre-order and siblings in order, so a node's
+  // children are exactly the nodes naming it as parent, in id order: the
+  // children table is rebuilt from the parent links afterwards.
+  int newNode(int parent, int depth, int cl, int64_t potBase, int64_t sc0, int64_t sc1,
+              int64_t pc0, int64_t pc1) {
     const int id = (int)T.kind.size();
+    T.parent.push_back(parent);
     T.kind.push_back(PPSG_DECIDE);
     T.owner.push_back(0);
     T.na.push_back(0);
@@ -85,16 +90,15 @@ struct Build {
     T.pc0.push_back(pc0);
     T.pc1.push_back(pc1);
     T.actBase.push_back(-1);
-    kids.emplace_back();
     T.maxDepth = std::max(T.maxDepth, depth);
     return id;
   }
 
   // Tree rules of cuda/gpu_cfr.cu:1116-1326.
-  int node(const std::vector<uint8_t>& board, int cl, int64_t potBase, int64_t sc0, int64_t sc1,
-           int actor, bool afterAllin, bool closed, int depth, int64_t pc0, int64_t pc1,
-           int numBets) {
-    const int id = newNode(depth, cl, potBase, sc0, sc1, pc0, pc1);
+  int node(int parent, const std::vector<uint8_t>& board, int cl, int64_t potBase, int64_t sc0,
+           int64_t sc1, int actor, bool afterAllin, bool closed, int depth, int64_t pc0,
+           int64_t pc1, int numBets) {
+    const int id = newNode(parent, depth, cl, potBase, sc0, sc1, pc0, pc1);
     if (!closed && spot.stack - pc0 <= 0) closed = true;  // nothing behind: dead street
     if (closed) {
       if (board.size() == 5) {
@@ -104,16 +108,16 @@ struct Build {
       T.kind[id] = PPSG_CHANCE;
       std::vector<uint8_t> next = board;
       next.push_back(0);
-      std::vector<int32_t> out;
+      int branches = 0;
       for (int c = 0; c < 52; ++c) {
         if (std::find(board.begin(), board.end(), (uint8_t)c) != board.end()) continue;
         next.back() = (uint8_t)c;
         const int ccl = classOf(next, cl, c);
-        out.push_back(node(next, ccl, potBase + sc0 + sc1, 0, 0, 0, false, false, depth + 1,
-                           pc0 + sc0, pc1 + sc1, 0));
+        node(id, next, ccl, potBase + sc0 + sc1, 0, 0, 0, false, false, depth + 1, pc0 + sc0,
+             pc1 + sc1, 0);
+        ++branches;
       }
-      T.na[id] = (uint8_t)out.size();  // chance: branch count
-      kids[id] = std::move(out);
+      T.na[id] = (uint8_t)branches;  // chance: branch count
       return id;
     }
     const std::vector<pf::TreeAction> acts = pf::betActions(
@@ -126,38 +130,65 @@ struct Build {
     T.na[id] = (uint8_t)acts.size();
     T.actBase[id] = (int32_t)T.actFlat.size();
     T.actFlat.insert(T.actFlat.end(), acts.begin(), acts.end());
-    std::vector<int32_t> out;
     for (const pf::TreeAction& a : acts) {
       int64_t n0 = sc0, n1 = sc1;
-      int child;
       switch (a.kind) {
         case pf::ActionKind::Check:
-          child = actor == 0 ? node(board, cl, potBase, sc0, sc1, 1, afterAllin, false, depth + 1,
-                                    pc0, pc1, numBets)
-                             : node(board, cl, potBase, sc0, sc1, 0, false, true, depth + 1, pc0,
-                                    pc1, 0);
+          if (actor == 0)
+            node(id, board, cl, potBase, sc0, sc1, 1, afterAllin, false, depth + 1, pc0, pc1,
+                 numBets);
+          else
+            node(id, board, cl, potBase, sc0, sc1, 0, false, true, depth + 1, pc0, pc1, 0);
           break;
-        case pf::ActionKind::Fold:
-          child = newNode(depth + 1, cl, potBase, sc0, sc1, pc0, pc1);
+        case pf::ActionKind::Fold: {
+          const int child = newNode(id, depth + 1, cl, potBase, sc0, sc1, pc0, pc1);
           T.kind[child] = PPSG_FOLD;
           T.owner[child] = (int8_t)actor;
           break;
+        }
         case pf::ActionKind::Call:
           (actor == 0 ? n0 : n1) = std::max(sc0, sc1);
-          child = node(board, cl, potBase, n0, n1, 0, false, true, depth + 1, pc0, pc1, 0);
+          node(id, board, cl, potBase, n0, n1, 0, false, true, depth + 1, pc0, pc1, 0);
           break;
         default:
           (actor == 0 ? n0 : n1) = a.amount;
-          child = node(board, cl, potBase, n0, n1, actor ^ 1,
-                       afterAllin || a.kind == pf::ActionKind::AllIn, false, depth + 1, pc0, pc1,
-                       numBets + 1);
+          node(id, board, cl, potBase, n0, n1, actor ^ 1,
+               afterAllin || a.kind == pf::ActionKind::AllIn, false, depth + 1, pc0, pc1,
+               numBets + 1);
       }
-      out.push_back(child);
     }
-    kids[id] = std::move(out);
     return id;
   }
 };
+
+// Runs fn(i) for i in [0, n) on up to `threads` host threads (items are
+// independent and write disjoint data, so the result does not depend on the
+// thread count).
+template <typename F>
+void parallelFor(int n, int threads, F&& fn) {
+  threads = std::max(1, std::min(threads, n));
+  if (threads == 1) {
+    for (int i = 0; i < n; ++i) fn(i);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t)
+    pool.emplace_back([&] {
+      for (int i = next++; i < n; i = next++) fn(i);
+    });
+  for (std::thread& th : pool) th.join();
+}
+
+// Host threads for the per-class tables of a tree with nc board classes:
+// flop trees (thousands of river classes) use the machine; turn and river
+// trees (dozens) stay on the calling thread, since callers that compile many
+// spots already run one compile per thread.
+int classThreads(int nc) {
+  if (nc < 256) return 1;
+  const unsigned hw = std::thread::hardware_concurrency();
+  return (int)std::min(16u, hw ? hw : 1u);
+}
 
 // Per-class tables: card codes, identical-combo slots, per-card segments,
 // and on 5-card boards the strength order plus per-traverser prefix bounds.
@@ -175,9 +206,9 @@ void classTables(CompiledTree& T) {
       T.listCards[p][e] = (uint16_t)(T.cards[p][2 * b] | (T.cards[p][2 * b + 1] << 8));
     }
   }
-  std::vector<uint64_t> key[2];
-  std::vector<int> pos[2];
-  for (int c = 0; c < nc; ++c) {
+  parallelFor(nc, classThreads(nc), [&](int c) {
+    std::vector<uint64_t> key[2];
+    std::vector<int> pos[2];
     const bool river = T.clsNBoard[c] == 5;
     for (int p = 0; p < 2; ++p) {
       const int q = 1 - p;
@@ -231,7 +262,7 @@ void classTables(CompiledTree& T) {
         seg[fill[cc >> 8]++] = (uint16_t)s;
       }
     }
-    if (!river) continue;
+    if (!river) return;
     // traverser tr's combos against the opponent's strength order
     for (int tr = 0; tr < 2; ++tr) {
       const int op = 1 - tr;
@@ -263,7 +294,7 @@ void classTables(CompiledTree& T) {
         m[3] = (uint32_t)cc | ((uint32_t)(T.listSame[tr][off + s] + 1) << 16);
       }
     }
-  }
+  });
 }
 
 }  // namespace
@@ -332,20 +363,23 @@ CompiledTree compileSpot(const PostflopSpot& spot, const pf::BetConfig& cfg) {
   Build B(spot, cfg, T);
   std::vector<uint8_t> board(spot.board, spot.board + spot.nBoard);
   const int root = B.classOf(board, -1, -1);
-  B.node(board, root, spot.pot, 0, 0, 0, false, false, 0, 0, 0, 0);
+  B.node(-1, board, root, spot.pot, 0, 0, 0, false, false, 0, 0, 0, 0);
   T.numNodes = (int64_t)T.kind.size();
 
-  // children table + parents
-  T.parent.assign(T.numNodes, -1);
-  T.childBase.resize(T.numNodes);
+  // children table from the parent links (children in id order)
+  T.childBase.assign(T.numNodes, 0);
+  for (int64_t u = 1; u < T.numNodes; ++u) ++T.childBase[T.parent[u]];
+  int32_t acc = 0;
   for (int64_t u = 0; u < T.numNodes; ++u) {
-    T.childBase[u] = (int32_t)T.childFlat.size();
-    for (int32_t c : B.kids[u]) {
-      T.childFlat.push_back(c);
-      T.parent[c] = (int32_t)u;
-    }
+    const int32_t c = T.childBase[u];
+    T.childBase[u] = acc;
+    acc += c;
   }
-  B.kids.clear();
+  T.childFlat.assign(acc, 0);
+  {
+    std::vector<int32_t> fill(T.childBase);
+    for (int64_t u = 1; u < T.numNodes; ++u) T.childFlat[fill[T.parent[u]]++] = (int32_t)u;
+  }
   classTables(T);
 
   // rows (64-bit offsets) and terminal values. Sibling rows are contiguous:
