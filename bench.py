@@ -105,6 +105,43 @@ def cpu_baseline_single(spot, seconds):
                       f"(C++ CPU oracle, OpenMP over river deals), {dt:.1f} s"}
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_table(configs, seconds):
+    """The CPU oracle on the plain turn (C1) and large turn (C2) spots at
+    1 thread and at all cores (up to 16): iterations are timed until about
+    `seconds` per cell have passed (each cell runs whole iterations; the
+    first iteration of a fresh solve is part of the sample)."""
+    rows = []
+    for spot in (configs.plain_turn(), configs.large_turn()):
+        for threads in (1, cpu_threads()):
+            o = oracle_of(spot, threads)
+            done, t0 = 0, time.perf_counter()
+            while True:
+                if done == 0:
+                    o.solve(1, spot["algo"])
+                else:
+                    o.continue_solve(1)
+                done += 1
+                if time.perf_counter() - t0 >= seconds:
+                    break
+            dt = time.perf_counter() - t0
+            rows.append({"workload": spot["name"], "threads": threads, "iterations": done,
+                         "seconds": round(dt, 2), "iters_per_s": round(done / dt, 3)})
+            log(f"bench: cpu {json.dumps(rows[-1])}")
+    return {"nproc": os.cpu_count(), "model": cpu_model(), "kind": "port",
+            "solver": "C++ CPU oracle (oracle/), OpenMP over river deals", "rows": rows}
+
+
 def roofline_batch(prof):
     """Per-kernel achieved bandwidth of the resident batch (algorithmic f32 row
     bytes per launch / mean launch time, HIP events on the engine stream);
@@ -413,7 +450,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy kernel used to scale byte counters")
+    ap.add_argument("--cpu-table", action="store_true",
+                    help="only the CPU oracle at 1 thread and all cores on the plain and "
+                         "large turn spots (no GPU)")
     args = ap.parse_args()
+    if args.cpu_table:
+        from pps import configs
+        print(json.dumps({"cpu_table": cpu_table(configs, args.cpu_seconds)}), flush=True)
+        return 0
     if args.copy_calibration:
         from pps import capi
         nbytes = 256 << 20
