@@ -797,50 +797,77 @@ __device__ __forceinline__ int blockExclusiveScan(int v, int* total) {
   return base + inc - v;
 }
 
-__global__ __launch_bounds__(kBlock) void compactCount(const Item* __restrict__ in, int64_t n,
-                                                       const Ctl* __restrict__ ctl,
+// One list of a compaction pass: its items go from `in` to `out` (the
+// list's spare buffer); its blocks are [block0, block0 + nb) of the grid.
+struct CompactJob {
+  const Item* in;
+  Item* out;
+  int64_t n;
+  int64_t block0;
+  int64_t nb;
+};
+
+// The job owning grid block b (jobs sorted by block0).
+__device__ __forceinline__ int jobOf(const CompactJob* __restrict__ jobs, int nJobs, int64_t b) {
+  int lo = 0, hi = nJobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (jobs[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void compactCount(const CompactJob* __restrict__ jobs,
+                                                       int nJobs, const Ctl* __restrict__ ctl,
                                                        int* __restrict__ counts) {
-  const int64_t i0 = (int64_t)blockIdx.x * kCompactItems + 4 * (int64_t)threadIdx.x;
+  const CompactJob J = jobs[jobOf(jobs, nJobs, blockIdx.x)];
+  const int64_t i0 = ((int64_t)blockIdx.x - J.block0) * kCompactItems + 4 * (int64_t)threadIdx.x;
   int c = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k)
-    if (i0 + k < n && ctl[in[i0 + k].f].alive) ++c;
+    if (i0 + k < J.n && ctl[J.in[i0 + k].f].alive) ++c;
   int total = 0;
   (void)blockExclusiveScan(c, &total);
   if (threadIdx.x == 0) counts[blockIdx.x] = total;
 }
 
-// One block: counts -> exclusive offsets (in place); *total = sum.
-__global__ __launch_bounds__(kBlock) void compactScan(int* __restrict__ counts, int nb,
-                                                      int* __restrict__ total) {
+// One block per job: its block counts -> exclusive offsets (in place);
+// totals[job] = its kept items.
+__global__ __launch_bounds__(kBlock) void compactScan(const CompactJob* __restrict__ jobs,
+                                                      int* __restrict__ counts,
+                                                      int* __restrict__ totals) {
+  const CompactJob J = jobs[blockIdx.x];
+  int* c = counts + J.block0;
+  const int nb = (int)J.nb;
   int carry = 0;
   for (int b0 = 0; b0 < nb; b0 += kBlock) {
     const int i = b0 + (int)threadIdx.x;
-    const int v = i < nb ? counts[i] : 0;
+    const int v = i < nb ? c[i] : 0;
     int sum = 0;
     const int ex = blockExclusiveScan(v, &sum);
-    if (i < nb) counts[i] = carry + ex;
+    if (i < nb) c[i] = carry + ex;
     carry += sum;
   }
-  if (threadIdx.x == 0) *total = carry;
+  if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
-__global__ __launch_bounds__(kBlock) void compactScatter(const Item* __restrict__ in, int64_t n,
-                                                         const Ctl* __restrict__ ctl,
-                                                         const int* __restrict__ offs,
-                                                         Item* __restrict__ out) {
-  const int64_t i0 = (int64_t)blockIdx.x * kCompactItems + 4 * (int64_t)threadIdx.x;
+__global__ __launch_bounds__(kBlock) void compactScatter(const CompactJob* __restrict__ jobs,
+                                                         int nJobs, const Ctl* __restrict__ ctl,
+                                                         const int* __restrict__ offs) {
+  const CompactJob J = jobs[jobOf(jobs, nJobs, blockIdx.x)];
+  const int64_t i0 = ((int64_t)blockIdx.x - J.block0) * kCompactItems + 4 * (int64_t)threadIdx.x;
   bool keep[4];
   int c = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    keep[k] = i0 + k < n && ctl[in[i0 + k].f].alive;
+    keep[k] = i0 + k < J.n && ctl[J.in[i0 + k].f].alive;
     c += keep[k];
   }
   int pos = offs[blockIdx.x] + blockExclusiveScan(c, nullptr);
 #pragma unroll
   for (int k = 0; k < 4; ++k)
-    if (keep[k]) out[pos++] = in[i0 + k];
+    if (keep[k]) J.out[pos++] = J.in[i0 + k];
 }
 
 // Plain streaming copy (16 B per lane per step): the calibration kernel for
@@ -1235,6 +1262,8 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
 struct DevList {
   Item* buf = nullptr;
   int64_t count = 0, cap = 0;
+  Item* spare = nullptr;  // compaction target, swapped with buf afterwards
+  int64_t spareCap = 0;
   size_t lds = 0;  // forward lists: dynamic LDS of the largest terminal item
 };
 
@@ -1260,11 +1289,11 @@ struct ppsg_engine {
   std::vector<DevList> lists[2][kLists];
   int groupSize = 4;  // terminal nodes per work item
   // device compaction scratch (grow-only)
-  Item* cscratch = nullptr;
-  int64_t cscratchCap = 0;
+  CompactJob* cjobs = nullptr;
+  int64_t cjobsCap = 0;
   int* ccounts = nullptr;
   int64_t ccountsCap = 0;
-  int* ctotal = nullptr;
+  int* ctotals = nullptr;
   int64_t listedItems = 0, deadItems = 0;
   std::vector<int> live;
   int* liveDev = nullptr;
@@ -1296,7 +1325,6 @@ struct ppsg_engine {
   int64_t countItems(int slot) const;
   int appendSpot(int slot);
   int compact();
-  int compactList(DevList& l);
   int rebuild();
   int prepare();
   void dropGraph();
@@ -1323,15 +1351,17 @@ ppsg_engine::~ppsg_engine() {
   dropGraph();
   for (int tr = 0; tr < 2; ++tr)
     for (int ps = 0; ps < kLists; ++ps)
-      for (DevList& l : lists[tr][ps])
+      for (DevList& l : lists[tr][ps]) {
         if (l.buf) (void)hipFree(l.buf);
+        if (l.spare) (void)hipFree(l.spare);
+      }
   for (Spot& S : spots)
     if (S.coefDev) (void)hipFree(S.coefDev);
   if (liveDev) (void)hipFree(liveDev);
   if (flagsDev) (void)hipFree(flagsDev);
-  if (cscratch) (void)hipFree(cscratch);
+  if (cjobs) (void)hipFree(cjobs);
   if (ccounts) (void)hipFree(ccounts);
-  if (ctotal) (void)hipFree(ctotal);
+  if (ctotals) (void)hipFree(ctotals);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (stream) (void)hipStreamDestroy(stream);
@@ -1475,54 +1505,72 @@ int ppsg_engine::appendSpot(int slot) {
   return 0;
 }
 
-// Drops the items of retired and parked spots from one list, on the device.
-int ppsg_engine::compactList(DevList& l) {
-  if (l.count == 0) return 0;
-  const int64_t nb = (l.count + kCompactItems - 1) / kCompactItems;
-  if (l.count > cscratchCap) {
-    if (cscratch) (void)hipFree(cscratch);
-    cscratch = nullptr;
-    void* p = nullptr;
-    HIP_TRY(hipMalloc(&p, (size_t)l.count * sizeof(Item)));
-    cscratch = static_cast<Item*>(p);
-    cscratchCap = l.count;
-  }
-  if (nb > ccountsCap) {
-    if (ccounts) (void)hipFree(ccounts);
-    ccounts = nullptr;
-    void* p = nullptr;
-    HIP_TRY(hipMalloc(&p, (size_t)nb * sizeof(int)));
-    ccounts = static_cast<int*>(p);
-    ccountsCap = nb;
-  }
-  if (!ctotal) {
-    void* p = nullptr;
-    HIP_TRY(hipMalloc(&p, sizeof(int)));
-    ctotal = static_cast<int*>(p);
-  }
-  compactCount<<<(unsigned)nb, kBlock, 0, stream>>>(l.buf, l.count, ctl, ccounts);
-  compactScan<<<1, kBlock, 0, stream>>>(ccounts, (int)nb, ctotal);
-  compactScatter<<<(unsigned)nb, kBlock, 0, stream>>>(l.buf, l.count, ctl, ccounts, cscratch);
-  HIP_TRY(hipGetLastError());
-  int kept = 0;
-  HIP_TRY(hipMemcpyAsync(&kept, ctotal, sizeof(int), hipMemcpyDeviceToHost, stream));
-  HIP_TRY(hipStreamSynchronize(stream));
-  if (kept > 0)
-    HIP_TRY(hipMemcpyAsync(l.buf, cscratch, (size_t)kept * sizeof(Item), hipMemcpyDeviceToDevice,
-                           stream));
-  HIP_TRY(hipStreamSynchronize(stream));
-  l.count = kept;
-  return 0;
-}
-
 // Drops retired and parked spots' items from every list (on the device,
-// order kept) and frees removed spots.
+// order kept: three launches for all lists, each list compacted into its
+// spare buffer, which then becomes the list) and frees removed spots.
 int ppsg_engine::compact() {
   dropGraph();
+  std::vector<CompactJob> jobs;
+  std::vector<DevList*> owners;
+  int64_t blocks = 0;
   for (int tr = 0; tr < 2; ++tr)
     for (int ps = 0; ps < kLists; ++ps)
-      for (DevList& l : lists[tr][ps])
-        if (compactList(l) != 0) return -1;
+      for (DevList& l : lists[tr][ps]) {
+        if (l.count == 0) continue;
+        if (l.spareCap < l.count) {
+          if (l.spare) (void)hipFree(l.spare);
+          l.spare = nullptr;
+          l.spareCap = 0;
+          void* p = nullptr;
+          HIP_TRY(hipMalloc(&p, (size_t)l.cap * sizeof(Item)));
+          l.spare = static_cast<Item*>(p);
+          l.spareCap = l.cap;
+        }
+        const int64_t nb = (l.count + kCompactItems - 1) / kCompactItems;
+        jobs.push_back(CompactJob{l.buf, l.spare, l.count, blocks, nb});
+        owners.push_back(&l);
+        blocks += nb;
+      }
+  if (!jobs.empty()) {
+    const int64_t nj = (int64_t)jobs.size();
+    if (nj > cjobsCap) {
+      if (cjobs) (void)hipFree(cjobs);
+      if (ctotals) (void)hipFree(ctotals);
+      cjobs = nullptr;
+      ctotals = nullptr;
+      void* p = nullptr;
+      HIP_TRY(hipMalloc(&p, (size_t)nj * sizeof(CompactJob)));
+      cjobs = static_cast<CompactJob*>(p);
+      HIP_TRY(hipMalloc(&p, (size_t)nj * sizeof(int)));
+      ctotals = static_cast<int*>(p);
+      cjobsCap = nj;
+    }
+    if (blocks > ccountsCap) {
+      if (ccounts) (void)hipFree(ccounts);
+      ccounts = nullptr;
+      void* p = nullptr;
+      HIP_TRY(hipMalloc(&p, (size_t)blocks * sizeof(int)));
+      ccounts = static_cast<int*>(p);
+      ccountsCap = blocks;
+    }
+    if (blocks > INT32_MAX) return fail("compaction: too many items");
+    HIP_TRY(hipMemcpyAsync(cjobs, jobs.data(), (size_t)nj * sizeof(CompactJob),
+                           hipMemcpyHostToDevice, stream));
+    compactCount<<<(unsigned)blocks, kBlock, 0, stream>>>(cjobs, (int)nj, ctl, ccounts);
+    compactScan<<<(unsigned)nj, kBlock, 0, stream>>>(cjobs, ccounts, ctotals);
+    compactScatter<<<(unsigned)blocks, kBlock, 0, stream>>>(cjobs, (int)nj, ctl, ccounts);
+    HIP_TRY(hipGetLastError());
+    std::vector<int> kept((size_t)nj, 0);
+    HIP_TRY(hipMemcpyAsync(kept.data(), ctotals, (size_t)nj * sizeof(int), hipMemcpyDeviceToHost,
+                           stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    for (size_t j = 0; j < owners.size(); ++j) {
+      DevList& l = *owners[j];
+      std::swap(l.buf, l.spare);
+      std::swap(l.cap, l.spareCap);
+      l.count = kept[j];
+    }
+  }
   listedItems = 0;
   deadItems = 0;
   for (int s = 0; s < maxSpots; ++s) {
