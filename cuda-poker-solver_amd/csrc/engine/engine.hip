@@ -109,9 +109,10 @@ enum ItemKind : uint8_t {
 //   kBwdDecide   a regret row, d strategy row, b node cfv row, c children's
 //                cfv block, m child stride (floats), n traverser combos,
 //                e = actor is the traverser
-//   kBwdChance   a branch x parent-slot map (i16: child slot or -1), b node
-//                cfv row, c children's cfv block, m child stride, n
-//                traverser combos (parent), na branches
+//   kBwdChance   a dealt-card x parent-slot map (i16: child slot or -1), b
+//                node cfv row, c children's cfv block, d the child serving
+//                each dealt card (u8), m child stride, n traverser combos
+//                (parent), na dealt cards
 //   kBwdFold /   a group of fold (or showdown) nodes of one board class:
 //   kBwdShowdown a the group's first TermNode, cnt nodes in the group, b the
 //                spot's chunk (TermNode offsets are relative to it), c the
@@ -119,8 +120,7 @@ enum ItemKind : uint8_t {
 //                segment bounds, then the u16 gather list: showdown, slots
 //                in strength order + the per-card segments; fold, the
 //                per-card segments), d per traverser combo metadata (fold: u32 cards
-//                + same; showdown: 4 u32), e gather entries per lane
-//                (showdown: 8, 12 or 16), na row registers per lane (3 or
+//                + same; showdown: 4 u32), na row registers per lane (3 or
 //                6), n traverser combos, m opponent combos
 struct alignas(16) Item {
   int64_t a, b, c, d;
@@ -705,6 +705,7 @@ __device__ __forceinline__ void chanceTile(const Dev& d, const Item& it) {
   // most 255 x 1326 floats): one VGPR per address instead of two
   const char* __restrict__ map = at<char>(d, it.a);
   const char* __restrict__ kid = at<char>(d, it.c);
+  const uint8_t* __restrict__ sib = at<uint8_t>(d, it.d);  // child serving each dealt card
   const uint32_t stride = (uint32_t)it.m;
   const int nb = it.na;
   constexpr int kChunk = 16;
@@ -720,7 +721,7 @@ __device__ __forceinline__ void chanceTile(const Dev& d, const Item& it) {
 #pragma unroll
     for (int k = 0; k < kChunk; ++k)
       x[k] = slot[k] >= 0
-                 ? *reinterpret_cast<const float*>(kid + 4u * ((uint32_t)(b0 + k) * stride + (uint32_t)slot[k]))
+                 ? *reinterpret_cast<const float*>(kid + 4u * ((uint32_t)sib[b0 + k] * stride + (uint32_t)slot[k]))
                  : 0.0f;
 #pragma unroll
     for (int k = 0; k < kChunk; ++k)
@@ -932,7 +933,7 @@ struct Arena {
 // Byte layout of one spot's chunk.
 struct Layout {
   int64_t bytes = 0;
-  int64_t toParent[2] = {0, 0}, bmap[2] = {0, 0}, gather[2] = {0, 0};
+  int64_t toParent[2] = {0, 0}, bmap[2] = {0, 0}, gather[2] = {0, 0}, dealSib = 0;
   int64_t cardSame[2] = {0, 0}, sdMeta[2] = {0, 0};
   int64_t regret = 0, strat = 0, cfv = 0, reach[2] = {0, 0};
   int64_t termNodes = 0;  // TermNode entries of every terminal group
@@ -955,8 +956,6 @@ size_t terminalLdsOf(const Item& it, int lanes) {
 
 // Host-side tables of one spot, built from its description.
 struct SpotTables {
-  std::vector<int16_t> bmap[2];       // per parent class: [branch][parent slot]
-  std::vector<int64_t> bmapOff[2];  // class -> element offset, -1: none
   std::vector<char> gather[2];  // per class and terminal kind, see gatherBlock
   std::vector<int64_t> sdOff[2], foldOff[2];  // class -> byte offset, -1: none
   std::vector<uint32_t> cardSame[2];  // per list entry: c1 | c2 << 8 | (same+1) << 16
@@ -1034,7 +1033,6 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
     e = u + 1 < N ? t->child_base[u + 1] : (int)t->n_child;
   };
   for (int p = 0; p < 2; ++p) {
-    T.bmapOff[p].assign(nc, -1);
     T.sdOff[p].assign(nc, -1);
     T.foldOff[p].assign(nc, -1);
   }
@@ -1055,17 +1053,10 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
     if (t->kind[u] == PPSG_CHANCE) {
       int b, e;
       kidsOf(u, b, e);
-      if (e - b > 255) return fail("more than 255 chance branches");
-      for (int p = 0; p < 2; ++p) {
-        if (T.bmapOff[p][cl] >= 0) continue;
-        T.bmapOff[p][cl] = (int64_t)T.bmap[p].size();
-        const int np = t->cls_n[p][cl];
-        for (int k = b; k < e; ++k) {
-          const int ccl = t->cls[t->child_flat[k]];
-          const int16_t* fp = t->from_parent[p] + t->from_parent_off[p][ccl];
-          T.bmap[p].insert(T.bmap[p].end(), fp, fp + np);
-        }
-      }
+      if (t->deal_n[cl] > 255 || t->deal_off[cl] < 0 || e <= b)
+        return fail("chance node without a valid deal table");
+      for (int32_t k = 0; k < t->deal_n[cl]; ++k)
+        if (t->deal_sib[t->deal_off[cl] + k] >= e - b) return fail("deal table names a missing child");
     } else if (t->kind[u] == PPSG_SHOWDOWN || t->kind[u] == PPSG_FOLD) {
       const bool sd = t->kind[u] == PPSG_SHOWDOWN;
       for (int p = 0; p < 2; ++p) T.gatherBlock(t, p, cl, sd);
@@ -1080,11 +1071,12 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
   Layout& L = S.L;
   for (int p = 0; p < 2; ++p) {
     L.toParent[p] = L.add(2 * t->list_total[p]);
-    L.bmap[p] = L.add(2 * (int64_t)T.bmap[p].size());
+    L.bmap[p] = L.add(2 * t->deal_map_total[p]);
     L.gather[p] = L.add((int64_t)T.gather[p].size());
     L.cardSame[p] = L.add(4 * t->list_total[p]);
     L.sdMeta[p] = L.add(16 * t->list_total[p]);
   }
+  L.dealSib = L.add(t->deal_total);
   L.regret = L.add(4 * t->reg_total);
   L.strat = L.add(4 * t->reg_total);
   L.cfv = L.add(4 * t->cfv_total);
@@ -1174,12 +1166,13 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
           Item it{};
           it.kind = kBwdChance;
           it.r = 1;
-          it.a = L.bmap[tr] + 2 * T.bmapOff[tr][cl];
+          it.a = L.bmap[tr] + 2 * t->deal_map_off[tr][cl];
           it.b = L.cfv + 4 * t->cfv_off[u];
           it.c = L.cfv + 4 * t->cfv_off[first];
+          it.d = L.dealSib + t->deal_off[cl];
           it.m = (int32_t)stride;
           it.n = nT;
-          it.na = (uint8_t)nb;
+          it.na = (uint8_t)t->deal_n[cl];
           it.tile = (uint8_t)q;
           bwd.push_back(it);
         }
@@ -1242,9 +1235,11 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
       kidsOf(u, b, e);
       for (int k = b; k < e; ++k) {
         const int ccl = t->cls[t->child_flat[k]];
-        const double nck = (double)t->cls_n[0][ccl] + t->cls_n[1][ccl];
-        S.kindBytes[kFwdChance] += 8 * nck;
-        S.kindBytes[kBwdChance] += 4 * nck;
+        S.kindBytes[kFwdChance] += 8 * ((double)t->cls_n[0][ccl] + t->cls_n[1][ccl]);
+      }
+      for (int32_t k = 0; k < t->deal_n[cl]; ++k) {
+        const int ccl = t->cls[t->child_flat[b + t->deal_sib[t->deal_off[cl] + k]]];
+        S.kindBytes[kBwdChance] += 4 * ((double)t->cls_n[0][ccl] + t->cls_n[1][ccl]);
       }
       S.kindBytes[kBwdChance] += 4 * (n0 + n1);
     } else {
@@ -1679,13 +1674,14 @@ int ppsg_engine::add(const ppsg_tree_desc* t, int* slot) {
   int rc = 0;
   for (int p = 0; p < 2 && rc == 0; ++p) {
     if (put(L.toParent[p], t->to_parent[p], 2 * t->list_total[p]) ||
-        put(L.bmap[p], T.bmap[p].data(), 2 * (int64_t)T.bmap[p].size()) ||
+        put(L.bmap[p], t->deal_map[p], 2 * t->deal_map_total[p]) ||
         put(L.gather[p], T.gather[p].data(), (int64_t)T.gather[p].size()) ||
         put(L.cardSame[p], T.cardSame[p].data(), 4 * t->list_total[p]) ||
         put(L.sdMeta[p], t->sd_meta[p], 16 * t->list_total[p]) ||
         put(L.reach[p] + 4 * t->reach_off[p][0], t->root_reach[p], 4 * (int64_t)t->n_base[1 - p]))
       rc = -1;
   }
+  if (rc == 0 && put(L.dealSib, t->deal_sib, t->deal_total)) rc = -1;
   if (rc == 0 &&
       put(L.termNodes, T.termNodes.data(), (int64_t)(T.termNodes.size() * sizeof(TermNode))))
     rc = -1;

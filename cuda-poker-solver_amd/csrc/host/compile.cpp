@@ -18,13 +18,103 @@ namespace {
 
 constexpr int kMaxActions = 32;
 
+int swapSuit(int card, int s1, int s2) {
+  const int s = card / 13, r = card % 13;
+  return s == s1 ? s2 * 13 + r : s == s2 ? s1 * 13 + r : card;
+}
+
 struct Build {
   const PostflopSpot& spot;
   const pf::BetConfig& cfg;
   CompiledTree& T;
   std::map<std::vector<uint8_t>, int> classIds;
+  bool rangeSym[4][4] = {};             // suits interchangeable in both ranges
+  std::vector<int32_t> baseOfPair[2];   // a * 52 + b -> base slot or -1
 
-  Build(const PostflopSpot& s, const pf::BetConfig& c, CompiledTree& t) : spot(s), cfg(c), T(t) {}
+  Build(const PostflopSpot& s, const pf::BetConfig& c, CompiledTree& t) : spot(s), cfg(c), T(t) {
+    for (int p = 0; p < 2; ++p) {
+      baseOfPair[p].assign(52 * 52, -1);
+      for (int i = 0; i < T.n[p]; ++i) {
+        const int a = T.cards[p][2 * i], b = T.cards[p][2 * i + 1];
+        baseOfPair[p][a * 52 + b] = baseOfPair[p][b * 52 + a] = i;
+      }
+    }
+    if (!T.suitIso) return;
+    for (int s1 = 0; s1 < 4; ++s1)
+      for (int s2 = s1 + 1; s2 < 4; ++s2) {
+        bool sym = true;
+        for (int p = 0; p < 2 && sym; ++p) {
+          const Range& r = p == 0 ? spot.oop : spot.ip;
+          for (int k = 0; k < kCombos && sym; ++k) {
+            uint8_t a, b;
+            comboCards(k, a, b);
+            const int a2 = swapSuit(a, s1, s2), b2 = swapSuit(b, s1, s2);
+            sym = r.w[k] == r.w[comboIndex((uint8_t)std::min(a2, b2), (uint8_t)std::max(a2, b2))];
+          }
+        }
+        rangeSym[s1][s2] = rangeSym[s2][s1] = sym;
+      }
+  }
+
+  // Per dealt card c (not on `board`): the card whose subtree serves it
+  // (c itself unless suit isomorphism maps it to a smaller card).
+  void dealReps(const std::vector<uint8_t>& board, int repOf[52]) const {
+    int rankset[4] = {0, 0, 0, 0};
+    for (uint8_t c : board) rankset[c / 13] |= 1 << (c % 13);
+    int isoSuit[4] = {-1, -1, -1, -1};
+    if (T.suitIso)
+      for (int s1 = 1; s1 < 4; ++s1)
+        for (int s2 = 0; s2 < s1; ++s2)
+          if (rankset[s1] == rankset[s2] && rangeSym[s1][s2]) {
+            isoSuit[s1] = s2;
+            break;
+          }
+    for (int c = 0; c < 52; ++c) repOf[c] = isoSuit[c / 13] >= 0 ? isoSuit[c / 13] * 13 + c % 13 : c;
+  }
+
+  // The deal table of parent class cl (first chance node of the class):
+  // repCls[k] is the class of the k-th child (representative card order).
+  void dealTable(int cl, const std::vector<uint8_t>& board, const int repOf[52],
+                 const std::vector<int>& repCard, const std::vector<int>& repCls) {
+    T.dealOff[cl] = (int64_t)T.dealSib.size();
+    int n = 0;
+    for (int c = 0; c < 52; ++c) {
+      if (std::find(board.begin(), board.end(), (uint8_t)c) != board.end()) continue;
+      const int k = (int)(std::find(repCard.begin(), repCard.end(), repOf[c]) - repCard.begin());
+      T.dealSib.push_back((uint8_t)k);
+      T.dealCard.push_back((uint8_t)c);
+      ++n;
+    }
+    T.dealN[cl] = n;
+    for (int p = 0; p < 2; ++p) {
+      T.dealMapOff[p][cl] = (int64_t)T.dealMap[p].size();
+      const int np = T.clsN[p][cl];
+      const int64_t off = T.clsListOff[p][cl];
+      const int32_t* list = T.listBase[p].data() + off;
+      for (int b = 0; b < n; ++b) {
+        const int c = T.dealCard[T.dealOff[cl] + b];
+        const int k = T.dealSib[T.dealOff[cl] + b];
+        const int16_t* fp = T.fromParent[p].data() + T.fromParentOff[p][repCls[k]];
+        if (repOf[c] == c) {
+          T.dealMap[p].insert(T.dealMap[p].end(), fp, fp + np);
+          continue;
+        }
+        // the value of combo h after c = the representative's value of the
+        // suit-swapped h (the swap maps c to its representative)
+        const int s1 = c / 13, s2 = repOf[c] / 13;
+        for (int s = 0; s < np; ++s) {
+          const int base = list[s];
+          const int a = swapSuit(T.cards[p][2 * base], s1, s2);
+          const int bb = swapSuit(T.cards[p][2 * base + 1], s1, s2);
+          const int base2 = baseOfPair[p][a * 52 + bb];
+          const int32_t* it = std::lower_bound(list, list + np, base2);
+          if (base2 < 0 || it == list + np || *it != base2)
+            throw std::runtime_error("suit isomorphism: swapped combo missing from the class list");
+          T.dealMap[p].push_back(fp[it - list]);
+        }
+      }
+    }
+  }
 
   bool holds(int p, int base, int card) const {
     return T.cards[p][2 * base] == card || T.cards[p][2 * base + 1] == card;
@@ -42,6 +132,10 @@ struct Build {
     T.clsNBoard.push_back((int32_t)board.size());
     T.clsParent.push_back(parentCls);
     T.clsDealt.push_back(dealt);
+    T.dealN.push_back(0);
+    T.dealOff.push_back(-1);
+    T.dealMapOff[0].push_back(-1);
+    T.dealMapOff[1].push_back(-1);
     for (int p = 0; p < 2; ++p) {
       T.clsListOff[p].push_back((int64_t)T.listBase[p].size());
       T.fromParentOff[p].push_back((int64_t)T.fromParent[p].size());
@@ -106,18 +200,23 @@ struct Build {
         return id;
       }
       T.kind[id] = PPSG_CHANCE;
+      int repOf[52];
+      dealReps(board, repOf);
       std::vector<uint8_t> next = board;
       next.push_back(0);
-      int branches = 0;
+      std::vector<int> repCard, repCls;
       for (int c = 0; c < 52; ++c) {
         if (std::find(board.begin(), board.end(), (uint8_t)c) != board.end()) continue;
+        if (repOf[c] != c) continue;  // served by a smaller card's subtree
         next.back() = (uint8_t)c;
         const int ccl = classOf(next, cl, c);
         node(id, next, ccl, potBase + sc0 + sc1, 0, 0, 0, false, false, depth + 1, pc0 + sc0,
              pc1 + sc1, 0);
-        ++branches;
+        repCard.push_back(c);
+        repCls.push_back(ccl);
       }
-      T.na[id] = (uint8_t)branches;  // chance: branch count
+      T.na[id] = (uint8_t)repCard.size();  // chance: child count
+      if (T.dealOff[cl] < 0) dealTable(cl, board, repOf, repCard, repCls);
       return id;
     }
     const std::vector<pf::TreeAction> acts = pf::betActions(
@@ -328,10 +427,11 @@ void discountCoefs(int t, int iters, const std::string& algo, double out[3]) {
   out[2] = std::pow(tg / (tg + 1.0), gamma);
 }
 
-CompiledTree compileSpot(const PostflopSpot& spot, const pf::BetConfig& cfg) {
+CompiledTree compileSpot(const PostflopSpot& spot, const pf::BetConfig& cfg, bool suitIso) {
   CompiledTree T;
   T.spot = spot;
   T.cfg = cfg;
+  T.suitIso = suitIso;
   for (int p = 0; p < 2; ++p) {
     const Range& r = p == 0 ? spot.oop : spot.ip;
     for (int i = 0; i < kCombos; ++i) {
@@ -474,7 +574,14 @@ ppsg_tree_desc CompiledTree::desc() const {
     d.reach_off[p] = reachOff[p].data();
     d.reach_total[p] = reachTotal[p];
     d.root_reach[p] = rootReach[p].data();
+    d.deal_map_off[p] = dealMapOff[p].data();
+    d.deal_map_total[p] = (int64_t)dealMap[p].size();
+    d.deal_map[p] = dealMap[p].data();
   }
+  d.deal_n = dealN.data();
+  d.deal_off = dealOff.data();
+  d.deal_total = (int64_t)dealSib.size();
+  d.deal_sib = dealSib.data();
   d.n_class = (int32_t)clsNBoard.size();
   d.cls_board_n = clsNBoard.data();
   d.n_nodes = numNodes;
@@ -509,9 +616,18 @@ double CompiledTree::iterationBytes() const {
       const double np = owner[u] == 0 ? n0 : n1, nq = owner[u] == 0 ? n1 : n0;
       b += 4 * np * (5 * a + 1) + 4 * np * (2 * a + 1) + 4 * nq * (a + 1);
     } else if (kind[u] == PPSG_CHANCE) {
-      for (int32_t k = childBase[u]; k < (u + 1 < numNodes ? childBase[u + 1] : (int32_t)childFlat.size()); ++k) {
+      // forward: each child's reach written from the parent's (8 per combo);
+      // backward: each dealt card's values read (4 per combo of the child
+      // serving it), the node's own written
+      const int32_t kb = childBase[u];
+      for (int32_t k = kb; k < (u + 1 < numNodes ? childBase[u + 1] : (int32_t)childFlat.size()); ++k) {
         const int c = childFlat[k];
-        b += 12.0 * (listSize(0, c) + listSize(1, c));
+        b += 8.0 * (listSize(0, c) + listSize(1, c));
+      }
+      const int cl = cls[u];
+      for (int32_t d = 0; d < dealN[cl]; ++d) {
+        const int c = childFlat[kb + dealSib[dealOff[cl] + d]];
+        b += 4.0 * (listSize(0, c) + listSize(1, c));
       }
       b += 4 * (n0 + n1);
     } else {

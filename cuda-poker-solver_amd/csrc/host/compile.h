@@ -49,6 +49,17 @@ struct CompiledTree {
   std::vector<uint16_t> segSlot[2];
   std::vector<uint16_t> sorted[2];
   std::vector<uint32_t> sdMeta[2];
+  // chance deals per parent class (ppsg_tree_desc deal_*): every card the
+  // class's chance nodes deal, ascending; the child whose rows serve it
+  // (itself, or under suit isomorphism the child of the card it maps to);
+  // per player, parent slot -> slot in that child's list (suit-swapped for
+  // mapped cards), -1 where the card blocks the combo
+  std::vector<int32_t> dealN;
+  std::vector<int64_t> dealOff;
+  std::vector<uint8_t> dealSib, dealCard;
+  std::vector<int64_t> dealMapOff[2];
+  std::vector<int16_t> dealMap[2];
+  bool suitIso = false;
 
   // nodes
   int64_t numNodes = 0;
@@ -75,7 +86,14 @@ struct CompiledTree {
 
 // Compiles a spot. Throws std::runtime_error on a bet config wider than
 // the engine supports (more than 32 actions at a node).
-CompiledTree compileSpot(const PostflopSpot& spot, const pf::BetConfig& cfg);
+//
+// suitIso: deal only one card of each set of suit-isomorphic cards at chance
+// nodes (postflop-solver src/card.rs:247-360, applied in src/solver.rs:204-219):
+// suits s1 > s2 are interchangeable at a chance node when the board holds
+// the same ranks in both and both ranges are unchanged by swapping them; a
+// card of suit s1 is then served by the same-rank card of the smallest such
+// s2, with its combos suit-swapped. Trees without such suits are unchanged.
+CompiledTree compileSpot(const PostflopSpot& spot, const pf::BetConfig& cfg, bool suitIso = false);
 
 // Disjoint weighted pair mass over the base lists (cuda/gpu_cfr.cu:871-888).
 double pairMass(const CompiledTree& t);

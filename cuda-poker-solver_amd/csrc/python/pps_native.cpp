@@ -196,12 +196,15 @@ py::dict jobDict(const pps::gpu::JobInfo& j) {
 class Solver {
  public:
   Solver(const py::object& board, const py::object& oop, const py::object& ip, int64_t pot,
-         int64_t stack, const py::object& bets, const py::object& raises, int maxRaises) {
+         int64_t stack, const py::object& bets, const py::object& raises, int maxRaises,
+         bool isomorphism) {
     pps::gpu::PostflopSpot spot;
     pps::pf::BetConfig cfg;
     readSpot(board, oop, ip, pot, stack, bets, raises, maxRaises, spot, cfg);
+    pps::gpu::SolverOptions opt;
+    opt.suitIsomorphism = isomorphism;
     py::gil_scoped_release nogil;
-    s_ = std::make_unique<GpuPostflopSolver>(spot, cfg);
+    s_ = std::make_unique<GpuPostflopSolver>(spot, cfg, opt);
   }
 
   Solver(std::shared_ptr<SpotServer> srv, int id) : srv_(std::move(srv)), id_(id) {}
@@ -467,16 +470,18 @@ class Solver {
 
   py::dict treeStructure() const {
     const pps::gpu::TreeStructure t = on([](GpuPostflopSolver& g) { return g.treeStructure(); });
-    py::list kinds, base, kids, boards;
+    py::list kinds, base, kids, boards, nboard;
     for (uint8_t k : t.kinds) kinds.append((int)k);
     for (int c : t.childBase) base.append(c);
     for (int c : t.children) kids.append(c);
     for (uint8_t c : t.boards) boards.append((int)c);
+    for (uint8_t c : t.nBoard) nboard.append((int)c);
     py::dict d;
     d["kinds"] = kinds;
     d["child_base"] = base;
     d["children"] = kids;
     d["boards"] = boards;
+    d["n_board"] = nboard;
     return d;
   }
 
@@ -510,14 +515,16 @@ class Server {
 
   std::unique_ptr<Solver> add(const py::object& board, const py::object& oop, const py::object& ip,
                               int64_t pot, int64_t stack, const py::object& bets,
-                              const py::object& raises, int maxRaises) {
+                              const py::object& raises, int maxRaises, bool isomorphism) {
     pps::gpu::PostflopSpot spot;
     pps::pf::BetConfig cfg;
     readSpot(board, oop, ip, pot, stack, bets, raises, maxRaises, spot, cfg);
+    pps::gpu::SolverOptions opt;
+    opt.suitIsomorphism = isomorphism;
     int id = -1;
     {
       py::gil_scoped_release nogil;
-      id = srv_->add(spot, cfg);
+      id = srv_->add(spot, cfg, opt);
     }
     return std::make_unique<Solver>(srv_, id);
   }
@@ -561,11 +568,13 @@ PYBIND11_MODULE(pps_native, m) {
   py::class_<Solver>(m, "Solver")
       .def(py::init([](const py::object& board, const py::object& oop, const py::object& ip,
                        int64_t pot, int64_t stack, const py::object& bets,
-                       const py::object& raises, int maxRaises) {
-             return std::make_unique<Solver>(board, oop, ip, pot, stack, bets, raises, maxRaises);
+                       const py::object& raises, int maxRaises, bool isomorphism) {
+             return std::make_unique<Solver>(board, oop, ip, pot, stack, bets, raises, maxRaises,
+                                             isomorphism);
            }),
            py::arg("board"), py::arg("oop"), py::arg("ip"), py::arg("pot"), py::arg("stack"),
-           py::arg("bets") = "0.75,a", py::arg("raises") = "2.5,3", py::arg("max_raises") = 0)
+           py::arg("bets") = "0.75,a", py::arg("raises") = "2.5,3", py::arg("max_raises") = 0,
+           py::arg("isomorphism") = false)
       .def_static("load", &Solver::load, py::arg("path"))
       .def("solve", &Solver::solve, py::arg("max_iters"), py::arg("algo") = "dcfr",
            py::arg("target") = -1.0)
@@ -601,7 +610,7 @@ PYBIND11_MODULE(pps_native, m) {
            py::arg("max_step") = 32)
       .def("add", &Server::add, py::arg("board"), py::arg("oop"), py::arg("ip"), py::arg("pot"),
            py::arg("stack"), py::arg("bets") = "0.75,a", py::arg("raises") = "2.5,3",
-           py::arg("max_raises") = 0)
+           py::arg("max_raises") = 0, py::arg("isomorphism") = false)
       .def("load", &Server::load, py::arg("path"))
       .def("stats", &Server::stats);
   py::register_exception<pps::gpu::ArenaFull>(m, "ArenaFull", PyExc_MemoryError);

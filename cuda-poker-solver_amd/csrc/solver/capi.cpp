@@ -158,8 +158,15 @@ int pps_schedule(const char* algo, int t, int iters, double out3[3]) {
 
 int pps_compile_info(const pps_spot* spot, const pps_bet_config* cfg, int64_t out[8],
                      double* bytes) {
+  return pps_compile_info_ex(spot, cfg, 0, out, bytes);
+}
+
+int pps_compile_info_ex(const pps_spot* spot, const pps_bet_config* cfg, int flags,
+                        int64_t out[8], double* bytes) {
   return guard([&] {
-    const pps::gpu::CompiledTree t = pps::gpu::compileSpot(toSpot(spot), toCfg(cfg));
+    if (flags & ~PPS_SUIT_ISOMORPHISM) throw std::runtime_error("unknown option flags");
+    const pps::gpu::CompiledTree t =
+        pps::gpu::compileSpot(toSpot(spot), toCfg(cfg), (flags & PPS_SUIT_ISOMORPHISM) != 0);
     if (t.empty) throw std::runtime_error("empty ranges");
     int64_t sd = 0, fd = 0, ch = 0;
     for (uint8_t k : t.kind) {
@@ -185,6 +192,20 @@ int pps_solver_create(const pps_spot* spot, const pps_bet_config* cfg, pps_solve
     if (!out) throw std::runtime_error("null output");
     auto h = std::make_unique<pps_solver>();
     h->s = std::make_unique<pps::gpu::GpuPostflopSolver>(toSpot(spot), toCfg(cfg));
+    *out = h.release();
+    return 0;
+  });
+}
+
+int pps_solver_create_ex(const pps_spot* spot, const pps_bet_config* cfg, int flags,
+                         pps_solver** out) {
+  return guard([&] {
+    if (!out) throw std::runtime_error("null output");
+    if (flags & ~PPS_SUIT_ISOMORPHISM) throw std::runtime_error("unknown option flags");
+    pps::gpu::SolverOptions opt;
+    opt.suitIsomorphism = (flags & PPS_SUIT_ISOMORPHISM) != 0;
+    auto h = std::make_unique<pps_solver>();
+    h->s = std::make_unique<pps::gpu::GpuPostflopSolver>(toSpot(spot), toCfg(cfg), opt);
     *out = h.release();
     return 0;
   });
@@ -598,6 +619,18 @@ int pps_server_add(pps_server* srv, const pps_spot* spot, const pps_bet_config* 
     const pps::gpu::PostflopSpot sp = toSpot(spot);
     const pps::pf::BetConfig c = toCfg(cfg);
     return admitted(srv, out, [&] { return srv->srv->add(sp, c); });
+  });
+}
+
+int pps_server_add_ex(pps_server* srv, const pps_spot* spot, const pps_bet_config* cfg, int flags,
+                      pps_solver** out) {
+  return guard([&] {
+    if (flags & ~PPS_SUIT_ISOMORPHISM) throw std::runtime_error("unknown option flags");
+    const pps::gpu::PostflopSpot sp = toSpot(spot);
+    const pps::pf::BetConfig c = toCfg(cfg);
+    pps::gpu::SolverOptions opt;
+    opt.suitIsomorphism = (flags & PPS_SUIT_ISOMORPHISM) != 0;
+    return admitted(srv, out, [&] { return srv->srv->add(sp, c, opt); });
   });
 }
 

@@ -184,10 +184,23 @@ GpuPostflopSolver::GpuPostflopSolver(const PostflopSpot& spot, const pf::BetConf
   }
 }
 
-GpuPostflopSolver::GpuPostflopSolver(Deferred, const PostflopSpot& spot, const pf::BetConfig& cfg)
+GpuPostflopSolver::GpuPostflopSolver(const PostflopSpot& spot, const pf::BetConfig& cfg,
+                                     const SolverOptions& opt)
     : impl_(new Impl()) {
   try {
-    init(spot, cfg, false);
+    init(spot, cfg, true, opt);
+  } catch (...) {
+    delete impl_;
+    impl_ = nullptr;
+    throw;
+  }
+}
+
+GpuPostflopSolver::GpuPostflopSolver(Deferred, const PostflopSpot& spot, const pf::BetConfig& cfg,
+                                     const SolverOptions& opt)
+    : impl_(new Impl()) {
+  try {
+    init(spot, cfg, false, opt);
   } catch (...) {
     delete impl_;
     impl_ = nullptr;
@@ -253,9 +266,10 @@ int GpuPostflopSolver::slot() const { return impl_->T.empty ? -1 : impl_->slot; 
 
 int GpuPostflopSolver::rootWidth(int tr) const { return impl_->T.empty ? 0 : impl_->T.n[tr]; }
 
-void GpuPostflopSolver::init(const PostflopSpot& spot, const pf::BetConfig& cfg, bool device) {
+void GpuPostflopSolver::init(const PostflopSpot& spot, const pf::BetConfig& cfg, bool device,
+                             const SolverOptions& opt) {
   Impl* I = impl_;
-  I->T = compileSpot(spot, cfg);
+  I->T = compileSpot(spot, cfg, opt.suitIsomorphism);
   if (I->T.empty) return;
   numNodes_ = (int)I->T.numNodes;
   maxDepth_ = I->T.maxDepth;
@@ -546,7 +560,12 @@ RunoutSummary GpuPostflopSolver::runoutSummary(int decideIdx, int actionIdx) {
   const int ch = I->T.childFlat[I->T.childBase[u] + actionIdx];
   if (I->T.kind[ch] != PPSG_CHANCE) throw std::runtime_error("action does not lead to a deal");
   RunoutSummary out;
-  const int nb = (int)I->childCount(ch);
+  // every dealt card, in card order; under suit isomorphism a card mapped to
+  // another is reported with that card's subtree (its aggregates are the
+  // same by symmetry)
+  const int dcl = I->T.cls[ch];
+  const int nb = I->T.dealN[dcl];
+  const int64_t doff = I->T.dealOff[dcl];
   // first decide node under a branch (breadth-first through further deals)
   auto firstDecide = [&](int start) -> int {
     if (I->T.kind[start] == PPSG_DECIDE) return start;
@@ -564,8 +583,8 @@ RunoutSummary GpuPostflopSolver::runoutSummary(int decideIdx, int actionIdx) {
   out.branches.resize(nb);
   int firstIdx = -1;
   for (int b = 0; b < nb; ++b) {
-    const int bc = I->T.childFlat[I->T.childBase[ch] + b];
-    out.branches[b].card = I->T.clsDealt[I->T.cls[bc]];
+    const int bc = I->T.childFlat[I->T.childBase[ch] + I->T.dealSib[doff + b]];
+    out.branches[b].card = I->T.dealCard[doff + b];
     const int fd = firstDecide(bc);
     if (fd >= 0) {
       out.branches[b].nextDecide = I->T.decideOf[fd];
@@ -640,9 +659,11 @@ TreeStructure GpuPostflopSolver::treeStructure() const {
   t.childBase = T.childBase;
   t.children = T.childFlat;
   t.boards.reserve((size_t)T.numNodes * 5);
+  t.nBoard.reserve((size_t)T.numNodes);
   for (int64_t u = 0; u < T.numNodes; ++u) {
     const int cl = T.cls[u];
     for (int i = 0; i < 5; ++i) t.boards.push_back(i < T.clsNBoard[cl] ? T.clsBoard[cl][i] : 0);
+    t.nBoard.push_back((uint8_t)T.clsNBoard[cl]);
   }
   return t;
 }
@@ -767,6 +788,9 @@ struct Reader {
 void GpuPostflopSolver::save(const std::string& path) const {
   const Impl* I = impl_;
   if (I->T.empty) throw std::runtime_error("cannot save an empty solver");
+  if (I->T.suitIso)
+    throw std::runtime_error("cannot save a suit-isomorphic solve: solution files (PPS1 v1) do not "
+                             "record the option, and its tree differs from the file's");
   FILE* f = std::fopen(path.c_str(), "wb");
   if (!f) throw std::runtime_error("cannot open for writing: " + path);
   try {

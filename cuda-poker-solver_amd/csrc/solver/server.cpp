@@ -104,10 +104,10 @@ int SpotServer::insert(std::unique_ptr<GpuPostflopSolver> s) {
   return id;
 }
 
-int SpotServer::add(const PostflopSpot& spot, const pf::BetConfig& cfg) {
+int SpotServer::add(const PostflopSpot& spot, const pf::BetConfig& cfg, const SolverOptions& opt) {
   // compile on this thread, outside the lock; upload under it
   std::unique_ptr<GpuPostflopSolver> s(
-      new GpuPostflopSolver(GpuPostflopSolver::Deferred{}, spot, cfg));
+      new GpuPostflopSolver(GpuPostflopSolver::Deferred{}, spot, cfg, opt));
   Turn t(*this);
   return insert(std::move(s));
 }

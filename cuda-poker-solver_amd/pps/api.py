@@ -59,6 +59,9 @@ class SpotSpec(BaseModel):
     bets: BetSizesArg = "0.75,a"
     raises: Union[str, List[float]] = "2.5,3"
     max_raises: int = Field(0, ge=0)
+    isomorphism: bool = Field(False, description="addition: deal one card per set of "
+                              "suit-isomorphic cards (fewer nodes; chance nodes then have "
+                              "fewer children than dealt cards)")
 
 
 class SolveParams(BaseModel):
@@ -200,7 +203,7 @@ def _admit(make):
 def _new_solver(spec: SpotSpec):
     return _admit(lambda: server().add(board=spec.board, oop=spec.oop, ip=spec.ip, pot=spec.pot,
                                        stack=spec.stack, bets=spec.bets, raises=spec.raises,
-                                       max_raises=spec.max_raises))
+                                       max_raises=spec.max_raises, isomorphism=spec.isomorphism))
 
 
 def _plain(obj: Any) -> Any:
@@ -261,8 +264,8 @@ def _children(tree, u: int) -> List[int]:
 def _dealt(tree, parent: int, child: int) -> str:
     """Card a chance branch adds: in the child's board, not the parent's.
     Undealt board slots are 0, which is also a card (2c), so the parent's
-    board length comes from its branch count (52 - cards on board)."""
-    n = 52 - len(_children(tree, parent))
+    board length comes from the tree's per-node board sizes."""
+    n = tree["n_board"][parent]
     b = tree["boards"]
     have = list(b[5 * parent:5 * parent + n])
     extra = [c for c in b[5 * child:5 * child + n + 1]]

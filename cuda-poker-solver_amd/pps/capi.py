@@ -68,6 +68,7 @@ CAPI_SYMBOLS = [
     "pps_solver_iteration_bytes", "pps_solver_last_solve_ms", "pps_solver_profile",
     "pps_solver_engine_info", "pps_serve_defaults", "pps_serve", "pps_compile_footprint",
     "pps_server_create", "pps_server_destroy", "pps_server_add", "pps_server_load",
+    "pps_solver_create_ex", "pps_server_add_ex", "pps_compile_info_ex",
     "pps_server_stats", "pps_solver_start_solve", "pps_solver_start_continue", "pps_solver_job",
     "pps_solver_cancel",
     "pps_batch_profile",
@@ -102,8 +103,13 @@ def lib():
                                       ctypes.c_int, P(ctypes.c_int64), ctypes.c_int]
         L.pps_schedule.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, P(ctypes.c_double)]
         L.pps_compile_info.argtypes = [P(Spot), P(BetConfig), P(ctypes.c_int64), P(ctypes.c_double)]
+        L.pps_compile_info_ex.argtypes = [P(Spot), P(BetConfig), ctypes.c_int, P(ctypes.c_int64),
+                                          P(ctypes.c_double)]
         L.pps_solver_create.argtypes = [P(Spot), P(BetConfig), P(ctypes.c_void_p)]
         L.pps_solver_load.argtypes = [ctypes.c_char_p, P(ctypes.c_void_p)]
+        L.pps_solver_create_ex.argtypes = [P(Spot), P(BetConfig), ctypes.c_int, P(ctypes.c_void_p)]
+        L.pps_server_add_ex.argtypes = [ctypes.c_void_p, P(Spot), P(BetConfig), ctypes.c_int,
+                                        P(ctypes.c_void_p)]
         L.pps_solver_destroy.argtypes = [ctypes.c_void_p]
         L.pps_solver_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p,
                                        ctypes.c_double]
@@ -181,6 +187,7 @@ class ServerCounters(ctypes.Structure):
 
 
 JOB_STATUS = {0: None, 1: "running", 2: "done", 3: "cancelled", 4: "error"}
+SUIT_ISOMORPHISM = 1  # option flag of pps_solver_create_ex / pps_server_add_ex
 ARENA_FULL = 2
 
 
@@ -304,10 +311,11 @@ def schedule(algo, t, iters):
     return tuple(out)
 
 
-def compile_info(spot, cfg):
+def compile_info(spot, cfg, isomorphism=False):
     out = (ctypes.c_int64 * 8)()
     b = ctypes.c_double()
-    _check(lib().pps_compile_info(ctypes.byref(spot), ctypes.byref(cfg), out, ctypes.byref(b)))
+    _check(lib().pps_compile_info_ex(ctypes.byref(spot), ctypes.byref(cfg),
+                                     SUIT_ISOMORPHISM if isomorphism else 0, out, ctypes.byref(b)))
     keys = ["nodes", "max_depth", "decide", "reg_total", "showdown", "fold", "chance", "classes"]
     d = dict(zip(keys, [int(x) for x in out]))
     d["alg_bytes"] = b.value
@@ -317,10 +325,13 @@ def compile_info(spot, cfg):
 class Solver:
     """GpuPostflopSolver through the C ABI."""
 
-    def __init__(self, spot=None, cfg=None, path=None):
+    def __init__(self, spot=None, cfg=None, path=None, isomorphism=False):
         h = ctypes.c_void_p()
         if path is not None:
             _check(lib().pps_solver_load(path.encode(), ctypes.byref(h)))
+        elif isomorphism:
+            _check(lib().pps_solver_create_ex(ctypes.byref(spot), ctypes.byref(cfg),
+                                              SUIT_ISOMORPHISM, ctypes.byref(h)))
         else:
             _check(lib().pps_solver_create(ctypes.byref(spot), ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h

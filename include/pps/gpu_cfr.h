@@ -76,6 +76,18 @@ struct TreeStructure {
   std::vector<int> childBase;
   std::vector<int> children;
   std::vector<uint8_t> boards;  // 5 card slots per node
+  std::vector<uint8_t> nBoard;  // addition: cards on each node's board
+};
+
+// Addition (no reference counterpart): solver options.
+struct SolverOptions {
+  // Deal one card per set of suit-isomorphic cards at chance nodes (suits
+  // with the same board ranks, both ranges unchanged by swapping them); the
+  // others are served by that card's subtree with suits swapped. Fewer
+  // nodes on boards and ranges with such suits; the tree (node ids,
+  // children of chance nodes) then differs from the reference's, and
+  // solution files cannot record it (save refuses).
+  bool suitIsomorphism = false;
 };
 
 struct NodeEv {
@@ -110,6 +122,7 @@ struct RunoutSummary {
 class GpuPostflopSolver {
  public:
   GpuPostflopSolver(const PostflopSpot& spot, const pf::BetConfig& cfg);
+  GpuPostflopSolver(const PostflopSpot& spot, const pf::BetConfig& cfg, const SolverOptions& opt);
   explicit GpuPostflopSolver(const std::string& solutionPath);
   ~GpuPostflopSolver();
   GpuPostflopSolver(const GpuPostflopSolver&) = delete;
@@ -178,10 +191,12 @@ class GpuPostflopSolver {
   // the constructor, uploaded by attach (queries here; iterations run by the
   // server's batched steps).
   struct Deferred {};
-  GpuPostflopSolver(Deferred, const PostflopSpot& spot, const pf::BetConfig& cfg);
+  GpuPostflopSolver(Deferred, const PostflopSpot& spot, const pf::BetConfig& cfg,
+                    const SolverOptions& opt = SolverOptions());
   GpuPostflopSolver(Deferred, const std::string& solutionPath);
   void attach(ppsg_engine* arena);  // throws ArenaFull when there is no room
-  void init(const PostflopSpot& spot, const pf::BetConfig& cfg, bool device = true);
+  void init(const PostflopSpot& spot, const pf::BetConfig& cfg, bool device = true,
+            const SolverOptions& opt = SolverOptions());
   void loadFrom(const std::string& path, bool device);
   void replayChunks(int iterations, double target);
 

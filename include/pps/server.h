@@ -69,7 +69,8 @@ class SpotServer {
   // Compiles a spot (on the calling thread) and uploads it; returns its id.
   // Throws ArenaFull when the arena has no room (remove spots and retry),
   // std::runtime_error for a bad spot.
-  int add(const PostflopSpot& spot, const pf::BetConfig& cfg);
+  int add(const PostflopSpot& spot, const pf::BetConfig& cfg,
+          const SolverOptions& opt = SolverOptions());
   // Loads a PPS1 solution file into the arena (GpuPostflopSolver's loading
   // constructor); returns its id.
   int load(const std::string& path);

@@ -59,11 +59,20 @@ int pps_schedule(const char* algo, int t, int iters, double out3[3]);
    [5] fold [6] chance [7] board classes; *bytes = algorithmic bytes/iter */
 int pps_compile_info(const pps_spot* spot, const pps_bet_config* cfg, int64_t out[8],
                      double* bytes);
+/* Addition: pps_compile_info of the tree compiled with option flags
+   (PPS_SUIT_ISOMORPHISM). */
+int pps_compile_info_ex(const pps_spot* spot, const pps_bet_config* cfg, int flags,
+                        int64_t out[8], double* bytes);
 
 /* ---- solver: GpuPostflopSolver ---- */
 int pps_solver_create(const pps_spot* spot, const pps_bet_config* cfg,
                       pps_solver** out);                      /* cuda/gpu_cfr.h:145 */
 int pps_solver_load(const char* path, pps_solver** out);     /* cuda/gpu_cfr.h:150 */
+/* Addition: pps_solver_create with option flags (PPS_SUIT_ISOMORPHISM:
+   include/pps/gpu_cfr.h SolverOptions::suitIsomorphism). */
+#define PPS_SUIT_ISOMORPHISM 1
+int pps_solver_create_ex(const pps_spot* spot, const pps_bet_config* cfg, int flags,
+                         pps_solver** out);
 void pps_solver_destroy(pps_solver* s);                       /* cuda/gpu_cfr.h:151 */
 int pps_solver_solve(pps_solver* s, int iters, const char* algo,
                      double target);                          /* cuda/gpu_cfr.h:161 */
@@ -222,6 +231,8 @@ int pps_server_create(const pps_server_options* opt, pps_server** out); /* opt m
 void pps_server_destroy(pps_server* srv);
 int pps_server_add(pps_server* srv, const pps_spot* spot, const pps_bet_config* cfg,
                    pps_solver** out);
+int pps_server_add_ex(pps_server* srv, const pps_spot* spot, const pps_bet_config* cfg, int flags,
+                      pps_solver** out);
 int pps_server_load(pps_server* srv, const char* path, pps_solver** out);
 int pps_server_stats(pps_server* srv, pps_server_counters* out);
 /* Jobs of a served solver: start and return at once (-1 with a running

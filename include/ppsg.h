@@ -71,6 +71,20 @@ typedef struct ppsg_tree_desc {
   const float* term_val;           /* 8 per node: [tr][win, tie, lose, -] */
   int64_t reg_total, cfv_total, reach_total[2];
   const float* root_reach[2];      /* layout tr root row: opponent weights */
+  /* chance deals, per parent class (0 / -1 for classes without a chance
+     node): the deal_n[c] cards the class's chance nodes deal (ascending);
+     deal_sib[deal_off[c] + b] = the child (index among the chance node's
+     children) whose values serve dealt card b (b itself without suit
+     isomorphism); deal_map[p][deal_map_off[p][c] + b * n + s] = slot in that
+     child's list whose value counts for parent slot s (suit-swapped for a
+     card served by another card's child), or -1 where the card blocks s */
+  const int32_t* deal_n;
+  const int64_t* deal_off;
+  int64_t deal_total;
+  const uint8_t* deal_sib;
+  const int64_t* deal_map_off[2];
+  int64_t deal_map_total[2];
+  const int16_t* deal_map[2];
 } ppsg_tree_desc;
 
 typedef struct ppsg_engine ppsg_t;

@@ -1,0 +1,105 @@
+"""Suit isomorphism at chance nodes (opt-in). A solve that deals one card
+per set of suit-isomorphic cards computes the full tree's solve with the
+mapped subtrees' values summed in another order: the first iteration is
+bit-identical, the second agrees to f32 rounding, and from there RM+
+amplifies ulp-level differences in near-zero regrets (as in any
+whole-trajectory comparison), so long solves are held to the parity gates
+(root EV within 0.01 chips, exploitability ratio in [0.5, 2], strategies
+within 0.10 at 500 iterations). Runout summaries cover every dealt card;
+spots without interchangeable suits compile to the very same tree."""
+import numpy as np
+import pytest
+
+import pps
+from pps import capi, configs
+
+pytestmark = pytest.mark.gpu
+
+SYM_OOP, SYM_IP = "TT+,AKo,AQs,KQs", "99+,AJs+,KQo"
+
+
+def solvers(board, iters, bets="0.5,a", raises="2.5", pot=60, stack=200):
+    out = []
+    for iso in (False, True):
+        s = pps.Solver(board=board, oop=SYM_OOP, ip=SYM_IP, pot=pot, stack=stack, bets=bets,
+                       raises=raises, isomorphism=iso)
+        s.solve(iters)
+        out.append(s)
+    return out
+
+
+def close(a, b, tol, what):
+    d = float(np.max(np.abs(np.asarray(a, dtype=float) - np.asarray(b, dtype=float))))
+    assert d <= tol, f"{what}: max diff {d:g} > {tol:g}"
+
+
+@pytest.mark.parametrize("board", ["Ks8s5s2s", "7s7h7d2c"])
+def test_turn_early_iterations_agree(board):
+    for iters, tol in ((1, 0.0), (2, 1e-4)):
+        full, iso = solvers(board, iters)
+        assert iso.num_nodes < full.num_nodes
+        a, b = full.stats(), iso.stats()
+        for k in ("ev_oop", "ev_ip", "exploitability"):
+            close(a[k], b[k], tol, f"{k} after {iters}")
+        assert a["pair_mass"] == b["pair_mass"]
+        ra, rb = full.root_ev(), iso.root_ev()
+        for side in ("oop", "ip"):
+            close(ra[side]["ev"], rb[side]["ev"], max(tol, 1e-12) * 10, f"{side} combo EVs")
+        close(full.strategy(0)["freqs"], iso.strategy(0)["freqs"], tol * 10, "root strategy")
+
+
+@pytest.mark.parametrize("board", ["Ks8s5s2s", "7s7h7d2c"])
+def test_turn_gates_at_500(board):
+    full, iso = solvers(board, 500)
+    a, b = full.stats(), iso.stats()
+    assert abs(a["ev_oop"] - b["ev_oop"]) < 0.01, (a, b)
+    assert 0.5 <= b["exploitability"] / a["exploitability"] <= 2.0, (a, b)
+    close(full.strategy(0)["freqs"], iso.strategy(0)["freqs"], 0.10, "root strategy")
+    close(full.strategy(1)["freqs"], iso.strategy(1)["freqs"], 0.10, "node 1 strategy")
+
+
+def test_flop_runout_summary_agrees():
+    full, iso = solvers("Qs9s2d", 2, bets="0.75", raises="")
+    assert iso.num_nodes < 0.7 * full.num_nodes
+    close(full.stats()["ev_oop"], iso.stats()["ev_oop"], 1e-4, "EV oop")
+    # node 1 (IP after the OOP check), action 0 (check): the turn deal
+    a, b = full.runout_summary(1, 0), iso.runout_summary(1, 0)
+    assert len(a["branches"]) == len(b["branches"]) == 49
+    assert [x["card"] for x in a["branches"]] == [x["card"] for x in b["branches"]]
+    for x, y in zip(a["branches"], b["branches"]):
+        close(x["freq"], y["freq"], 1e-4, f"freq after {x['card']}")
+        for side in ("oop", "ip"):
+            close(x["agg_ev"][side], y["agg_ev"][side], 1e-4, f"{side} EV after {x['card']}")
+    # the tree lists one child per representative turn card
+    t = iso.tree_structure()
+    deal = t["children"][t["child_base"][iso.decide_nodes()[1]["children"][0]]]
+    assert t["kinds"][iso.decide_nodes()[1]["children"][0]] == 3 and deal > 0
+    assert t["n_board"][0] == 3
+
+
+def test_no_interchangeable_suits_same_tree():
+    """Random weights (the plain turn spot): no two suits are
+    interchangeable, so the option changes nothing, bit for bit."""
+    s = configs.plain_turn()
+    sp = capi.make_spot(s["board"], s["w0"], s["w1"], s["pot"], s["stack"])
+    cf = capi.make_config(s["bets"], s["raises"], s["allin"], False, s["max_raises"])
+    assert capi.compile_info(sp, cf) == capi.compile_info(sp, cf, isomorphism=True)
+    a, b = capi.Solver(sp, cf), capi.Solver(sp, cf, isomorphism=True)
+    a.solve(64, s["algo"])
+    b.solve(64, s["algo"])
+    assert a.stats() == b.stats()
+    assert np.array_equal(a.rows(0), b.rows(0))
+
+
+def test_served_isomorphic_spot_matches_and_save_refuses(tmp_path):
+    own = pps.Solver(board="Ks8s5s2s", oop=SYM_OOP, ip=SYM_IP, pot=60, stack=200, bets="0.5,a",
+                     raises="2.5", isomorphism=True)
+    own.solve(100)
+    srv = pps.Server(arena_bytes=1 << 30)
+    s = srv.add(board="Ks8s5s2s", oop=SYM_OOP, ip=SYM_IP, pot=60, stack=200, bets="0.5,a",
+                raises="2.5", isomorphism=True)
+    s.solve(100)
+    assert s.stats() == own.stats()
+    assert s.num_nodes == own.num_nodes
+    with pytest.raises(RuntimeError):
+        own.save(str(tmp_path / "iso.sol"))

@@ -157,3 +157,37 @@ def test_workload_streams_are_seeded_and_size_filtered():
     rivers = [x for x in s if len(x["board"]) == 5]
     assert 0 < len(rivers) < len(s)
     assert all(2000 <= x["nodes"] <= 60000 for x in s if len(x["board"]) == 4)
+
+
+def test_suit_isomorphism_shrinks_only_symmetric_trees():
+    """Compile only (no device): with suit-symmetric ranges, monotone and
+    two-tone boards deal fewer subtrees; a rainbow turn board, or ranges
+    with random weights, compile to the same tree."""
+    from conftest import board_of
+
+    def info(board, oop, ip, iso):
+        w0 = capi.parse_range(oop) if isinstance(oop, str) else oop
+        sp = capi.make_spot(board_of(board), w0, capi.parse_range(ip), 60, 200)
+        cf = capi.make_config([0.5], [2.5], False, False, 0)
+        return capi.compile_info(sp, cf, isomorphism=iso)
+
+    sym = ("TT+,AKo,AQs,KQs", "99+,AJs+,KQo")
+    for board, least in (("Ks8s5s2s", 2.0), ("Ks8s5s", 3.0), ("Qs9s2d", 1.5)):
+        full, iso = info(board, *sym, False), info(board, *sym, True)
+        assert full["nodes"] >= least * iso["nodes"], (board, full["nodes"], iso["nodes"])
+        assert iso["alg_bytes"] < full["alg_bytes"]
+    assert info("Qs9h2d7c", *sym, False) == info("Qs9h2d7c", *sym, True)
+    # a heart combo weighted apart from its club / diamond images: hearts no
+    # longer swap, clubs and diamonds still do
+    w = capi.parse_range(sym[0])
+    ah, kh = board_of("AhKh")
+    k = [i for i, (x, y) in enumerate((x, y) for x in range(52) for y in range(x + 1, 52))
+         if {x, y} == {ah, kh}][0]
+    w[k] = 0.5
+    a, b, c = info("Ks8s5s2s", w, sym[1], True), info("Ks8s5s2s", w, sym[1], False), \
+        info("Ks8s5s2s", *sym, True)
+    assert c["nodes"] < a["nodes"] < b["nodes"]
+    s = configs.plain_turn()
+    sp = capi.make_spot(s["board"], s["w0"], s["w1"], s["pot"], s["stack"])
+    cf = capi.make_config(s["bets"], s["raises"], s["allin"], False, s["max_raises"])
+    assert capi.compile_info(sp, cf) == capi.compile_info(sp, cf, isomorphism=True)

@@ -19,12 +19,13 @@ def build_tree():
     fold / call after the bet; every street-closing action deals the river
     (48 branches); the first river branch after check-check has an OOP
     decision, every other branch is a showdown."""
-    kinds, kids, boards = [], [], []
+    kinds, kids, boards, nboard = [], [], [], []
 
     def node(kind, board):
         kinds.append(kind)
         kids.append([])
         boards.append(list(board) + [0] * (5 - len(board)))
+        nboard.append(len(board))
         return len(kinds) - 1
 
     decides = []
@@ -57,7 +58,7 @@ def build_tree():
         base.append(len(flat))
         flat += k
     tree = {"kinds": kinds, "child_base": base, "children": flat,
-            "boards": [c for b in boards for c in b]}
+            "boards": [c for b in boards for c in b], "n_board": nboard}
     nodes = [{"index": i, "node_id": u, "player": p, "depth": 0, "n_board": 4, "n_combos": 3,
               "actions": [{"kind": k, "amount": a} for k, a in acts], "children": kids[u]}
              for i, (u, p, acts) in enumerate(sorted(decides))]
