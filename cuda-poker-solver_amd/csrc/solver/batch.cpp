@@ -153,7 +153,7 @@ std::vector<SpotResult> serve(const std::vector<SpotRequest>& reqs, const ServeO
       }
       Compiled x;
       try {
-        auto T = std::make_unique<CompiledTree>(compileSpot(reqs[r].spot, reqs[r].cfg));
+        auto T = std::make_unique<CompiledTree>(compileSpot(reqs[r].spot, reqs[r].cfg, reqs[r].options.suitIsomorphism));
         if (T->empty) {
           x.error = "empty ranges";
         } else {
@@ -381,7 +381,7 @@ BatchProfile profileBatch(const std::vector<SpotRequest>& reqs, const std::strin
     th.emplace_back([&] {
       for (int k; (k = next.fetch_add(1)) < (int)reqs.size();) {
         try {
-          trees[k] = compileSpot(reqs[k].spot, reqs[k].cfg);
+          trees[k] = compileSpot(reqs[k].spot, reqs[k].cfg, reqs[k].options.suitIsomorphism);
         } catch (const std::exception& e) {
           errs[k] = e.what();
         }

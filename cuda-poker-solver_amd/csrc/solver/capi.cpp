@@ -471,6 +471,7 @@ void pps_serve_defaults(pps_serve_options* opt) {
   opt->compile_threads = d.compileThreads;
   opt->real_time = d.realTime ? 1 : 0;
   opt->prefill = d.prefill ? 1 : 0;
+  opt->flags = 0;
 }
 
 int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const double* arrival_ms,
@@ -479,7 +480,10 @@ int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const do
   return guard([&] {
     if (n < 0 || (n > 0 && (!spots || !cfgs || !out))) throw std::runtime_error("null argument");
     pps::gpu::ServeOptions o;
+    pps::gpu::SolverOptions spotOpt;
     if (opt) {
+      if (opt->flags & ~PPS_SUIT_ISOMORPHISM) throw std::runtime_error("unknown option flags");
+      spotOpt.suitIsomorphism = (opt->flags & PPS_SUIT_ISOMORPHISM) != 0;
       o.targetFrac = opt->target_frac;
       o.checkEvery = opt->check_every;
       o.maxIters = opt->max_iters;
@@ -496,6 +500,7 @@ int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const do
       reqs[i].spot = toSpot(spots + i);
       reqs[i].cfg = toCfg(cfgs + i);
       reqs[i].arrivalMs = arrival_ms ? arrival_ms[i] : 0.0;
+      reqs[i].options = spotOpt;
     }
     pps::gpu::ServeReport rep;
     const std::vector<pps::gpu::SpotResult> res = pps::gpu::serve(reqs, o, &rep);

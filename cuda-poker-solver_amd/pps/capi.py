@@ -31,7 +31,7 @@ class ServeOptions(ctypes.Structure):
                 ("max_iters", ctypes.c_int32), ("arena_bytes", ctypes.c_int64),
                 ("max_spots", ctypes.c_int32), ("max_live", ctypes.c_int32),
                 ("compile_threads", ctypes.c_int32), ("real_time", ctypes.c_int32),
-                ("prefill", ctypes.c_int32)]
+                ("prefill", ctypes.c_int32), ("flags", ctypes.c_int32)]
 
 
 class ServeResult(ctypes.Structure):

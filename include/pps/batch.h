@@ -27,6 +27,7 @@ struct SpotRequest {
   PostflopSpot spot;
   pf::BetConfig cfg;
   double arrivalMs = 0.0;  // offset from the start of the stream
+  SolverOptions options;   // e.g. suit isomorphism
 };
 
 struct SpotResult {

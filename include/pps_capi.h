@@ -137,6 +137,7 @@ typedef struct pps_serve_options {
   int32_t compile_threads;  /* 8 */
   int32_t real_time;        /* 1: honour arrival_ms; 0: all arrive at once */
   int32_t prefill;          /* 1: admit all that fit before the first step */
+  int32_t flags;            /* option flags of every request (PPS_SUIT_ISOMORPHISM) */
 } pps_serve_options;
 
 typedef struct pps_serve_result {

@@ -103,3 +103,26 @@ def test_served_isomorphic_spot_matches_and_save_refuses(tmp_path):
     assert s.num_nodes == own.num_nodes
     with pytest.raises(RuntimeError):
         own.save(str(tmp_path / "iso.sol"))
+
+
+def test_serve_isomorphic_requests_equal_solvers():
+    """pps_serve with PPS_SUIT_ISOMORPHISM: each served spot equals an
+    isomorphic single-spot solve run for the same number of iterations."""
+    from conftest import board_of
+    spots = []
+    for board in ("Ks8s5s2s", "7s7h7d2c", "Qs9s2d8s"):
+        sp = capi.make_spot(board_of(board), capi.parse_range(SYM_OOP), capi.parse_range(SYM_IP),
+                            60, 200)
+        spots.append((sp, capi.make_config([0.5], [2.5], True, False, 0)))
+    res, rep = capi.serve([p[0] for p in spots], [p[1] for p in spots], "dcfr", None,
+                          arena_bytes=1 << 30, real_time=0, flags=capi.SUIT_ISOMORPHISM)
+    for (sp, cf), r in zip(spots, res):
+        assert r["ok"], r["error"]
+        own = capi.Solver(sp, cf, isomorphism=True)
+        own.solve(r["iterations"], "dcfr")
+        st = own.stats()
+        assert abs(r["ev0"] - st["ev_oop"]) <= 1e-12, (r["ev0"], st)
+        assert abs(r["expl"] - st["exploitability"]) <= 1e-12, (r["expl"], st)
+        assert r["nodes"] == own.num_nodes
+        plain = capi.compile_info(sp, cf)["nodes"]
+        assert r["nodes"] < plain
