@@ -517,16 +517,25 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
     const double total = xs(nO);
 #pragma unroll
     for (int q = 0; q < ROWS; ++q) {
+      if ((int)tid + q * NT >= nT) continue;  // whole waves past the list skip
       const uint4 m = meta[q];
       const int le = m.x & 0xffff, te = m.x >> 16;
       const int c1 = m.w & 0xff, c2 = (m.w >> 8) & 0xff;
       const int a1 = (m.y & 0xffff) + c1, b1 = (m.y >> 16) + c1;
       const int a2 = (m.z & 0xffff) + c2, b2 = (m.z >> 16) + c2;
       const int same = (int)(m.w >> 16) - 1;
-      const double rs = same >= 0 ? (double)raw[same] : 0.0;
-      double win = xs(le) - (double)Y[a1] - (double)Y[a2];
-      const double leq = xs(te) - (double)Y[b1] - (double)Y[b2] + rs;
-      const double all = total - (double)segTot[c1] - (double)segTot[c2] + rs;
+      // the nine lookups are issued together (no branch on `same`: the
+      // sentinel raw[nO] is 0), then the arithmetic
+      const float fs = raw[same >= 0 ? same : nO];
+      const double xl = xs(le), xt = xs(te);
+      const float y1 = Y[a1], y2 = Y[a2], y3 = Y[b1], y4 = Y[b2];
+      const float t1 = segTot[c1], t2 = segTot[c2];
+      __builtin_amdgcn_sched_group_barrier(0x0100, 9, 0);  // LDS reads
+      __builtin_amdgcn_sched_group_barrier(0x0002, 64, 0);  // then VALU
+      const double rs = (double)fs;
+      double win = xl - (double)y1 - (double)y2;
+      const double leq = xt - (double)y3 - (double)y4 + rs;
+      const double all = total - (double)t1 - (double)t2 + rs;
       if (win < 0) win = 0;
       double tie = leq - win;
       if (tie < 0) tie = 0;
