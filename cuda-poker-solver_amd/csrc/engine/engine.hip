@@ -48,7 +48,11 @@ namespace {
 
 constexpr int kBlock = 256;  // 4 wave64s
 constexpr int kWideBlock = 512;  // wide showdown groups: 8 wave64s
-constexpr int kTile = 256;   // combos per tile item
+// Decide and chance work items are per wave: a 256-lane block runs four
+// consecutive items, one per wave, so a node's last partial chunk wastes at
+// most 63 lanes (not 255) and the waves of a block may serve different nodes.
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kMaxList = 1326;
 constexpr int kRowRegs = (kMaxList + kBlock - 1) / kBlock;  // 6
 constexpr int64_t kAlign = 256;
@@ -57,7 +61,7 @@ constexpr int kGatherHeader = 256;  // bytes: 53 int32 segment bounds + pad
 // fold: entries of a card's segment per lane (4 lanes per card)
 constexpr int kFoldPart = 12;
 constexpr int kMaxGroup = 64;  // terminal nodes per work item at most
-// combos per lane (kTile apart) of forward decide, backward decide and
+// combos per lane (kWave apart) of forward decide, backward decide and
 // forward chance items (chosen by measurement: more loads in flight per lane
 // help the read-mostly forward items; the read-modify-write backward tiles
 // do best with one combo per lane and more lanes busy)
@@ -80,14 +84,17 @@ int fail(const std::string& msg) {
                   std::to_string(__LINE__) + ")");                                        \
   } while (0)
 
-// Per-spot control block: the device-side schedule index and table, and
-// whether the spot's work items still run (retired spots stay in the merged
-// work lists until the next compaction, masked out by `alive`).
+// Per-spot control block: the device-side schedule index and table, the
+// running iteration's coefficients, and whether the spot's work items still
+// run (retired spots stay in the merged work lists until the next
+// compaction, masked out by `alive`).
 struct alignas(32) Ctl {
   long long counter;  // schedule index of the running iteration
-  const float* coef;  // 3 floats per schedule index
+  const float* coef;  // 3 floats per schedule index (+ one zero entry)
+  float cur[3];       // coef[3 * counter ..]: positive, negative, average
   int alive;
 };
+static_assert(sizeof(Ctl) == 32, "control block is one 32-byte load");
 
 enum ItemKind : uint8_t {
   kFwdDecide = 0,
@@ -104,8 +111,9 @@ enum ItemKind : uint8_t {
 //                reach block (stride n), n opponent combos, na actions
 //   kFwdChance   a to-parent map (u16), b parent reach row, c child reach
 //                row, n child opponent combos, v0 = 1/(52-board-4)
-//   (decide and chance items cover combos [tile * kTile * r, +kTile * r),
-//   lane t taking t, t + kTile, ... so that r loads per row are in flight)
+//   (decide and chance items are one wave's work: combos [tile * kWave * r,
+//   +kWave * r), lane t taking t, t + kWave, ... so that r loads per row
+//   are in flight)
 //   kBwdDecide   a regret row, d strategy row, b node cfv row, c children's
 //                cfv block, m child stride (floats), n traverser combos,
 //                e = actor is the traverser
@@ -126,8 +134,8 @@ struct alignas(16) Item {
   int64_t a, b, c, d;
   int32_t n, m, e, f;
   uint8_t kind, na;
-  uint8_t tile;  // decide / chance: which run of kTile * r combos
-  uint8_t r;     // decide / chance: combos per lane (kTile apart)
+  uint8_t tile;  // decide / chance: which run of kWave * r combos
+  uint8_t r;     // decide / chance: combos per lane (kWave apart)
   float v0;
   int32_t cnt;   // terminal groups: nodes in the group
   int32_t pad;
@@ -153,6 +161,34 @@ __device__ __forceinline__ T* at(const Dev& d, int64_t off) {
   return reinterpret_cast<T*>(d.arena + off);
 }
 
+// Per-wave work items: this wave's item index (a scalar, so the item's
+// fields are scalar loads) and the lane within the wave.
+__device__ __forceinline__ int waveItem() {
+  return (int)blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
+}
+__device__ __forceinline__ int waveLane() { return (int)threadIdx.x & (kWave - 1); }
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// Item w (clamped into the list, so the load needs no branch first), all
+// 64 bytes in one round trip: the empty asm takes every word, so none of
+// the loads is split off behind the kind test; the arena bases are taken
+// with the list arguments.
+__device__ __forceinline__ Item fetchItem(const Dev& d, const Item* __restrict__ items, int w,
+                                          int count) {
+  asm volatile("" : : "s"(d.arena), "s"(d.ctl));
+  const v4i* p = reinterpret_cast<const v4i*>(items + min(w, count - 1));
+  v4i q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+  asm volatile("" : "+s"(q0), "+s"(q1), "+s"(q2), "+s"(q3));
+  Item it;
+  v4i* o = reinterpret_cast<v4i*>(&it);
+  o[0] = q0;
+  o[1] = q1;
+  o[2] = q2;
+  o[3] = q3;
+  return it;
+}
+
 __device__ __forceinline__ float sigmaOf(float x, float acc, float uniform, bool plus) {
   if (plus) return acc > 0.0f ? (x > 0.0f ? x / acc : 0.0f) : uniform;
   return acc > 0.0f ? x / acc : uniform;
@@ -164,10 +200,10 @@ __device__ __forceinline__ float sigmaOf(float x, float acc, float uniform, bool
 
 // Decide (opponent acts): child reach a = reach * sigma_a. NA > 0: action
 // count fixed at compile time (the row values stay in registers); every
-// lane handles up to R combos kTile apart, all loads issued first.
+// lane handles up to R combos kWave apart, all loads issued first.
 template <int NA, int R>
 __device__ __forceinline__ void fwdDecide(const Dev& d, const Item& it, int fromStrat) {
-  const int c0 = it.tile * kTile * R + (int)threadIdx.x;
+  const int c0 = it.tile * kWave * R + waveLane();
   if (c0 >= it.n) return;
   const int n = it.n, na = NA > 0 ? NA : it.na;
   const float* __restrict__ reach = at<float>(d, it.b);
@@ -179,14 +215,14 @@ __device__ __forceinline__ void fwdDecide(const Dev& d, const Item& it, int from
     float x[R][NA > 0 ? NA : 1], r[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      const int c = c0 + k * kTile;
+      const int c = c0 + k * kWave;
       r[k] = c < n ? reach[c] : 0.0f;
 #pragma unroll
       for (int a = 0; a < NA; ++a) x[k][a] = c < n ? src[(int64_t)a * n + c] : 0.0f;
     }
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      const int c = c0 + k * kTile;
+      const int c = c0 + k * kWave;
       if (c >= n) continue;
       float acc = 0.0f;
 #pragma unroll
@@ -197,7 +233,7 @@ __device__ __forceinline__ void fwdDecide(const Dev& d, const Item& it, int from
     return;
   }
   for (int k = 0; k < R; ++k) {
-    const int c = c0 + k * kTile;
+    const int c = c0 + k * kWave;
     if (c >= n) return;
     const float rr = reach[c];
     float acc = 0.0f;
@@ -212,16 +248,16 @@ __device__ __forceinline__ void fwdDecide(const Dev& d, const Item& it, int from
 
 __device__ __forceinline__ void fwdItem(const Dev& d, const Item& it, int fromStrat) {
   if (it.kind == kFwdChance) {
-    const int c0 = it.tile * kTile * kRChance + (int)threadIdx.x;
+    const int c0 = it.tile * kWave * kRChance + waveLane();
     const float* __restrict__ in = at<float>(d, it.b);
     const uint16_t* __restrict__ tp = at<uint16_t>(d, it.a);
     float* __restrict__ out = at<float>(d, it.c);
     int p[kRChance];
 #pragma unroll
-    for (int k = 0; k < kRChance; ++k) p[k] = c0 + k * kTile < it.n ? tp[c0 + k * kTile] : -1;
+    for (int k = 0; k < kRChance; ++k) p[k] = c0 + k * kWave < it.n ? tp[c0 + k * kWave] : -1;
 #pragma unroll
     for (int k = 0; k < kRChance; ++k)
-      if (p[k] >= 0) out[c0 + k * kTile] = in[p[k]] * it.v0;
+      if (p[k] >= 0) out[c0 + k * kWave] = in[p[k]] * it.v0;
     return;
   }
   switch (it.na) {
@@ -586,8 +622,11 @@ __global__ __launch_bounds__(kWideBlock) void showdownWideKernel(Dev d,
 // Forward launch of one depth: opponent reach of the next depth.
 // kindMask: profiling attribution only (-1 = every kind in real launches).
 __global__ __launch_bounds__(kBlock) void fwdKernel(Dev d, const Item* __restrict__ items,
-                                                    int fromStrat, int kindMask, int masked) {
-  const Item it = items[blockIdx.x];
+                                                    int count, int fromStrat, int kindMask,
+                                                    int masked) {
+  const int w = waveItem();
+  const Item it = fetchItem(d, items, w, count);
+  if (w >= count) return;
   if (!((kindMask >> it.kind) & 1)) return;
   if (masked && !d.ctl[it.f].alive) return;
   fwdItem(d, it, fromStrat);
@@ -597,12 +636,12 @@ __global__ __launch_bounds__(kBlock) void fwdKernel(Dev d, const Item* __restric
 // Backward.
 // ---------------------------------------------------------------------------
 
-// Decide tile: every lane handles up to R traverser combos kTile apart
+// Decide tile: every lane handles up to R traverser combos kWave apart
 // (all row loads issued first). NA > 0: action count fixed at compile time
 // (values stay in registers); NA == 0: generic loop.
 template <int MODE, int NA, int R>
 __device__ __forceinline__ void decideTile(const Dev& d, const Item& it) {
-  const int c0 = it.tile * kTile * R + (int)threadIdx.x;
+  const int c0 = it.tile * kWave * R + waveLane();
   const int n = it.n;
   if (c0 >= n) return;
   const int na = NA > 0 ? NA : it.na;
@@ -618,7 +657,7 @@ __device__ __forceinline__ void decideTile(const Dev& d, const Item& it) {
     for (int a = 0; a < na; ++a)
 #pragma unroll
       for (int k = 0; k < R; ++k) {
-        const int c = c0 + k * kTile;
+        const int c = c0 + k * kWave;
         if (c < n) {
           const float x = kid[a * stride + c];
           v[k] = sum ? v[k] + x : fmaxf(v[k], x);
@@ -626,7 +665,7 @@ __device__ __forceinline__ void decideTile(const Dev& d, const Item& it) {
       }
 #pragma unroll
     for (int k = 0; k < R; ++k)
-      if (c0 + k * kTile < n) out[c0 + k * kTile] = v[k];
+      if (c0 + k * kWave < n) out[c0 + k * kWave] = v[k];
     return;
   }
   constexpr bool plus = MODE == PPSG_MODE_CFR;
@@ -638,7 +677,7 @@ __device__ __forceinline__ void decideTile(const Dev& d, const Item& it) {
     float x[R][A], v[R][A], sm[R][A];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      const int c = c0 + k * kTile;
+      const int c = c0 + k * kWave;
 #pragma unroll
       for (int a = 0; a < A; ++a) {
         x[k][a] = c < n ? src[(int64_t)a * n + c] : 0.0f;
@@ -649,14 +688,13 @@ __device__ __forceinline__ void decideTile(const Dev& d, const Item& it) {
     float pos = 0.0f, neg = 0.0f, avg = 0.0f;
     if (MODE == PPSG_MODE_CFR) {
       const Ctl& ctl = d.ctl[it.f];
-      const float* co = ctl.coef + 3 * ctl.counter;
-      pos = co[0];
-      neg = co[1];
-      avg = co[2];
+      pos = ctl.cur[0];
+      neg = ctl.cur[1];
+      avg = ctl.cur[2];
     }
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      const int c = c0 + k * kTile;
+      const int c = c0 + k * kWave;
       if (c >= n) continue;
       float acc = 0.0f;
 #pragma unroll
@@ -678,7 +716,7 @@ __device__ __forceinline__ void decideTile(const Dev& d, const Item& it) {
     return;
   }
   for (int k = 0; k < R; ++k) {
-    const int c = c0 + k * kTile;
+    const int c = c0 + k * kWave;
     if (c >= n) return;
     float acc = 0.0f;
     for (int a = 0; a < na; ++a) {
@@ -691,8 +729,7 @@ __device__ __forceinline__ void decideTile(const Dev& d, const Item& it) {
     out[c] = val;
     if (MODE != PPSG_MODE_CFR) continue;
     const Ctl& ctl = d.ctl[it.f];
-    const float* co = ctl.coef + 3 * ctl.counter;
-    const float pos = co[0], neg = co[1], avg = co[2];
+    const float pos = ctl.cur[0], neg = ctl.cur[1], avg = ctl.cur[2];
     for (int a = 0; a < na; ++a) {
       const float x = src[(int64_t)a * n + c];
       const float sg = sigmaOf(x, acc, uniform, plus);
@@ -707,7 +744,7 @@ __device__ __forceinline__ void decideTile(const Dev& d, const Item& it) {
 // go in chunks: all map loads of a chunk, then all value loads, then the
 // adds (keeps many loads in flight; same summation order).
 __device__ __forceinline__ void chanceTile(const Dev& d, const Item& it) {
-  const int i = it.tile * kTile + (int)threadIdx.x;
+  const int i = it.tile * kWave + waveLane();
   const int n = it.n;
   if (i >= n) return;
   // 32-bit unsigned byte offsets from uniform bases (a branch block is at
@@ -741,8 +778,10 @@ __device__ __forceinline__ void chanceTile(const Dev& d, const Item& it) {
 
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void bwdKernel(Dev d, const Item* __restrict__ items,
-                                                    int kindMask, int masked) {
-  const Item it = items[blockIdx.x];
+                                                    int count, int kindMask, int masked) {
+  const int w = waveItem();
+  const Item it = fetchItem(d, items, w, count);
+  if (w >= count) return;
   if (!((kindMask >> it.kind) & 1)) return;
   if (masked && !d.ctl[it.f].alive) return;
   if (it.kind == kBwdChance) {
@@ -761,12 +800,25 @@ __global__ __launch_bounds__(kBlock) void bwdKernel(Dev d, const Item* __restric
 }
 
 // Bumps the schedule index of every active spot (the last node of an
-// iteration's graph).
+// iteration's graph) and fetches the next iteration's coefficients (runs
+// never pass the table's end, and the entry there is zero).
 __global__ void advanceCounters(Ctl* ctl, const int* __restrict__ slots, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     Ctl& c = ctl[slots[i]];
-    c.counter = c.counter + 1;
+    const long long t = c.counter + 1;
+    c.counter = t;
+    if (c.coef)
+      for (int k = 0; k < 3; ++k) c.cur[k] = c.coef[3 * t + k];
   }
+}
+
+// Coefficients of a spot's current schedule index (after the host sets the
+// index or the table); zeros outside the table's len entries.
+__global__ void refreshCoef(Ctl* ctl, int slot, long long len) {
+  if (threadIdx.x != 0) return;
+  Ctl& c = ctl[slot];
+  const bool in = c.coef && c.counter >= 0 && c.counter < len;
+  for (int k = 0; k < 3; ++k) c.cur[k] = in ? c.coef[3 * c.counter + k] : 0.0f;
 }
 
 // Sets every control block's alive flag (walks of a subset of spots mask all
@@ -1103,7 +1155,7 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
   const int D = t->max_depth + 1;
   for (int tr = 0; tr < 2; ++tr)
     for (int ps = 0; ps < 2; ++ps) S.items[tr][ps].assign(D, {});
-  auto tiles = [](int n, int r) { return (n + kTile * r - 1) / (kTile * r); };
+  auto tiles = [](int n, int r) { return (n + kWave * r - 1) / (kWave * r); };
   std::vector<int> groupOf[2] = {std::vector<int>((size_t)2 * nc, 0), std::vector<int>((size_t)2 * nc, 0)};  // (class, kind) -> group index + 1
   std::vector<std::vector<TermNode>> members[2];
   for (int64_t u = 0; u < N; ++u) {
@@ -1796,7 +1848,8 @@ int64_t ppsg_engine::kernelsPerIter() const {
 int ppsg_engine::launchFwd(int tr, int dpt, int fromStrat, int kindMask, hipStream_t s) {
   const DevList& l = lists[tr][0][dpt];
   if (l.count == 0) return 0;
-  fwdKernel<<<(unsigned)l.count, kBlock, 0, s>>>(dev(), l.buf, fromStrat, kindMask, masked());
+  const unsigned g = (unsigned)((l.count + kWavesPerBlock - 1) / kWavesPerBlock);
+  fwdKernel<<<g, kBlock, 0, s>>>(dev(), l.buf, (int)l.count, fromStrat, kindMask, masked());
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1815,11 +1868,11 @@ int ppsg_engine::launchTerm(int tr, int ps, hipStream_t s) {
 int ppsg_engine::launchBwd(int tr, int dpt, int mode, int kindMask, hipStream_t s) {
   const DevList& l = lists[tr][1][dpt];
   if (l.count == 0) return 0;
-  const unsigned g = (unsigned)l.count;
-  const int mk = masked();
-  if (mode == PPSG_MODE_CFR) bwdKernel<PPSG_MODE_CFR><<<g, kBlock, 0, s>>>(dev(), l.buf, kindMask, mk);
-  else if (mode == PPSG_MODE_EV) bwdKernel<PPSG_MODE_EV><<<g, kBlock, 0, s>>>(dev(), l.buf, kindMask, mk);
-  else bwdKernel<PPSG_MODE_BR><<<g, kBlock, 0, s>>>(dev(), l.buf, kindMask, mk);
+  const unsigned g = (unsigned)((l.count + kWavesPerBlock - 1) / kWavesPerBlock);
+  const int c = (int)l.count, mk = masked();
+  if (mode == PPSG_MODE_CFR) bwdKernel<PPSG_MODE_CFR><<<g, kBlock, 0, s>>>(dev(), l.buf, c, kindMask, mk);
+  else if (mode == PPSG_MODE_EV) bwdKernel<PPSG_MODE_EV><<<g, kBlock, 0, s>>>(dev(), l.buf, c, kindMask, mk);
+  else bwdKernel<PPSG_MODE_BR><<<g, kBlock, 0, s>>>(dev(), l.buf, c, kindMask, mk);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1880,8 +1933,8 @@ int setSchedule(ppsg_t* h, int slot, const float* coef, int64_t n) {
   if (!h->valid(slot)) return fail("no such spot");
   Spot& S = h->spots[slot];
   HIP_TRY(hipStreamSynchronize(h->stream));
-  if (n > S.coefCap) {
-    const int64_t cap = std::max<int64_t>(n, 2 * S.coefCap);
+  if (n + 1 > S.coefCap) {  // one more entry: zeros after the last index
+    const int64_t cap = std::max<int64_t>(n + 1, 2 * S.coefCap);
     void* p = nullptr;
     HIP_TRY(hipMalloc(&p, (size_t)cap * 3 * sizeof(float)));
     if (S.coefDev) (void)hipFree(S.coefDev);
@@ -1892,7 +1945,11 @@ int setSchedule(ppsg_t* h, int slot, const float* coef, int64_t n) {
     HIP_TRY(hipMemcpy(&h->ctl[slot].coef, &dp, sizeof(dp), hipMemcpyHostToDevice));
   }
   HIP_TRY(hipMemcpy(S.coefDev, coef, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(S.coefDev + 3 * n, 0, 3 * sizeof(float)));
   S.coefLen = n;
+  refreshCoef<<<1, 64, 0, h->stream>>>(h->ctl, slot, (long long)n);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(h->stream));
   return 0;
 }
 
@@ -1903,6 +1960,9 @@ int setCounter(ppsg_t* h, int slot, int64_t t) {
   const long long v = t;
   HIP_TRY(hipMemcpy(&h->ctl[slot].counter, &v, sizeof(v), hipMemcpyHostToDevice));
   h->spots[slot].counter = t;
+  refreshCoef<<<1, 64, 0, h->stream>>>(h->ctl, slot, (long long)h->spots[slot].coefLen);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(h->stream));
   return 0;
 }
 
