@@ -495,10 +495,20 @@ CompiledTree compileSpot(const PostflopSpot& spot, const pf::BetConfig& cfg, boo
     b = T.childBase[u];
     e = u + 1 < T.numNodes ? T.childBase[u + 1] : (int32_t)T.childFlat.size();
   };
+  // Rows are placed breadth-first (by depth, then node id): each depth's
+  // launches sweep one contiguous stretch of every table instead of rows
+  // spread between deeper subtrees.
+  std::vector<int32_t> byDepth(T.numNodes);
+  {
+    std::vector<int64_t> at((size_t)T.maxDepth + 2, 0);
+    for (int64_t u = 0; u < T.numNodes; ++u) ++at[(size_t)T.depth[u] + 1];
+    for (size_t k = 1; k < at.size(); ++k) at[k] += at[k - 1];
+    for (int64_t u = 0; u < T.numNodes; ++u) byDepth[at[(size_t)T.depth[u]]++] = (int32_t)u;
+  }
   T.cfvTotal = std::max(T.listSize(0, 0), T.listSize(1, 0));
   T.reachTotal[0] = T.listSize(1, 0);
   T.reachTotal[1] = T.listSize(0, 0);
-  for (int64_t u = 0; u < T.numNodes; ++u) {
+  for (const int32_t u : byDepth) {
     int32_t b, e;
     kidsOf(u, b, e);
     if (b == e) continue;
@@ -519,13 +529,15 @@ CompiledTree compileSpot(const PostflopSpot& spot, const pf::BetConfig& cfg, boo
       T.reachTotal[tr] += (int64_t)(e - b) * rs;
     }
   }
+  for (const int32_t u : byDepth) {
+    if (T.kind[u] != PPSG_DECIDE) continue;
+    T.regOff[u] = T.regTotal;
+    T.regTotal += (int64_t)T.na[u] * T.listSize(T.owner[u], u);
+  }
   for (int64_t u = 0; u < T.numNodes; ++u) {
-    const int n0 = T.listSize(0, u), n1 = T.listSize(1, u);
     if (T.kind[u] == PPSG_DECIDE) {
       T.decideOf[u] = (int32_t)T.decideNodes.size();
       T.decideNodes.push_back((int32_t)u);
-      T.regOff[u] = T.regTotal;
-      T.regTotal += (int64_t)T.na[u] * (T.owner[u] == 0 ? n0 : n1);
     } else if (T.kind[u] == PPSG_SHOWDOWN) {
       const int64_t gap = T.sc0[u] > T.sc1[u] ? T.sc0[u] - T.sc1[u] : T.sc1[u] - T.sc0[u];
       const float pot = (float)(T.potBase[u] + T.sc0[u] + T.sc1[u] - gap);

@@ -69,6 +69,7 @@ struct CompiledTree {
   std::vector<uint8_t> na;
   std::vector<int32_t> depth, cls, childBase, childFlat, parent;
   std::vector<int64_t> potBase, sc0, sc1, pc0, pc1;
+  // device row offsets (floats), placed breadth-first: by depth, then node id
   std::vector<int64_t> regOff, cfvOff, reachOff[2];
   std::vector<float> termVal;
   int64_t regTotal = 0, cfvTotal = 0, reachTotal[2] = {0, 0};

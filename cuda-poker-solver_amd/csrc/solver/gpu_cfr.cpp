@@ -70,6 +70,31 @@ struct GpuPostflopSolver::Impl {
   void write(int buf, int64_t off, int64_t n, const float* v) const {
     check(ppsg_batch_write(eng, slot, buf, off, n, v));
   }
+  // Whole regret / strategy tables in node order: each decide node's
+  // action-major block after the blocks of all lower node ids (the row
+  // layout of the reference's tables and of solution files). On the device
+  // the blocks sit breadth-first (T.regOff).
+  std::vector<float> readTable(int buf) const {
+    const std::vector<float> dev = read(buf, 0, T.regTotal);
+    std::vector<float> out((size_t)T.regTotal);
+    int64_t at = 0;
+    for (const int32_t u : T.decideNodes) {
+      const int64_t len = (int64_t)T.na[u] * T.listSize(T.owner[u], u);
+      std::copy(dev.begin() + T.regOff[u], dev.begin() + T.regOff[u] + len, out.begin() + at);
+      at += len;
+    }
+    return out;
+  }
+  void writeTable(int buf, const float* v) const {
+    std::vector<float> dev((size_t)T.regTotal);
+    int64_t at = 0;
+    for (const int32_t u : T.decideNodes) {
+      const int64_t len = (int64_t)T.na[u] * T.listSize(T.owner[u], u);
+      std::copy(v + at, v + at + len, dev.begin() + T.regOff[u]);
+      at += len;
+    }
+    write(buf, 0, T.regTotal, dev.data());
+  }
   // EV / BR walk of this spot alone; cfv (n[tr] floats) may be null.
   void walk(int tr, int mode, float* cfv) const {
     if (!attached) {
@@ -292,8 +317,8 @@ void GpuPostflopSolver::attach(ppsg_t* arena) {
   I->attached = true;
   check(ppsg_batch_set_active(arena, slot, 0));  // parked until a job runs it
   if (I->pendingIters >= 0) {  // a loaded solution's rows
-    I->write(PPSG_REGRET, 0, I->T.regTotal, I->pendingReg.data());
-    I->write(PPSG_STRAT, 0, I->T.regTotal, I->pendingStr.data());
+    I->writeTable(PPSG_REGRET, I->pendingReg.data());
+    I->writeTable(PPSG_STRAT, I->pendingStr.data());
     I->setCounter(I->pendingIters);
     I->pendingReg = std::vector<float>();
     I->pendingStr = std::vector<float>();
@@ -672,13 +697,13 @@ double GpuPostflopSolver::iterationBytes() const { return impl_->T.iterationByte
 
 std::vector<float> GpuPostflopSolver::rows(int which) const {
   if (impl_->T.empty) return {};
-  return impl_->read(which == 0 ? PPSG_REGRET : PPSG_STRAT, 0, impl_->T.regTotal);
+  return impl_->readTable(which == 0 ? PPSG_REGRET : PPSG_STRAT);
 }
 
 void GpuPostflopSolver::setRows(int which, const std::vector<float>& v) {
   if (impl_->T.empty) return;
   if ((int64_t)v.size() != impl_->T.regTotal) throw std::runtime_error("row size mismatch");
-  impl_->write(which == 0 ? PPSG_REGRET : PPSG_STRAT, 0, (int64_t)v.size(), v.data());
+  impl_->writeTable(which == 0 ? PPSG_REGRET : PPSG_STRAT, v.data());
 }
 
 void GpuPostflopSolver::halfStep(int tr, int t, const std::string& algo, int planned) {
@@ -818,8 +843,8 @@ void GpuPostflopSolver::save(const std::string& path) const {
     w.val<int32_t>((int32_t)I->lastAlgo.size());
     w.raw(I->lastAlgo.data(), I->lastAlgo.size());
     w.val<int64_t>(I->T.regTotal);
-    const std::vector<float> reg = I->read(PPSG_REGRET, 0, I->T.regTotal);
-    const std::vector<float> str = I->read(PPSG_STRAT, 0, I->T.regTotal);
+    const std::vector<float> reg = I->readTable(PPSG_REGRET);
+    const std::vector<float> str = I->readTable(PPSG_STRAT);
     w.raw(reg.data(), reg.size() * 4);
     w.raw(str.data(), str.size() * 4);
     if (std::fflush(f) != 0) throw std::runtime_error("write failed: " + path);
@@ -900,8 +925,8 @@ void GpuPostflopSolver::loadFrom(const std::string& solutionPath, bool device) {
     r.raw(reg.data(), reg.size() * 4);
     r.raw(str.data(), str.size() * 4);
     if (device) {
-      I->write(PPSG_REGRET, 0, regTotal, reg.data());
-      I->write(PPSG_STRAT, 0, regTotal, str.data());
+      I->writeTable(PPSG_REGRET, reg.data());
+      I->writeTable(PPSG_STRAT, str.data());
       I->setCounter(savedIters);
     } else {
       I->pendingReg = std::move(reg);

@@ -119,6 +119,48 @@ def test_save_load(solved):
             pps.Solver.load(path)
 
 
+def read_solution_rows(path):
+    """The regret and strategy-sum tables of a PPS1 v1 file (layout of
+    cuda/gpu_cfr.cu:2686-2853): header fields, then regTotal f32 each."""
+    import struct
+    with open(path, "rb") as f:
+        b = f.read()
+    at = 4 + 4 + 4 + 5 + 8 + 8 + 2 * 1326 * 8
+    for _ in range(2):  # bet fractions, raise multiples
+        (n,) = struct.unpack_from("<i", b, at)
+        at += 4 + 8 * n
+    at += 1 + 1 + 4 + 3 * 8 + 8 + 8
+    (alen,) = struct.unpack_from("<i", b, at)
+    at += 4 + alen
+    (total,) = struct.unpack_from("<q", b, at)
+    at += 8
+    reg = np.frombuffer(b, dtype="<f4", count=total, offset=at)
+    strat = np.frombuffer(b, dtype="<f4", count=total, offset=at + 4 * total)
+    assert at + 8 * total == len(b)
+    return reg, strat
+
+
+def test_solution_file_rows_in_node_order(solved):
+    """Solution files hold each decide node's action-major block in node
+    order (the reference's table layout), whatever the device layout: the
+    strategy sums of decide node i, found at the sum of the earlier nodes'
+    block sizes, normalise to strategy(i)."""
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "spot.sol")
+        solved.save(path)
+        reg, strat = read_solution_rows(path)
+    assert len(reg) == len(strat)
+    at = 0
+    for i, info in enumerate(solved.decide_nodes()):
+        na, nc = len(info["actions"]), info["n_combos"]
+        blk = strat[at:at + na * nc].astype(float).reshape(na, nc)
+        at += na * nc
+        s = blk.sum(axis=0)
+        want = np.where(s > 0, blk / np.where(s > 0, s, 1), 1.0 / na)
+        close(want, np.asarray(solved.strategy(i)["freqs"]), 1e-6, f"decide node {i}")
+    assert at == len(strat)
+
+
 def test_errors(solved):
     with pytest.raises((ValueError, RuntimeError)):
         pps.Solver(board="Qs9h", oop=OOP, ip=IP, pot=POT, stack=STACK)
