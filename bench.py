@@ -273,6 +273,12 @@ def bench_batched(args, capi, configs):
     prof = capi.batch_profile(S, C, args.profile_iters)
     log(f"bench: profile {json.dumps(prof)}")
     roof = roofline_batch(prof)
+    # a plain copy on this GPU (4 GiB each way, one float4 per lane): the
+    # streaming rate the HBM-bound kernels compare against
+    copy_bytes = 4 << 30
+    copy_ms = capi.copy_probe(copy_bytes, 5)
+    roof["copy_gbs"] = round(2 * copy_bytes / (copy_ms * 1e-3) / 1e9, 1)
+    roof["frac_of_copy"] = round(roof["achieved"] / roof["copy_gbs"], 4)
 
     steps = rep["steps"] * CHECK_EVERY
     line = {
