@@ -30,6 +30,12 @@ TINY_SPOTS = {
                   bets=[0.5, 1.0], raises=[2.5], allin=True),
     "turn": dict(board="Qs9h2d7c", oop=["AhKh", "QhQd", "JhTc"], ip=["AhKh", "KcQc", "8h8d"],
                  pot=60, stack=100, bets=[0.75], raises=[], allin=True),
+    # Two chance levels (turn deal, then river deal): 31,124 nodes over
+    # 2,352 river boards. Check / all-in on every street keeps the brute
+    # force affordable while every betting line still reaches both deals.
+    "flop": dict(board="Qs9h2d", oop=["AhKh", "QhQd", "JhTc", "9c8c"],
+                 ip=["AhKh", "KcQc", "8h8d", "Ts9s"], pot=60, stack=30, bets=[], raises=[],
+                 allin=True),
 }
 
 

@@ -186,7 +186,8 @@ def _tiny(name):
     return spots.tiny(name)
 
 
-@pytest.mark.parametrize("name,steps", [("river", [0, 1, 2, 7, 30]), ("turn", [0, 1, 4, 15])])
+@pytest.mark.parametrize("name,steps", [("river", [0, 1, 2, 7, 30]), ("turn", [0, 1, 4, 15]),
+                                        ("flop", [0, 3])])
 def test_half_steps_match_per_pair_brute_force(name, steps):
     sp = _tiny(name)
     o = orc.Oracle(sp)
@@ -205,7 +206,7 @@ def test_half_steps_match_per_pair_brute_force(name, steps):
                 assert err < 2e-4, (name, t, tr, which, err)
 
 
-@pytest.mark.parametrize("name", ["river", "turn"])
+@pytest.mark.parametrize("name", ["river", "turn", "flop"])
 def test_walks_match_per_pair_brute_force(name):
     sp = _tiny(name)
     o = orc.Oracle(sp)
@@ -245,3 +246,48 @@ def test_exploitability_decreases_with_iterations():
     e300 = o.stats()["exploitability"]
     assert e300 < 0.2 * e10
     assert math.isfinite(e300)
+
+
+def _aggregate(bf, u, strat):
+    """Reach-free strategy aggregate of decide node u: each combo's strategy
+    sums normalised (uniform when they are all zero), averaged with the
+    acting player's range weights (cuda/gpu_cfr.cu:2109-2150)."""
+    n = bf.nodes[u]
+    p = n["player"]
+    w = np.array([bf.base[p][k][2] for k in n["lists"][p]])
+    tot = strat.sum(axis=0)
+    na = strat.shape[0]
+    sigma = np.where(tot > 0, strat / np.where(tot > 0, tot, 1.0), 1.0 / na)
+    return (sigma * w).sum(axis=1) / w.sum()
+
+
+@pytest.mark.parametrize("name", ["river", "turn"])
+def test_strategy_aggregates_equal_numpy_restatement(name):
+    """orc_root_strategy / orc_node_strategy against an aggregate computed
+    here from the brute force's node lists and base weights, on strategy
+    rows with all-zero combos (the uniform case) and uneven magnitudes."""
+    sp = _tiny(name)
+    o = orc.Oracle(sp)
+    bf = BruteForce(sp)
+    rng = np.random.default_rng(7)
+    flat = rng.uniform(0.0, 5.0, o.counts()["reg_total"]).astype(np.float32)
+    flat[rng.uniform(size=flat.shape) < 0.2] = 0.0
+    o.set_rows(1, flat)
+    bf.set_rows(1, flat)
+    for idx in (0, 1):
+        u = bf.decide[idx]
+        strat = np.array(bf.nodes[u]["strat"], dtype=np.float64)
+        strat[:, ::3] = 0.0  # every third combo: no strategy mass at all
+        # write the zeroed columns back through the flat rows
+        off = sum(bf.nodes[v]["strat"].size for v in bf.decide[:idx])
+        flat[off:off + strat.size] = strat.astype(np.float32).ravel()
+        o.set_rows(1, flat)
+        want = _aggregate(bf, u, strat)
+        got = o.root_strategy() if idx == 0 else o.node_strategy(idx)
+        assert np.allclose(got, want, rtol=0, atol=1e-12), (idx, got, want)
+        assert abs(sum(got) - 1.0) < 1e-12
+    # decide nodes past the first street have no aggregate (chance-compacted
+    # lists), as the reference returns none for them
+    later = [i for i, u in enumerate(bf.decide) if len(bf.nodes[u]["board"]) > sp.n_board]
+    if later:
+        assert o.node_strategy(later[0]) == []
