@@ -5,14 +5,19 @@ served-stream latency, every spot solved to exploitability <= 0.5% of pot.
 Default workload (batched turn): 256 seeded turn spots (seeds 1000..1255,
 pot 20-200, SPR 0.5-8, 1-3 sizes from {33,50,75,100,150}% + all-in, raise
 cap 1-3, 30-100% ranges with weights in (0, 1], trees of 2k-60k nodes;
-reference DCFR schedule, f32 rows). A step is one DCFR iteration (both
-alternating half-steps) of one spot; spots are checked every 32 iterations
-and retire at exploitability <= 0.005 * pot.
+reference DCFR schedule, f32 rows). Spots are checked every 32 iterations
+and retire at exploitability <= 0.005 * pot; every spot runs to that target
+(a per-spot cap of 4096 iterations is far above what any spot needs), and
+the line reports how many converged.
 
   value           batched: all 256 spots resident in one arena, advanced
                   together (depth d of every live spot in one launch);
                   sum of per-spot iterations / wall time of the steps,
                   checks and retirements (inputs already in HBM)
+  steps           --steps K batched iterations of the full resident batch
+                  (every spot advanced one DCFR iteration per step; graph
+                  replays timed with HIP events on the engine stream after
+                  --warmup W untimed ones); ms_per_step is their mean
   single          the same spots one at a time through the same engine
                   (a batch of one), same definition of time
   serving         a Poisson stream (1000 requests at 40/s, 70% turn spots as
@@ -26,6 +31,10 @@ and retire at exploitability <= 0.005 * pot.
   cpu_baseline    the CPU oracle (the tests' checker, a C++ OpenMP port of
                   the same algorithm) on the first spots of the stream, each
                   run for the iteration count the GPU needed, ~15 s
+
+  single_spot     the plain turn, large turn and large flop spots, each
+                  alone (500 / 2000 / --flop-iters iterations from the
+                  captured graph), with their own roofline
 
 --workload large-turn / plain-turn / large-flop measures one spot alone
 (2000 / 500 / 1000 iterations from the captured graph) instead.
@@ -43,6 +52,8 @@ METRIC = "aggregate DCFR iters/s + p50/p99 solve latency @ expl≤0.5% pot, batc
 HBM_PEAK_GBS = 8000.0
 TARGET = 0.005
 CHECK_EVERY = 32
+MAX_ITERS = 4096  # per-spot cap of the at-target runs; spots retire at the target far below it
+PROFILES = os.path.join(ROOT, "profiles")
 
 
 def log(msg):
@@ -173,6 +184,24 @@ def roofline_batch(prof):
             "mean_launch_us": d["mean_launch_us"], "kernels": per}
 
 
+def counter_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE runs, each scaled
+    by a plain copy's reading): a line `traffic <kernel> <workload>: <bytes>
+    bytes per launch` under profiles/. None when no summary covers it."""
+    import re
+    pat = re.compile(r"^traffic (\S+) (\S+): (\d+) bytes per launch")
+    for name in sorted(os.listdir(PROFILES)) if os.path.isdir(PROFILES) else []:
+        if not name.endswith((".md", ".txt")):
+            continue
+        with open(os.path.join(PROFILES, name)) as f:
+            for line in f:
+                m = pat.match(line.strip().strip("`"))
+                if m and m.group(1) == kernel and m.group(2) == workload:
+                    return int(m.group(3))
+    return None
+
+
 def roofline_of(prof, iters_key="iters"):
     """Single spot: forward vs backward pass, algorithmic bytes per launch /
     mean launch time."""
@@ -220,7 +249,7 @@ def bench_batched(args, capi, configs):
     S, C = [p[0] for p in pairs], [p[1] for p in pairs]
     log(f"bench: {len(spots)} batched-turn spots, {sum(s['nodes'] for s in spots)} nodes in "
         f"total, generated in {time.perf_counter() - t0:.1f}s")
-    opt = dict(target_frac=TARGET, check_every=CHECK_EVERY, max_iters=args.steps, real_time=0,
+    opt = dict(target_frac=TARGET, check_every=CHECK_EVERY, max_iters=MAX_ITERS, real_time=0,
                prefill=1)
     if args.warmup > 0:  # code objects, allocator, graph machinery
         capi.serve(S[:8], C[:8], "dcfr", None, **dict(opt, target_frac=0.0, check_every=args.warmup,
@@ -230,6 +259,8 @@ def bench_batched(args, capi, configs):
         log(f"bench: profile {json.dumps(prof)}")
         print(json.dumps({"profile": prof, "roofline": roofline_batch(prof)}), flush=True)
         return 0
+    if args.steps <= 0:
+        raise SystemExit("--steps must be positive")
 
     res, rep = capi.serve(S, C, "dcfr", None, **opt)
     bad = [r["error"] for r in res if not r["ok"]]
@@ -268,11 +299,14 @@ def bench_batched(args, capi, configs):
 
     http = None
     if args.http_spots > 0:
-        http = bench_http(spots[:args.http_spots], args.http_clients, args.steps)
+        http = bench_http(spots[:args.http_spots], args.http_clients, MAX_ITERS)
 
-    prof = capi.batch_profile(S, C, args.profile_iters)
+    # the timed steps: K batched iterations of the full resident batch after
+    # W untimed ones, then per-launch timing over K more
+    prof = capi.batch_profile(S, C, args.steps, warmup=args.warmup)
     log(f"bench: profile {json.dumps(prof)}")
     roof = roofline_batch(prof)
+    roof["traffic"] = counter_traffic(roof["kernel"], f"batched-turn-{len(spots)}")
     # a plain copy on this GPU (4 GiB each way, one float4 per lane): the
     # streaming rate the HBM-bound kernels compare against
     copy_bytes = 4 << 30
@@ -280,30 +314,46 @@ def bench_batched(args, capi, configs):
     roof["copy_gbs"] = round(2 * copy_bytes / (copy_ms * 1e-3) / 1e9, 1)
     roof["frac_of_copy"] = round(roof["achieved"] / roof["copy_gbs"], 4)
 
-    steps = rep["steps"] * CHECK_EVERY
+    single_spot = None
+    if not args.no_single_spot:
+        single_spot = {w: single_spot_record(capi, configs, w, n, args.warmup)
+                       for w, n in (("plain-turn", 500), ("large-turn", 2000),
+                                    ("large-flop", args.flop_iters))}
+
+    n_conv = {k: (v["converged"], v["spots"]) for k, v in
+              (("batched", batched), ("single", single), ("serving", serving)) if v}
     line = {
         "metric": METRIC,
         "value": round(value, 3),
         "unit": "iters/s",
         "n_gpus": 1,
-        "steps": steps,
+        "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(rep["busy_ms"] / steps, 4),
+        "ms_per_step": round(prof["graph_ms_per_iter"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
+        "converged": {k: f"{c}/{n}" for k, (c, n) in n_conv.items()},
+        "all_converged": all(c == n for c, n in n_conv.values()),
         "config": {
             "workload": f"batched-turn-{len(spots)}",
+            "value_is": "at-target run: every spot solved to exploitability <= 0.5% pot; sum of "
+                        "per-spot iterations / wall time of its device steps, checks and "
+                        "retirements. steps / ms_per_step: the full resident batch (all spots "
+                        "live) replayed K times from the captured graph",
             "spots": len(spots), "target_expl_frac_pot": TARGET, "check_every": CHECK_EVERY,
-            "max_iters": args.steps, "nodes_total": sum(s["nodes"] for s in spots),
+            "max_iters": MAX_ITERS, "nodes_total": sum(s["nodes"] for s in spots),
+            "at_target_batched_iterations": rep["steps"] * CHECK_EVERY,
             "batched": batched,
             "single": single,
             "batched_vs_single": round(value / single["iters_per_s_busy"], 3) if single else None,
             "serving": serving,
             "http": http,
+            "single_spot": single_spot,
             "resident_batch": {
+                "steps": args.steps, "warmup": args.warmup,
                 "graph_ms_per_batched_iter": round(prof["graph_ms_per_iter"], 4),
                 "iters_per_s": round(prof["spots"] / (prof["graph_ms_per_iter"] * 1e-3), 1),
                 "alg_bytes_per_batched_iter": prof["fwd_bytes"] + prof["bwd_bytes"],
@@ -391,6 +441,39 @@ def bench_http(spots, clients, max_iters):
     return out
 
 
+def single_spot_record(capi, configs, workload, iters, warmup):
+    """One spot alone: `iters` iterations replayed from the captured graph
+    after `warmup` (graph capture, caches), with the forward/backward
+    roofline of its launches."""
+    spot = {"large-turn": configs.large_turn, "plain-turn": configs.plain_turn,
+            "large-flop": configs.large_flop}[workload]()
+    sp, cf = spot_cfg(capi, spot)
+    t0 = time.perf_counter()
+    solver = capi.Solver(sp, cf)
+    setup_s = time.perf_counter() - t0
+    solver.solve(max(1, warmup), spot["algo"])
+    t0 = time.perf_counter()
+    solver.continue_solve(iters)
+    wall = time.perf_counter() - t0
+    st = solver.stats()
+    info = solver.engine_info()
+    prof = solver.profile(8)
+    prof.update(fwd_bytes=info["fwd_bytes"], bwd_bytes=info["bwd_bytes"])
+    rec = {"nodes": solver.num_nodes, "iterations": iters,
+           "iters_per_s": round(iters / wall, 2), "ms_per_iter": round(wall * 1e3 / iters, 4),
+           "device_ms_per_iter": round(solver.last_solve_ms / iters, 4),
+           "alg_bytes_per_iter": solver.iteration_bytes,
+           "kernels_per_iter": info["kernels_per_iter"],
+           "compile_upload_s": round(setup_s, 3),
+           "device_gb": round(info["device_bytes"] / 1e9, 3),
+           "exploitability_pct_pot": round(100 * st["exploitability"] / spot["pot"], 4),
+           "roofline": roofline_of(prof)}
+    rec["roofline"]["traffic"] = counter_traffic(rec["roofline"]["kernel"].split()[0], workload)
+    solver.close()
+    log(f"bench: single spot {workload} {json.dumps(rec)}")
+    return rec
+
+
 def bench_single(args, capi, configs):
     spot = {"large-turn": configs.large_turn, "plain-turn": configs.plain_turn,
             "large-flop": configs.large_flop}[args.workload]()
@@ -399,7 +482,7 @@ def bench_single(args, capi, configs):
     solver = capi.Solver(sp, cf)
     log(f"bench: {spot['name']} nodes={solver.num_nodes} compile+upload "
         f"{time.perf_counter() - t0:.2f}s")
-    steps = args.steps if args.steps != 4096 else spot["iters"]
+    steps = args.steps if args.steps_given else spot["iters"]
     solver.solve(args.warmup, spot["algo"])  # graph capture + warm caches
     t0 = time.perf_counter()
     solver.continue_solve(steps)
@@ -437,15 +520,20 @@ def bench_single(args, capi, configs):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4096,
-                    help="per-spot iteration cap (batched) / timed iterations (single spot)")
-    ap.add_argument("--warmup", type=int, default=64)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed batched iterations of the full resident batch (batched) / timed "
+                         "iterations (single spot; default: the workload's own count)")
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="batched-turn",
                     choices=["batched-turn", "large-turn", "plain-turn", "large-flop"])
     ap.add_argument("--spots", type=int, default=256)
     ap.add_argument("--serve-spots", type=int, default=1000)
     ap.add_argument("--qps", type=float, default=40.0)
     ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--no-single-spot", action="store_true",
+                    help="skip the plain turn / large turn / large flop records")
+    ap.add_argument("--flop-iters", type=int, default=200,
+                    help="timed iterations of the large flop record")
     ap.add_argument("--http-spots", type=int, default=64,
                     help="spots of the batched stream sent through the HTTP API (0: skip)")
     ap.add_argument("--http-clients", type=int, default=16)
@@ -460,6 +548,7 @@ def main():
                     help="only the CPU oracle at 1 thread and all cores on the plain and "
                          "large turn spots (no GPU)")
     args = ap.parse_args()
+    args.steps_given = any(a == "--steps" or a.startswith("--steps=") for a in sys.argv[1:])
     if args.cpu_table:
         from pps import configs
         print(json.dumps({"cpu_table": cpu_table(configs, args.cpu_seconds)}), flush=True)

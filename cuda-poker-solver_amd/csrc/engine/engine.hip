@@ -278,11 +278,16 @@ __device__ __forceinline__ void fwdItem(const Dev& d, const Item& it, int fromSt
 // ---------------------------------------------------------------------------
 
 // Loads node g's opponent reach row (ROWS floats per lane, NT lanes) when
-// g < cnt.
+// g < cnt; zeros when the class has no opponent combos (nO == 0).
 template <int ROWS, int NT = kBlock>
 __device__ __forceinline__ void loadReach(float (&rv)[ROWS], const char* base,
                                           const TermNode* nodes, int g, int cnt, int nO) {
   if (g >= cnt) return;
+  if (nO <= 0) {  // a dealt card blocks the whole opposing list: no mass
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q) rv[q] = 0.0f;
+    return;
+  }
   const float* __restrict__ r = reinterpret_cast<const float*>(base + nodes[g].reach);
   // unconditional loads (index clamped into the row; callers stage only
   // j < nO): no register is written while a load into it is in flight
@@ -740,9 +745,14 @@ __device__ __forceinline__ void decideTile(const Dev& d, const Item& it) {
 }
 
 // Chance tile, one lane per traverser (parent) combo: sum of the child
-// values over the branches the combo survives, in branch order. Branches
-// go in chunks: all map loads of a chunk, then all value loads, then the
-// adds (keeps many loads in flight; same summation order).
+// values over the branches the combo survives, in branch order, accumulated
+// in f64 and rounded to f32 once (postflop-solver's chance sum,
+// solver.rs:191-199). Combos that a suit swap maps onto each other get the
+// same terms in different orders; the f64 sum makes their values equal
+// bit for bit, which a suit-isomorphic tree relies on (f32 rounding
+// differences there grow under regret matching and break the symmetry the
+// deal tables assume). Branches go in chunks: all map loads of a chunk,
+// then all value loads, then the adds (keeps many loads in flight).
 __device__ __forceinline__ void chanceTile(const Dev& d, const Item& it) {
   const int i = it.tile * kWave + waveLane();
   const int n = it.n;
@@ -755,7 +765,7 @@ __device__ __forceinline__ void chanceTile(const Dev& d, const Item& it) {
   const uint32_t stride = (uint32_t)it.m;
   const int nb = it.na;
   constexpr int kChunk = 16;
-  float v = 0.0f;
+  double v = 0.0;
   for (int b0 = 0; b0 < nb; b0 += kChunk) {
     int slot[kChunk];
     float x[kChunk];
@@ -771,9 +781,9 @@ __device__ __forceinline__ void chanceTile(const Dev& d, const Item& it) {
                  : 0.0f;
 #pragma unroll
     for (int k = 0; k < kChunk; ++k)
-      if (slot[k] >= 0) v += x[k];
+      if (slot[k] >= 0) v += (double)x[k];
   }
-  at<float>(d, it.b)[i] = v;
+  at<float>(d, it.b)[i] = (float)v;
 }
 
 template <int MODE>
@@ -2232,6 +2242,33 @@ int ppsg_profile(ppsg_t* h, int iters, double* out, int cap) {
   // cap 13: terminal launches attributed; cap 17: also decide / chance
   // re-runs of the forward and backward launches
   const bool detail = cap >= 17;
+  // The backward re-runs apply the regret and strategy update a second
+  // time, so in detail mode every live spot's regret and strategy tables
+  // and its schedule counter are saved first and put back afterwards: the
+  // profiled iterations leave no trace in the rows.
+  char* snap = nullptr;
+  std::vector<int64_t> savedCounter;
+  if (detail) {
+    size_t need = 0;
+    for (int s : h->live) need += 8 * (size_t)h->spots[s].regTotal;
+    if (hipMalloc(&snap, std::max<size_t>(need, 256)) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+      return fail("profile: no device memory to save the rows of a detailed profile");
+    }
+    size_t at = 0;
+    for (int s : h->live) {
+      const Spot& S = h->spots[s];
+      const size_t n = 4 * (size_t)S.regTotal;
+      HIP_TRY(hipMemcpyAsync(snap + at, h->arena.base + S.off + S.L.regret, n,
+                             hipMemcpyDeviceToDevice, h->stream));
+      HIP_TRY(hipMemcpyAsync(snap + at + n, h->arena.base + S.off + S.L.strat, n,
+                             hipMemcpyDeviceToDevice, h->stream));
+      at += 2 * n;
+      savedCounter.push_back(S.counter);
+    }
+  }
   const int D = h->depthCount();
   int rc = 0;
   for (int it = 0; it < iters && rc == 0; ++it) {
@@ -2274,9 +2311,30 @@ int ppsg_profile(ppsg_t* h, int iters, double* out, int cap) {
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
+  if (snap) {
+    size_t at = 0;
+    for (int s : h->live) {
+      const Spot& S = h->spots[s];
+      const size_t n = 4 * (size_t)S.regTotal;
+      if (hipMemcpyAsync(h->arena.base + S.off + S.L.regret, snap + at, n,
+                         hipMemcpyDeviceToDevice, h->stream) != hipSuccess ||
+          hipMemcpyAsync(h->arena.base + S.off + S.L.strat, snap + at + n, n,
+                         hipMemcpyDeviceToDevice, h->stream) != hipSuccess)
+        rc = rc ? rc : fail("profile: restoring the saved rows failed");
+      at += 2 * n;
+    }
+    if (hipStreamSynchronize(h->stream) != hipSuccess && rc == 0)
+      rc = fail("profile: restoring the saved rows failed");
+    (void)hipFree(snap);
+    if (rc != 0) return rc;
+    size_t k = 0;
+    for (int s : h->live)
+      if (setCounter(h, s, savedCounter[k++]) != 0) return -1;
+  }
   if (rc != 0) return rc;
   HIP_TRY(hipStreamSynchronize(h->stream));
-  for (int s : h->live) h->spots[s].counter += iters;
+  if (!detail)
+    for (int s : h->live) h->spots[s].counter += iters;
   acc[4] = iters;
   for (int i = 0; i < cap && i < 17; ++i) out[i] = acc[i];
   return 0;

@@ -352,11 +352,13 @@ std::vector<SpotResult> serve(const std::vector<SpotRequest>& reqs, const ServeO
 }
 
 BatchProfile profileBatch(const std::vector<SpotRequest>& reqs, const std::string& algo, int iters,
-                          int64_t arenaBytes, bool detail) {
+                          int64_t arenaBytes, bool detail, int warmup) {
   if (algo != "dcfr" && algo != "dcfr-g2")
     throw std::runtime_error("profile: algo must be dcfr or dcfr-g2");
   if (iters <= 0 || reqs.empty()) throw std::runtime_error("profile: nothing to run");
-  const int tab = (detail ? 3 : 2) * iters;  // replay + launch timing (+ attribution)
+  if (warmup < 0) throw std::runtime_error("profile: negative warmup");
+  // warmup + replay + launch timing (+ attribution)
+  const int tab = warmup + (detail ? 3 : 2) * iters;
   std::vector<float> coef((size_t)tab * 3);
   for (int t = 0; t < tab; ++t) {
     double c[3];
@@ -401,6 +403,7 @@ BatchProfile profileBatch(const std::vector<SpotRequest>& reqs, const std::strin
   P.tuneMs.assign(kNumGroupCandidates, 0.0);
   check(ppsg_autotune(E.h, kGroupCandidates, kNumGroupCandidates, 2, &P.groupSize, P.tuneMs.data()),
         "profile: autotune");
+  if (warmup > 0) check(ppsg_batch_step(E.h, warmup), "profile: warmup");
   check(ppsg_batch_step(E.h, iters), "profile: replay");
   double ms = 0.0;
   check(ppsg_last_iterate_ms(E.h, &ms), "profile: replay time");

@@ -547,7 +547,7 @@ int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const do
 }
 
 int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, const char* algo,
-                      int iters, int64_t arena_bytes, int detail, double out[33]) {
+                      int iters, int warmup, int64_t arena_bytes, int detail, double out[33]) {
   return guard([&] {
     if (n <= 0 || !spots || !cfgs || !out) throw std::runtime_error("null argument");
     std::vector<pps::gpu::SpotRequest> reqs(n);
@@ -556,7 +556,8 @@ int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, 
       reqs[i].cfg = toCfg(cfgs + i);
     }
     const pps::gpu::BatchProfile P =
-        pps::gpu::profileBatch(reqs, algo ? algo : "dcfr", iters, arena_bytes, detail != 0);
+        pps::gpu::profileBatch(reqs, algo ? algo : "dcfr", iters, arena_bytes, detail != 0,
+                               warmup);
     const double v[16] = {(double)P.spots, (double)P.iters, P.graphMsPerIter, P.fwdMs,
                           P.fwdLaunches, P.bwdMs, P.bwdLaunches, P.fwdBytes, P.bwdBytes,
                           (double)P.kernelsPerIter, (double)P.itemsPerIter, (double)P.arenaBytes,

@@ -729,7 +729,9 @@ std::vector<double> GpuPostflopSolver::profileIterations(int iters, bool detail)
   I->setSchedule(scheduleTable(t0, tEnd, (int)tEnd, I->lastAlgo), tEnd);
   std::vector<double> out(detail ? 17 : 13, 0.0);
   check(ppsg_profile(I->eng, iters, out.data(), (int)out.size()));
-  totalIters_ = tEnd;
+  // detail mode puts the rows and the schedule counter back: no iteration
+  // is added
+  if (!detail) totalIters_ = tEnd;
   return out;
 }
 

@@ -150,7 +150,7 @@ def lib():
                                 ctypes.c_char_p, P(ServeOptions), P(ServeResult), P(ServeReport)]
         L.pps_compile_footprint.argtypes = [P(Spot), P(BetConfig), P(ctypes.c_int64)]
         L.pps_batch_profile.argtypes = [ctypes.c_int, P(Spot), P(BetConfig), ctypes.c_char_p,
-                                        ctypes.c_int, ctypes.c_int64, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int,
                                         P(ctypes.c_double)]
         L.pps_server_create.argtypes = [P(ServerOptions), P(ctypes.c_void_p)]
         L.pps_server_destroy.restype = None
@@ -515,13 +515,14 @@ def serve(spots, cfgs, algo="dcfr", arrival_ms=None, **options):
     return out, report
 
 
-def batch_profile(spots, cfgs, iters, algo="dcfr", arena_bytes=0, detail=False):
+def batch_profile(spots, cfgs, iters, algo="dcfr", arena_bytes=0, detail=False, warmup=0):
     """Launch-level timing of the spots resident in one arena (pps_batch_profile)."""
     n = len(spots)
     S = (Spot * n)(*spots)
     C = (BetConfig * n)(*cfgs)
     out = (ctypes.c_double * 33)()
-    _check(lib().pps_batch_profile(n, S, C, algo.encode(), int(iters), int(arena_bytes),
+    _check(lib().pps_batch_profile(n, S, C, algo.encode(), int(iters), int(warmup),
+                                   int(arena_bytes),
                                    1 if detail else 0, out))
     keys = ["spots", "iters", "graph_ms_per_iter", "fwd_ms", "fwd_launches", "bwd_ms",
             "bwd_launches", "fwd_bytes", "bwd_bytes", "kernels_per_iter", "items_per_iter",

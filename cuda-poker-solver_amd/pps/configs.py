@@ -101,18 +101,34 @@ def batch_turn(seed):
     return s
 
 
-def river_spot(seed):
-    """Serving-stream river spot: 5-card board, 1-4 sizes + all-in, cap 1-4."""
+RAISE_SIZES = [2.0, 2.5, 3.0, 4.0]
+
+
+def _river_geometry(seed):
     rng = XorShift64Star(seed * 104729 + 3)
     pot = 20 + rng.next_below(181)
-    spr = 0.5 + 7.5 * rng.next_double()
+    spr = 1.0 + 31.0 * rng.next_double()
     stack = max(1, int(round(pot * spr)))
     nsz = 1 + rng.next_below(4)
     pool = list(SIZES)
     sizes = sorted(pool.pop(rng.next_below(len(pool))) for _ in range(nsz))
     cap = 1 + rng.next_below(4)
+    nr = 1 + rng.next_below(4)
+    pool = list(RAISE_SIZES)
+    raises = sorted(pool.pop(rng.next_below(len(pool))) for _ in range(nr))
     frac = 0.3 + 0.7 * rng.next_double()
-    s = random_spot(seed, 5, frac, pot, stack, sizes, [2.5], True, cap)
+    return pot, stack, sizes, raises, cap, frac
+
+
+def river_spot(seed):
+    """Serving-stream river spot: 5-card board, 1-4 bet sizes from
+    {33, 50, 75, 100, 150}% + all-in, 1-4 raise sizes from {2, 2.5, 3, 4}x
+    the previous bet, raise cap 1-4, 30-100% ranges. Stacks run deeper than
+    the turn stream's (SPR 1-32): a river tree only reaches hundreds of
+    nodes with several raise sizes, a cap of 3-4 and room behind, and
+    river_sized keeps the draws of 500-20k nodes."""
+    pot, stack, sizes, raises, cap, frac = _river_geometry(seed)
+    s = random_spot(seed, 5, frac, pot, stack, sizes, raises, True, cap)
     s.update(name=f"river-{seed}", algo="dcfr")
     return s
 
@@ -127,10 +143,10 @@ def _nodes(spot):
 RESAMPLE_STEP = 100003
 
 
-def _sized(make, seed, lo, hi):
+def _sized(make, seed, lo, hi, tries=64):
     """make(seed) for the first of seed, seed + RESAMPLE_STEP, ... whose tree
     has lo..hi nodes (the size filter of the batched and served streams)."""
-    for k in range(64):
+    for k in range(tries):
         s = make(seed + k * RESAMPLE_STEP)
         n = _nodes(s)
         if lo <= n <= hi:
@@ -146,6 +162,27 @@ def _parallel(fn, args, threads=16):
         return list(ex.map(fn, args))
 
 
+def river_sized(seed):
+    """river_spot from the first of seed, seed + RESAMPLE_STEP, ... whose
+    tree has 500-20k nodes (about one draw in 16 qualifies). Tree size
+    depends on the betting geometry only, so each draw is sized on a fixed
+    board with one combo a side and only the accepted one draws its ranges."""
+    from . import capi
+    one0, one1 = [0.0] * 1326, [0.0] * 1326
+    one0[COMBOS.index((20, 21))] = 1.0
+    one1[COMBOS.index((30, 31))] = 1.0
+    for k in range(512):
+        sd = seed + k * RESAMPLE_STEP
+        pot, stack, sizes, raises, cap, _ = _river_geometry(sd)
+        n = capi.compile_info(capi.make_spot([0, 1, 2, 3, 4], one0, one1, pot, stack),
+                              capi.make_config(sizes, raises, True, False, cap))["nodes"]
+        if 500 <= n <= 20000:
+            s = river_spot(sd)
+            s["nodes"] = n
+            return s
+    raise RuntimeError(f"no river spot with 500..20000 nodes from seed {seed}")
+
+
 def batch_stream(n=256, first_seed=1000):
     """The batched-turn workload: batch_turn spots from seeds first_seed ..
     first_seed + n - 1, each resampled until its tree has 2k-60k nodes."""
@@ -156,9 +193,8 @@ def batch_stream(n=256, first_seed=1000):
 def serving_stream(n, qps, seed=7):
     """The serving workload: n requests with Poisson arrivals at `qps` per
     second (exponential gaps from one seeded stream), 70% batched-turn
-    spots (2k-60k nodes) and 30% river spots. River trees with these bet
-    menus have 9-105 nodes, so they are taken as drawn (no size filter).
-    Each spot dict gains "arrival_ms"."""
+    spots (2k-60k nodes) and 30% river spots (500-20k nodes). Each spot
+    dict gains "arrival_ms"."""
     rng = XorShift64Star(seed * 1000003 + 11)
     plan, t = [], 0.0
     for i in range(n):
@@ -169,8 +205,7 @@ def serving_stream(n, qps, seed=7):
         if turn:
             s = _sized(batch_turn, sd, 2000, 60000)
         else:
-            s = river_spot(sd)
-            s["nodes"] = _nodes(s)
+            s = river_sized(sd)
         s["arrival_ms"] = t_ms
         return s
     return _parallel(make, plan)
@@ -186,3 +221,8 @@ def large_flop():
 
 def card_text(c):
     return "23456789TJQKA"[c % 13] + "cdhs"[c // 13]
+
+
+def card_id(text):
+    """"Qs" -> suit * 13 + rank (suits c, d, h, s; ranks 2..A)."""
+    return "cdhs".index(text[1].lower()) * 13 + "23456789TJQKA".index(text[0].upper())

@@ -107,7 +107,8 @@ struct BatchProfile {
 constexpr int kGroupCandidates[] = {1, 2, 4, 8, 16, 32};
 constexpr int kNumGroupCandidates = 6;
 BatchProfile profileBatch(const std::vector<SpotRequest>& requests, const std::string& algo,
-                          int iters, int64_t arenaBytes = 0, bool detail = false);
+                          int iters, int64_t arenaBytes = 0, bool detail = false,
+                          int warmup = 0);
 
 }  // namespace gpu
 }  // namespace pps

@@ -173,7 +173,8 @@ class GpuPostflopSolver {
   // timing each launch (ppsg_profile layout): [forward ms, forward launches,
   // backward ms, backward launches, iterations, -, -, -, -, fold ms, fold
   // launches, showdown ms, showdown launches]; detail: 17 values with the
-  // per-kind re-runs (perturbs the rows — measurement only).
+  // per-kind re-runs; the rows and the iteration count are then restored
+  // afterwards (the profiled iterations are not kept).
   std::vector<double> profileIterations(int iters, bool detail = false);
   // Engine facts (include/ppsg.h ppsg_info).
   std::vector<int64_t> engineInfo() const;

@@ -112,7 +112,9 @@ double pps_solver_iteration_bytes(pps_solver* s);
 /* per-launch event timing of `iters` more iterations (profileIterations),
    as ppsg_profile: 13 values (forward incl. fold/showdown launches,
    backward, per-terminal-kind launches); detail != 0: 17 values with the
-   per-kind re-runs of the forward and backward launches */
+   per-kind re-runs of the forward and backward launches, after which
+   the regret and strategy rows and the iteration count are restored
+   (temporary device memory: 8 bytes per regret entry) */
 int pps_solver_profile(pps_solver* s, int iters, int detail, double* out);
 /* ppsg_info of the solver's engine: 6 values */
 int pps_solver_engine_info(pps_solver* s, int64_t out6[6]);
@@ -172,7 +174,8 @@ int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const do
               const char* algo, const pps_serve_options* opt, pps_serve_result* out,
               pps_serve_report* report);
 /* Launch-level timing of n spots resident in one arena (all admitted at
-   once): out = [spots, iters, graph ms per batched iteration, forward ms,
+   once). After the autotuner's walks, `warmup` batched iterations are
+   replayed untimed, then `iters` timed: out = [spots, iters, graph ms per batched iteration, forward ms,
    forward launches, backward ms, backward launches, forward bytes and
    backward bytes per batched iteration (algorithmic f32 rows), kernels
    per iteration, work items per iteration, arena bytes, (detail != 0)
@@ -184,7 +187,7 @@ int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const do
    chance, backward decide, backward chance, fold, showdown, and the fold
    and showdown launch counts over the timed iterations]. */
 int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, const char* algo,
-                      int iters, int64_t arena_bytes, int detail, double out[33]);
+                      int iters, int warmup, int64_t arena_bytes, int detail, double out[33]);
 /* Device bytes the compiled spot takes in an arena (host only). */
 int pps_compile_footprint(const pps_spot* spot, const pps_bet_config* cfg, int64_t* bytes);
 

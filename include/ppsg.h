@@ -131,8 +131,11 @@ int ppsg_info(ppsg_t* h, int64_t* out, int cap);
    launches (ms, launches; part of the forward figures). With cap >= 17 the
    forward and backward launches are also re-run restricted to one node
    kind: out[5..6] backward decide, [7..8] backward chance, [13..14]
-   forward decide, [15..16] forward chance; the decide re-runs apply the
-   updates again, so use that mode for attribution only.
+   forward decide, [15..16] forward chance; the backward re-runs apply the
+   updates again, so in that mode every live spot's regret and strategy
+   tables and its counter are saved first (temporary device memory, 8
+   bytes per regret entry) and restored after: the rows and counters are
+   left as they were.
    (No reference counterpart; the reference's GPU_CFR_PROFILE mode plays
    this role, cuda/gpu_cfr.cu:1895-1987.) */
 int ppsg_profile(ppsg_t* h, int iters, double* out, int cap);

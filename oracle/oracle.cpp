@@ -724,12 +724,15 @@ void orc_solver::cfr(int u, int tr, int mode, const float* reach, float* out,
     } else {
       for (int k = 0; k < nb; ++k) visit(k);
     }
-    for (int i = 0; i < nTr; ++i) out[i] = 0.0f;
+    // branch order, f64 accumulation, one rounding to f32
+    // (postflop-solver solver.rs:191-199)
+    std::vector<double> acc(nTr, 0.0);
     for (int k = 0; k < nb; ++k) {
       const BoardClass& cc = cls[nodes[n.kids[k]].cls];
       const std::vector<int>& tp = cc.toParent[tr];
-      for (size_t j = 0; j < tp.size(); ++j) out[tp[j]] += kidOut[k][j];
+      for (size_t j = 0; j < tp.size(); ++j) acc[tp[j]] += (double)kidOut[k][j];
     }
+    for (int i = 0; i < nTr; ++i) out[i] = (float)acc[i];
     return;
   }
 

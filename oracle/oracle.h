@@ -16,7 +16,9 @@
  *   - tree rules                             cuda/gpu_cfr.cu:1116-1326
  *   - DCFR half-step arithmetic (f32 rows)   cuda/gpu_cfr.cu:227-579
  *       (restated as a recursive per-node walk; showdown via the
- *        strength sweep of postflop-solver game/evaluation.rs:152-250)
+ *        strength sweep of postflop-solver game/evaluation.rs:152-250;
+ *        chance sums accumulated in f64 as postflop-solver's
+ *        solver.rs:191-199, one rounding to f32)
  *   - discount schedules                     cuda/gpu_cfr.cu:585-614
  *   - f64 walks, pair mass, exploitability   cuda/gpu_cfr.cu:871-923,2036-2107
  *   - seeded-strategy probe profiles         cuda/gpu_cfr.cu:2062-2097

@@ -30,6 +30,11 @@ TINY_SPOTS = {
                   bets=[0.5, 1.0], raises=[2.5], allin=True),
     "turn": dict(board="Qs9h2d7c", oop=["AhKh", "QhQd", "JhTc"], ip=["AhKh", "KcQc", "8h8d"],
                  pot=60, stack=100, bets=[0.75], raises=[], allin=True),
+    # Every OOP combo holds the As: after an As river, OOP's list is empty
+    # and the showdown / fold groups of that class have no opponent combos.
+    "blocked": dict(board="Qs9h2d7c", oop=["AsKh", "AsQd", "AsJc"],
+                    ip=["KcQc", "8h8d", "AhKh", "AsTd"], pot=60, stack=100, bets=[0.75],
+                    raises=[], allin=True),
     # Two chance levels (turn deal, then river deal): 31,124 nodes over
     # 2,352 river boards. Check / all-in on every street keeps the brute
     # force affordable while every betting line still reaches both deals.

@@ -58,7 +58,7 @@ def mix():
 def test_per_spot_gates_against_oracle(golden_dir):
     with open(os.path.join(golden_dir, "oracle_batch.json")) as f:
         fixtures = json.load(f)
-    spots = [configs.batch_turn(x["seed"]) if x["kind"] == "turn" else configs.river_spot(x["seed"])
+    spots = [configs.batch_turn(x["seed"]) if x["kind"] == "turn" else configs.river_sized(x["seed"])
              for x in fixtures]
     res, rep = serve(spots, target_frac=0.0, check_every=500, max_iters=500, prefill=1)
     assert rep["peak_live"] == len(spots)

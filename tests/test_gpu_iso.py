@@ -6,10 +6,16 @@ amplifies ulp-level differences in near-zero regrets (as in any
 whole-trajectory comparison), so long solves are held to the parity gates
 (root EV within 0.01 chips, exploitability ratio in [0.5, 2], strategies
 within 0.10 at 500 iterations). Runout summaries cover every dealt card;
-spots without interchangeable suits compile to the very same tree."""
+spots without interchangeable suits compile to the very same tree.
+Against the CPU oracle (which always solves the full tree): early
+iterations agree to f32 rounding, 500-iteration solves pass the gates."""
+import json
+import os
+
 import numpy as np
 import pytest
 
+import orc
 import pps
 from pps import capi, configs
 
@@ -126,3 +132,69 @@ def test_serve_isomorphic_requests_equal_solvers():
         assert r["nodes"] == own.num_nodes
         plain = capi.compile_info(sp, cf)["nodes"]
         assert r["nodes"] < plain
+
+
+ISO_NAMES = ["iso-turn-monotone", "iso-turn-trips", "iso-flop-two-tone", "iso-flop-monotone"]
+
+
+def _iso_ref(golden_dir, name):
+    with open(os.path.join(golden_dir, "oracle_iso.json")) as f:
+        ref = json.load(f)[name]
+    sp = ref["spot"]
+    board = [configs.card_id(ref["board"][i:i + 2]) for i in range(0, len(ref["board"]), 2)]
+    w0, w1 = orc.parse_range(ref["oop"]), orc.parse_range(ref["ip"])
+    return ref, sp, board, w0, w1
+
+
+def _iso_solver(sp, board, w0, w1):
+    return capi.Solver(capi.make_spot(board, w0, w1, sp["pot"], sp["stack"]),
+                       capi.make_config(sp["bets"], sp["raises"], sp["allin"], False, 0),
+                       isomorphism=True)
+
+
+@pytest.mark.parametrize("name", ISO_NAMES)
+def test_isomorphic_early_iterations_match_oracle_full_tree(golden_dir, name):
+    ref, sp, board, w0, w1 = _iso_ref(golden_dir, name)
+    o = orc.Oracle(orc.make_spot(board, w0, w1, sp["pot"], sp["stack"], bets=sp["bets"],
+                                 raises=sp["raises"], bet_allin=sp["allin"]), threads=8)
+    g = _iso_solver(sp, board, w0, w1)
+    assert g.num_nodes < o.counts()["nodes"] == ref["counts"]["nodes"]
+    for iters in (1, 2):
+        o.solve(iters, "dcfr")
+        g.solve(iters, "dcfr")
+        a, b = o.stats(), g.stats()
+        for k in ("ev_oop", "ev_ip", "exploitability"):
+            close(a[k], b[k], 1e-3, f"{name} {k} after {iters}")
+        close(o.root_strategy(), g.root_strategy(), 1e-4, f"{name} root strategy after {iters}")
+
+
+@pytest.mark.parametrize("name", ISO_NAMES)
+def test_isomorphic_solve_gates_against_oracle_full_tree(golden_dir, name):
+    ref, sp, board, w0, w1 = _iso_ref(golden_dir, name)
+    g = _iso_solver(sp, board, w0, w1)
+    g.solve(500, "dcfr")
+    st = g.stats()
+    assert abs(st["ev_oop"] - ref["ev_oop"]) < 0.01, (st, ref["ev_oop"])
+    assert abs(st["ev_oop"] + st["ev_ip"] - sp["pot"]) < 1e-2
+    assert 0.5 <= st["exploitability"] / ref["exploitability"] <= 2.0, (st, ref)
+    close(g.root_strategy(), ref["root_strategy"], 0.10, "root strategy")
+    if ref["node1_strategy"]:
+        close(g.node_strategy(1), ref["node1_strategy"], 0.10, "node 1 strategy")
+
+
+def test_isomorphic_flop_keeps_converging_past_500_iterations():
+    """A monotone flop with bet, raise and all-in: the isomorphic solve
+    follows the full tree to 1,000 iterations. Each dealt card of a mapped
+    suit reads the serving subtree's values through a suit swap, which is
+    only exact while the iterate stays suit-symmetric; chance sums
+    accumulated in f32 broke that symmetry by rounding and the solve
+    drifted off (exploitability 0.13 chips at 1,000 iterations against
+    0.002 for the full tree). With f64 chance sums symmetric combos stay
+    bitwise equal."""
+    full, iso = solvers("Ks8s5s", 1000, bets="0.5,a", raises="2.5", stack=120)
+    a, b = full.stats(), iso.stats()
+    assert iso.num_nodes < 0.5 * full.num_nodes
+    close(a["ev_oop"], b["ev_oop"], 1e-3, "EV oop after 1000")
+    assert 0.5 <= b["exploitability"] / a["exploitability"] <= 2.0, (a, b)
+    assert b["exploitability"] < 0.01, b
+    close(full.strategy(0)["freqs"], iso.strategy(0)["freqs"], 1e-2, "root strategy")

@@ -34,7 +34,9 @@ def rel_err(a, c):
     return float(np.max(np.abs(a - c) / np.maximum(1.0, np.abs(c)))) if len(c) else 0.0
 
 
-@pytest.mark.parametrize("name,steps", [("river", [0, 1, 2, 7, 30]), ("turn", [0, 1, 4, 15])])
+@pytest.mark.parametrize("name,steps", [("river", [0, 1, 2, 7, 30]), ("turn", [0, 1, 4, 15]),
+                                        ("blocked", [0, 1, 6]),
+                                        ("flop", [0, 1, 4, 15])])
 def test_half_steps_match_oracle(name, steps):
     s = spots.tiny_dict(name)
     g = gpu_solver(s)
@@ -52,7 +54,7 @@ def test_half_steps_match_oracle(name, steps):
             assert rel_err(g.rows(1), o.rows(1)) < 1e-5, (name, t, tr)
 
 
-@pytest.mark.parametrize("name", ["river", "turn"])
+@pytest.mark.parametrize("name", ["river", "turn", "blocked", "flop"])
 def test_walks_match_oracle(name):
     s = spots.tiny_dict(name)
     o = orc.Oracle(spots.orc_spot(s))
@@ -81,6 +83,35 @@ def test_plain_turn_half_step_at_full_size():
         g.half_step(tr, 3, s["algo"], 500)
         assert rel_err(g.rows(0), o.rows(0)) < 1e-3
         assert rel_err(g.rows(1), o.rows(1)) < 1e-5
+
+
+def full_range_flop():
+    """Every combo on both sides (1,176 each, weights in (0, 1]): each card's
+    opponent segment holds 48 combos on the flop and 47 after the turn, the
+    largest per-card segments any tree has; check / all-in over two deals."""
+    s = configs.random_spot(11, 3, 1.0, 60, 30, [], [], True, 0)
+    s.update(name="full-range-flop", algo="dcfr")
+    return s
+
+
+def test_flop_half_step_at_full_range():
+    s = full_range_flop()
+    g = gpu_solver(s)
+    o = orc.Oracle(spots.orc_spot(s), threads=8)
+    assert g.num_nodes == o.counts()["nodes"] == 31124
+    o.solve(3, s["algo"])
+    for tr in (0, 1):
+        g.set_rows(0, o.rows(0))
+        g.set_rows(1, o.rows(1))
+        o.half_step(tr, 3, s["algo"], 500)
+        g.half_step(tr, 3, s["algo"], 500)
+        assert rel_err(g.rows(0), o.rows(0)) < 1e-3, tr
+        assert rel_err(g.rows(1), o.rows(1)) < 1e-5, tr
+    for tr in (0, 1):
+        for mode in (1, 2):
+            v, _ = o.root_value(tr, mode)
+            vg, _ = g.root_value(tr, mode)
+            assert vg == pytest.approx(v, abs=1e-4), (tr, mode)
 
 
 def _gates(g, ref, pot):
@@ -130,4 +161,89 @@ def test_warm_start_equals_long_solve_bitwise():
     b.solve(25, "dcfr")
     b.continue_solve(35)
     assert b.total_iterations == 60
+    assert np.array_equal(a.rows(0), b.rows(0)) and np.array_equal(a.rows(1), b.rows(1))
+
+
+def _text_solver(board, oop, ip, pot, stack, bets, raises, allin, geo=False):
+    sp = capi.make_spot([configs.card_id(board[i:i + 2]) for i in range(0, len(board), 2)],
+                        orc.parse_range(oop), orc.parse_range(ip), pot, stack)
+    return capi.Solver(sp, capi.make_config(bets, raises, allin, geo, 0))
+
+
+def test_shortstack_flop_gates_at_500_iterations(golden_dir):
+    """The reference's shortstack flop geometry (tools/quality/bench_flop.sh:47:
+    Qs9h2d, pot 500, stack 150, "0.4,a", raises 2.5x) with its gate ranges,
+    against the oracle's 500-iteration solve."""
+    with open(os.path.join(golden_dir, "oracle_flop.json")) as f:
+        ref = json.load(f)["shortstack-flop"]
+    sp = ref["spot"]
+    g = _text_solver(ref["board"], ref["oop"], ref["ip"], sp["pot"], sp["stack"], sp["bets"],
+                     sp["raises"], sp["allin"])
+    assert g.num_nodes == ref["counts"]["nodes"]
+    g.solve(500, "dcfr")
+    _gates(g, ref, sp["pot"])
+
+
+def test_k83_scenario_reaches_the_reference_answer():
+    """tools/scenarios/k83_utg_bb.sh: flop Kc8h3s, BB (~786 combos) vs UTG,
+    pot 500, stack 3750, geometric + all-in bets, 2.5x raises, solved to
+    exploitability 0.5 chips. The reference records (lines 45-48) EV
+    169.90 / 330.10 at exploitability 0.364 after 256 iterations, with the
+    postflop-solver oracle at 169.91 / 330.09, and BB checking 99.9995 % at
+    the root. EVs agree within the exploitability band."""
+    bb = ("22+,A2s+,K2s+,Q2s+,J2s+,T2s+,98s-92s,87s-82s,76s-72s,65s-62s,54s-52s,43s-42s,32s,"
+          "A2o+,K2o+,Q8o+,J9o+,T9o,98o,97o,87o")
+    utg = "55+,A5s+,K9s+,Q9s+,J9s+,T9s,98s,AJo+,KQo"
+    g = _text_solver("Kc8h3s", bb, utg, 500, 3750, [], [2.5], True, geo=True)
+    assert g.num_nodes == 579660
+    g.solve(100000, "dcfr", target=0.5)
+    st = g.stats()
+    assert st["exploitability"] <= 0.5
+    assert g.iterations_run <= 512, g.iterations_run
+    assert abs(st["ev_oop"] + st["ev_ip"] - 500) < 1e-2
+    band = 0.5
+    assert abs(st["ev_oop"] - 169.90) <= band, st
+    assert abs(st["ev_ip"] - 330.10) <= band, st
+    assert g.root_strategy()[0] >= 0.9999, g.root_strategy()
+
+
+def test_large_flop_properties_at_full_size():
+    """The large flop workload (3.57 M nodes, 30.6 GB of rows) at its full
+    size: zero-sum EVs, exploitability falling from 200 to 1000 iterations,
+    and a second solve of 200 iterations bitwise equal to the first."""
+    s = configs.large_flop()
+    g = gpu_solver(s)
+    assert g.num_nodes == 3572482
+    g.solve(200, s["algo"])
+    a = g.stats()
+    ra = g.root_strategy()
+    g.continue_solve(800)
+    b = g.stats()
+    assert g.total_iterations == 1000
+    for st in (a, b):
+        assert abs(st["ev_oop"] + st["ev_ip"] - s["pot"]) < 1e-2, st
+        assert st["exploitability"] >= 0
+    assert b["exploitability"] < 0.5 * a["exploitability"], (a, b)
+    g.close()
+    h = gpu_solver(s)
+    h.solve(200, s["algo"])
+    assert h.stats() == a
+    assert h.root_strategy() == ra
+
+
+def test_detailed_profile_leaves_the_rows_as_they_were():
+    """profile(detail=True) re-runs the backward launches (their update would
+    apply twice); the rows and the iteration count are restored, so a
+    continued solve equals an uninterrupted one bit for bit."""
+    s = configs.plain_turn()
+    a = gpu_solver(s)
+    a.solve(30, "dcfr")
+    r0, r1 = a.rows(0), a.rows(1)
+    prof = a.profile(3, detail=True)
+    assert prof["iters"] == 3 and prof["bwd_launches"] > 0
+    assert a.total_iterations == 30
+    assert np.array_equal(a.rows(0), r0) and np.array_equal(a.rows(1), r1)
+    a.continue_solve(20)
+    b = gpu_solver(s)
+    b.solve(50, "dcfr")
     assert np.array_equal(a.rows(0), b.rows(0)) and np.array_equal(a.rows(1), b.rows(1))

@@ -187,6 +187,7 @@ def _tiny(name):
 
 
 @pytest.mark.parametrize("name,steps", [("river", [0, 1, 2, 7, 30]), ("turn", [0, 1, 4, 15]),
+                                        ("blocked", [0, 1, 6]),
                                         ("flop", [0, 3])])
 def test_half_steps_match_per_pair_brute_force(name, steps):
     sp = _tiny(name)
@@ -206,7 +207,7 @@ def test_half_steps_match_per_pair_brute_force(name, steps):
                 assert err < 2e-4, (name, t, tr, which, err)
 
 
-@pytest.mark.parametrize("name", ["river", "turn", "flop"])
+@pytest.mark.parametrize("name", ["river", "turn", "blocked", "flop"])
 def test_walks_match_per_pair_brute_force(name):
     sp = _tiny(name)
     o = orc.Oracle(sp)
