@@ -1009,6 +1009,7 @@ struct Layout {
   int64_t toParent[2] = {0, 0}, bmap[2] = {0, 0}, gather[2] = {0, 0}, dealSib = 0;
   int64_t cardSame[2] = {0, 0}, sdMeta[2] = {0, 0};
   int64_t regret = 0, strat = 0, cfv = 0, reach[2] = {0, 0};
+  int64_t weight[2] = {0, 0};  // f64 base weights (root value sums)
   int64_t termNodes = 0;  // TermNode entries of every terminal group
   int64_t add(int64_t b) {
     const int64_t off = bytes;
@@ -1096,6 +1097,7 @@ struct Spot {
 // Plans a spot: chunk layout, host tables and chunk-relative work items.
 int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
   if (!t || t->n_nodes <= 0 || t->n_base[0] <= 0 || t->n_base[1] <= 0) return fail("empty tree");
+  if (!t->weight[0] || !t->weight[1]) return fail("tree description lacks the base weights");
   const int nc = t->n_class;
   const int64_t N = t->n_nodes;
   for (int p = 0; p < 2; ++p)
@@ -1150,6 +1152,7 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
     L.sdMeta[p] = L.add(16 * t->list_total[p]);
   }
   L.dealSib = L.add(t->deal_total);
+  for (int p = 0; p < 2; ++p) L.weight[p] = L.add(8 * (int64_t)t->n_base[p]);
   L.regret = L.add(4 * t->reg_total);
   L.strat = L.add(4 * t->reg_total);
   L.cfv = L.add(4 * t->cfv_total);
@@ -1338,6 +1341,54 @@ struct DevList {
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Root value sums: per listed spot, sum_i w[i] * cfv_root[i] over its base
+// list for one traverser, in index order, in f64 with separately rounded
+// multiplies and adds (the host loop's arithmetic, bit for bit). One 64-lane
+// block per spot: the lanes stage 64 entries at a time in LDS, lane 0 adds
+// them in order.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct RootSumJob {
+  int64_t cfv, w;  // byte offsets in the arena: root cfv row (f32), weights (f64)
+  int32_t n, pad;
+};
+
+__global__ __launch_bounds__(kWave) void rootSumKernel(const char* __restrict__ arena,
+                                                       const RootSumJob* __restrict__ jobs,
+                                                       double* __restrict__ out) {
+  __shared__ float c[kWave];
+  __shared__ double w[kWave];
+  const RootSumJob j = jobs[blockIdx.x];
+  const float* __restrict__ cf = reinterpret_cast<const float*>(arena + j.cfv);
+  const double* __restrict__ wt = reinterpret_cast<const double*>(arena + j.w);
+  const int t = (int)threadIdx.x;
+  double s = 0.0;
+  for (int b = 0; b < j.n; b += kWave) {
+    c[t] = b + t < j.n ? cf[b + t] : 0.0f;
+    w[t] = b + t < j.n ? wt[b + t] : 0.0;
+    __syncthreads();
+    if (t == 0) {
+#pragma clang fp contract(off)
+      const int m = min(kWave, j.n - b);
+      for (int k = 0; k < m; ++k) s += w[k] * (double)c[k];  // no fused multiply-add
+    }
+    __syncthreads();
+  }
+  if (t == 0) out[blockIdx.x] = s;
+}
+
+// Root-sum staging: up to kRootSumsMax descriptors, then as many results
+// (the engine's pinned host staging has this layout).
+constexpr int kRootSumsMax = 4096;
+constexpr size_t kSumJobBytes = sizeof(RootSumJob) * kRootSumsMax;
+constexpr size_t kSumStageBytes = kSumJobBytes + sizeof(double) * kRootSumsMax;
+static_assert(kSumJobBytes % 256 == 0, "results start 256-byte aligned");
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // Engine: one arena, any number of spots.
 //
 // Spot lifecycle: add -> live (items appended to the merged lists); park /
@@ -1374,6 +1425,10 @@ struct ppsg_engine {
   double lastMs = -1.0;
   int* flagsDev = nullptr;  // maxSpots alive flags staged for setAliveFlags
   int forceMask = 0;        // kernels check alive flags even with no dead items
+  // root value sums: descriptors + results staged here (grow-only)
+  char* sumStage = nullptr;
+  int64_t sumStageCap = 0;
+  char* sumHost = nullptr;     // pinned host staging of the root sums (kSumStageBytes, set at init)
 
   Dev dev() const { return Dev{arena.base, ctl}; }
   bool valid(int slot) const {
@@ -1434,11 +1489,18 @@ ppsg_engine::~ppsg_engine() {
   if (ev1) (void)hipEventDestroy(ev1);
   if (stream) (void)hipStreamDestroy(stream);
   if (ctl) (void)hipFree(ctl);
+  if (sumStage) (void)hipFree(sumStage);
+  if (sumHost) (void)hipHostFree(sumHost);
   if (arena.base) (void)hipFree(arena.base);
 }
 
 int ppsg_engine::init(int64_t bytes, int spotsCap) {
   if (bytes <= 0 || spotsCap <= 0 || spotsCap > 65536) return fail("engine: bad capacity");
+  {
+    void* hp = nullptr;
+    HIP_TRY(hipHostMalloc(&hp, kSumStageBytes, hipHostMallocDefault));
+    sumHost = static_cast<char*>(hp);
+  }
   bytes = roundUp(bytes);
   void* p = nullptr;
   HIP_TRY(hipMalloc(&p, (size_t)bytes));
@@ -1751,7 +1813,8 @@ int ppsg_engine::add(const ppsg_tree_desc* t, int* slot) {
         put(L.gather[p], T.gather[p].data(), (int64_t)T.gather[p].size()) ||
         put(L.cardSame[p], T.cardSame[p].data(), 4 * t->list_total[p]) ||
         put(L.sdMeta[p], t->sd_meta[p], 16 * t->list_total[p]) ||
-        put(L.reach[p] + 4 * t->reach_off[p][0], t->root_reach[p], 4 * (int64_t)t->n_base[1 - p]))
+        put(L.reach[p] + 4 * t->reach_off[p][0], t->root_reach[p], 4 * (int64_t)t->n_base[1 - p]) ||
+        put(L.weight[p], t->weight[p], 8 * (int64_t)t->n_base[p]))
       rc = -1;
   }
   if (rc == 0 && put(L.dealSib, t->deal_sib, t->deal_total)) rc = -1;
@@ -2108,6 +2171,46 @@ int transfer(ppsg_t* h, int slot, int buf, int64_t off, int64_t n, void* host, b
   return 0;
 }
 
+// Root value sums of the listed spots, descriptors and results staged in
+// `stage` (device memory: the descriptors, padded to 256 bytes, then one
+// double per listed spot). Up to kRootSumsMax spots go through the engine's
+// pinned host staging (asynchronous copies); more through plain host memory.
+int rootSums(ppsg_t* h, int tr, int nSlots, const int* slots, double* sums, char* stage) {
+  const size_t jobBytes = roundUp((int64_t)(sizeof(RootSumJob) * (size_t)nSlots));
+  const bool pinned = nSlots <= kRootSumsMax;
+  std::vector<RootSumJob> own;
+  if (!pinned) own.resize((size_t)nSlots);
+  RootSumJob* jobs = pinned ? reinterpret_cast<RootSumJob*>(h->sumHost) : own.data();
+  double* res = pinned ? reinterpret_cast<double*>(h->sumHost + kSumJobBytes) : sums;
+  for (int k = 0; k < nSlots; ++k) {
+    const Spot& S = h->spots[slots[k]];
+    jobs[k].cfv = S.off + S.rootCfv;
+    jobs[k].w = S.off + S.L.weight[tr];
+    jobs[k].n = S.n[tr];
+    jobs[k].pad = 0;
+  }
+  HIP_TRY(hipMemcpyAsync(stage, jobs, sizeof(RootSumJob) * (size_t)nSlots,
+                         hipMemcpyHostToDevice, h->stream));
+  rootSumKernel<<<(unsigned)nSlots, kWave, 0, h->stream>>>(
+      h->arena.base, reinterpret_cast<const RootSumJob*>(stage),
+      reinterpret_cast<double*>(stage + jobBytes));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(res, stage + jobBytes, sizeof(double) * (size_t)nSlots,
+                         hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  if (pinned) std::memcpy(sums, res, sizeof(double) * (size_t)nSlots);
+  return 0;
+}
+
+int rootSumsArgs(ppsg_t* h, int tr, int nSlots, const int* slots, double* sums) {
+  if (!h) return fail("null handle");
+  if (tr != 0 && tr != 1) return fail("root sums: traverser must be 0 or 1");
+  if (nSlots < 0 || (nSlots > 0 && (!slots || !sums))) return fail("null argument");
+  for (int k = 0; k < nSlots; ++k)
+    if (!h->valid(slots[k])) return fail("root sums: no spot in slot " + std::to_string(slots[k]));
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2394,6 +2497,24 @@ int ppsg_batch_walk(ppsg_t* h, int tr, int mode, int n_slots, const int* slots,
 int ppsg_batch_walk_only(ppsg_t* h, int tr, int mode, int n_slots, const int* slots,
                          float* const* root_cfv) {
   return walkOnly(h, tr, mode, n_slots, slots, root_cfv);
+}
+
+int ppsg_batch_root_sums(ppsg_t* h, int tr, int n_slots, const int* slots, double* sums) {
+  if (rootSumsArgs(h, tr, n_slots, slots, sums) != 0) return -1;
+  if (n_slots == 0) return 0;
+  const int64_t need = roundUp((int64_t)(sizeof(RootSumJob) * (size_t)n_slots)) +
+                       (int64_t)sizeof(double) * n_slots;
+  if (need > h->sumStageCap) {
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (h->sumStage) (void)hipFree(h->sumStage);
+    h->sumStage = nullptr;
+    h->sumStageCap = 0;
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, (size_t)need));
+    h->sumStage = static_cast<char*>(p);
+    h->sumStageCap = need;
+  }
+  return rootSums(h, tr, n_slots, slots, sums, h->sumStage);
 }
 
 int ppsg_batch_read(ppsg_t* h, int slot, int buf, int64_t off, int64_t n, float* host) {

@@ -589,6 +589,7 @@ ppsg_tree_desc CompiledTree::desc() const {
     d.deal_map_off[p] = dealMapOff[p].data();
     d.deal_map_total[p] = (int64_t)dealMap[p].size();
     d.deal_map[p] = dealMap[p].data();
+    d.weight[p] = w[p].data();
   }
   d.deal_n = dealN.data();
   d.deal_off = dealOff.data();

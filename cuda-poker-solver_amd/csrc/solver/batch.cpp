@@ -232,27 +232,25 @@ std::vector<SpotResult> serve(const std::vector<SpotRequest>& reqs, const ServeO
     return true;
   };
 
-  std::vector<std::vector<float>> bufs;
-  std::vector<int> slots;
-  std::vector<float*> ptrs;
+  std::vector<int> slots, which;
+  std::vector<double> sums;
+  // root value per pair of every wanted live spot: one walk of all live
+  // spots, then the weighted root sums on the device (one launch, one copy
+  // back; the same f64 values the host loop over copied rows gave)
   auto rootValues = [&](int tr, int mode, const std::vector<char>& want, std::vector<double>& v) {
     const int k = (int)live.size();
-    bufs.resize(k);
-    slots.resize(k);
-    ptrs.resize(k);
-    for (int i = 0; i < k; ++i) {
-      slots[i] = live[i].slot;
-      bufs[i].resize(live[i].w[tr].size());
-      ptrs[i] = want[i] ? bufs[i].data() : nullptr;
-    }
-    check(ppsg_batch_walk(h, tr, mode, k, slots.data(), ptrs.data()), "batch walk");
+    check(ppsg_batch_walk(h, tr, mode, 0, nullptr, nullptr), "batch walk");
+    slots.clear();
+    which.clear();
+    for (int i = 0; i < k; ++i)
+      if (want[i] && live[i].Z > 0.0) {
+        slots.push_back(live[i].slot);
+        which.push_back(i);
+      }
+    sums.assign(slots.size(), 0.0);
+    check(ppsg_batch_root_sums(h, tr, (int)slots.size(), slots.data(), sums.data()), "root sums");
     v.assign(k, 0.0);
-    for (int i = 0; i < k; ++i) {
-      if (!want[i] || !(live[i].Z > 0.0)) continue;
-      double s = 0.0;
-      for (size_t c = 0; c < live[i].w[tr].size(); ++c) s += live[i].w[tr][c] * (double)bufs[i][c];
-      v[i] = s / live[i].Z;
-    }
+    for (size_t j = 0; j < which.size(); ++j) v[which[j]] = sums[j] / live[which[j]].Z;
   };
 
   while (finished < n) {

@@ -459,6 +459,19 @@ int pps_solver_engine_info(pps_solver* s, int64_t out6[6]) {
   });
 }
 
+int pps_solver_engine(pps_solver* s, ppsg_engine** engine, int* slot) {
+  return guard([&] {
+    if (!engine || !slot) throw std::runtime_error("null output");
+    return on(s, [&](pps::gpu::GpuPostflopSolver& g) {
+      ppsg_engine* e = g.engineHandle();
+      if (!e) throw std::runtime_error("solver compiled to empty ranges (no engine)");
+      *engine = e;
+      *slot = g.engineSlot();
+      return 0;
+    });
+  });
+}
+
 void pps_serve_defaults(pps_serve_options* opt) {
   if (!opt) return;
   const pps::gpu::ServeOptions d;

@@ -276,20 +276,16 @@ void GpuPostflopSolver::advanced(int n, double deviceMs) {
   impl_->lastMs += deviceMs;
 }
 
-double GpuPostflopSolver::explFrom(const float* br0, const float* br1) const {
+double GpuPostflopSolver::explFromSums(double s0, double s1) const {
   const Impl* I = impl_;
-  double v[2] = {0.0, 0.0};
-  const float* rows[2] = {br0, br1};
-  for (int tr = 0; tr < 2; ++tr) {
-    for (int i = 0; i < I->T.n[tr]; ++i) v[tr] += I->T.w[tr][i] * (double)rows[tr][i];
-    v[tr] = I->Z > 0.0 ? v[tr] / I->Z : 0.0;
-  }
-  return (v[0] + v[1] - (double)I->T.spot.pot) / 2.0;
+  const double v0 = I->Z > 0.0 ? s0 / I->Z : 0.0, v1 = I->Z > 0.0 ? s1 / I->Z : 0.0;
+  return (v0 + v1 - (double)I->T.spot.pot) / 2.0;
 }
 
 int GpuPostflopSolver::slot() const { return impl_->T.empty ? -1 : impl_->slot; }
 
-int GpuPostflopSolver::rootWidth(int tr) const { return impl_->T.empty ? 0 : impl_->T.n[tr]; }
+ppsg_t* GpuPostflopSolver::engineHandle() const { return impl_->T.empty ? nullptr : impl_->eng; }
+
 
 void GpuPostflopSolver::init(const PostflopSpot& spot, const pf::BetConfig& cfg, bool device,
                              const SolverOptions& opt) {

@@ -318,18 +318,19 @@ void SpotServer::step(std::unique_lock<std::mutex>& lk) {
   if (checks.empty()) return;
   const auto t0 = std::chrono::steady_clock::now();
   std::vector<int> slots;
-  std::vector<std::vector<float>> rows[2];
-  std::vector<float*> outs[2];
   for (Entry* e : checks) slots.push_back(e->s->slot());
+  // best-response walks of just these spots; their root sums on the device
+  const std::vector<float*> noRows(slots.size(), nullptr);
+  std::vector<double> sums[2];
   for (int tr = 0; tr < 2; ++tr) {
-    for (Entry* e : checks) rows[tr].emplace_back((size_t)e->s->rootWidth(tr));
-    for (auto& r : rows[tr]) outs[tr].push_back(r.data());
     check(ppsg_batch_walk_only(eng_, tr, PPSG_MODE_BR, (int)slots.size(), slots.data(),
-                               outs[tr].data()));
+                               noRows.data()));
+    sums[tr].assign(slots.size(), 0.0);
+    check(ppsg_batch_root_sums(eng_, tr, (int)slots.size(), slots.data(), sums[tr].data()));
   }
   for (size_t i = 0; i < checks.size(); ++i) {
     Entry* e = checks[i];
-    if (e->s->explFrom(rows[0][i].data(), rows[1][i].data()) <= e->target) {
+    if (e->s->explFromSums(sums[0][i], sums[1][i]) <= e->target) {
       finish(*e, "done");
     } else {
       e->nextStop = std::min<int64_t>(e->job.total, e->nextStop + kCheckEvery);

@@ -66,7 +66,8 @@ CAPI_SYMBOLS = [
     "pps_solver_num_base_combos", "pps_solver_save", "pps_solver_read_rows",
     "pps_solver_write_rows", "pps_solver_half_step", "pps_solver_root_value",
     "pps_solver_iteration_bytes", "pps_solver_last_solve_ms", "pps_solver_profile",
-    "pps_solver_engine_info", "pps_serve_defaults", "pps_serve", "pps_compile_footprint",
+    "pps_solver_engine_info", "pps_solver_engine", "pps_serve_defaults", "pps_serve",
+    "pps_compile_footprint",
     "pps_server_create", "pps_server_destroy", "pps_server_add", "pps_server_load",
     "pps_solver_create_ex", "pps_server_add_ex", "pps_compile_info_ex",
     "pps_server_stats", "pps_solver_start_solve", "pps_solver_start_continue", "pps_solver_job",
@@ -80,7 +81,7 @@ PPSG_SYMBOLS = [
     "ppsg_footprint", "ppsg_batch_create", "ppsg_batch_add", "ppsg_batch_remove",
     "ppsg_batch_set_active", "ppsg_batch_set_schedule", "ppsg_batch_set_counter",
     "ppsg_batch_get_counter", "ppsg_batch_reset", "ppsg_batch_step", "ppsg_batch_walk", "ppsg_batch_walk_only",
-    "ppsg_batch_read", "ppsg_batch_write", "ppsg_batch_free_bytes", "ppsg_batch_compact",
+    "ppsg_batch_read", "ppsg_batch_root_sums", "ppsg_batch_write", "ppsg_batch_free_bytes", "ppsg_batch_compact",
     "ppsg_device_memory", "ppsg_tune", "ppsg_autotune", "ppsg_kind_bytes",
 ]
 

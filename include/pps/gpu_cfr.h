@@ -178,6 +178,11 @@ class GpuPostflopSolver {
   std::vector<double> profileIterations(int iters, bool detail = false);
   // Engine facts (include/ppsg.h ppsg_info).
   std::vector<int64_t> engineInfo() const;
+  // Addition: the engine handle this solver drives (its own, or the arena of
+  // the SpotServer serving it; nullptr for an empty tree) and its slot there,
+  // for callers of the ppsg_* calls on it (e.g. ppsg_batch_root_sums).
+  ppsg_engine* engineHandle() const;
+  int engineSlot() const { return slot(); }
 
  private:
   friend class SpotServer;
@@ -203,15 +208,14 @@ class GpuPostflopSolver {
 
   // Server hooks. beginSolve / beginContinue check arguments as solve() /
   // continueSolve() do and set the spot's rows and schedule; advanced(n)
-  // books n iterations run by the server; explFrom turns the two root BR
-  // rows of this spot into its exploitability. slot() is -1 for an empty
-  // tree.
+  // books n iterations run by the server; explFromSums turns the two root
+  // BR sums of this spot (ppsg_batch_root_sums) into its exploitability.
+  // slot() is -1 for an empty tree.
   bool beginSolve(int iterations, const std::string& algo);
   bool beginContinue(int iterations);
   void advanced(int n, double deviceMs);
-  double explFrom(const float* br0, const float* br1) const;
+  double explFromSums(double br0Sum, double br1Sum) const;
   int slot() const;
-  int rootWidth(int tr) const;
 };
 
 }  // namespace gpu

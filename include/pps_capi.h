@@ -118,6 +118,13 @@ double pps_solver_iteration_bytes(pps_solver* s);
 int pps_solver_profile(pps_solver* s, int iters, int detail, double* out);
 /* ppsg_info of the solver's engine: 6 values */
 int pps_solver_engine_info(pps_solver* s, int64_t out6[6]);
+/* The device engine (include/ppsg.h) the solver drives and its slot there:
+   the solver's own one-spot engine (slot 0), or the arena of the server that
+   serves it. For callers of the ppsg_* calls on it, e.g. the root value sums
+   (ppsg_batch_root_sums). The handle stays owned by the solver or server; a
+   served solver's engine is driven by the server's thread while a job runs.
+   Error for an empty tree. No reference counterpart. */
+int pps_solver_engine(pps_solver* s, struct ppsg_engine** engine, int* slot);
 double pps_solver_last_solve_ms(pps_solver* s);
 
 /* ---- many spots on one GPU (no reference counterpart) ----

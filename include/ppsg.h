@@ -85,6 +85,8 @@ typedef struct ppsg_tree_desc {
   const int64_t* deal_map_off[2];
   int64_t deal_map_total[2];
   const int16_t* deal_map[2];
+  /* base weights (n_base[p] each, f64): the root value sums */
+  const double* weight[2];
 } ppsg_tree_desc;
 
 typedef struct ppsg_engine ppsg_t;
@@ -188,6 +190,17 @@ int ppsg_batch_walk(ppsg_t* h, int tr, int mode, int n_slots, const int* slots,
 int ppsg_batch_walk_only(ppsg_t* h, int tr, int mode, int n_slots, const int* slots,
                          float* const* root_cfv);
 int ppsg_batch_read(ppsg_t* h, int slot, int buf, int64_t off, int64_t n, float* host);
+/* Root value sums of the listed spots (live or parked; a slot may repeat):
+   sums[k] = sum over the spot's base list of w_tr[i] * root_cfv[i], in index
+   order, in f64 with each multiply and add rounded on its own (the host
+   loop `s += w[i] * (double)cfv[i]` bit for bit), where root_cfv is the
+   spot's root cfv row as the last launch for traverser tr left it (a walk
+   or half-step of tr, e.g. ppsg_batch_walk with no output rows). The
+   per-spot descriptors and the results are staged in device memory the
+   handle allocates on first use and keeps (grown as needed). Replaces the
+   host-side reductions of root rows copied back one spot at a time
+   (cuda/gpu_cfr.cu:2036-2107, rootValue / explNow). */
+int ppsg_batch_root_sums(ppsg_t* h, int tr, int n_slots, const int* slots, double* sums);
 int ppsg_batch_write(ppsg_t* h, int slot, int buf, int64_t off, int64_t n, const float* host);
 /* Free arena bytes in total and in the largest block (either may be NULL). */
 int ppsg_batch_free_bytes(ppsg_t* h, int64_t* total, int64_t* largest);
