@@ -39,6 +39,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1380,11 +1381,42 @@ __global__ __launch_bounds__(kWave) void rootSumKernel(const char* __restrict__ 
 }
 
 // Root-sum staging: up to kRootSumsMax descriptors, then as many results
-// (the engine's pinned host staging has this layout).
+// (the engine's pinned host staging, and the fixed device array of the
+// no-allocation call, have this layout).
 constexpr int kRootSumsMax = 4096;
 constexpr size_t kSumJobBytes = sizeof(RootSumJob) * kRootSumsMax;
 constexpr size_t kSumStageBytes = kSumJobBytes + sizeof(double) * kRootSumsMax;
 static_assert(kSumJobBytes % 256 == 0, "results start 256-byte aligned");
+
+}  // namespace
+
+// Staging for ppsg_batch_root_sums_noalloc: the callers that cannot allocate
+// get this fixed array (each GPU holds its own copy for the life of the
+// process).
+static __device__ __align__(256) char scratch[kSumStageBytes];
+
+namespace {
+
+// This device's address of `scratch` (the host-side name is not a device
+// address), looked up once per device with that device current.
+std::mutex g_scratchMu;
+char* g_scratchAddr[64] = {};
+
+int scratchAddress(char** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail("scratch: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratchMu);
+  if (!g_scratchAddr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(std::string("scratch: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+    g_scratchAddr[dev] = static_cast<char*>(p);
+  }
+  *out = g_scratchAddr[dev];
+  return 0;
+}
 
 }  // namespace
 
@@ -1428,6 +1460,7 @@ struct ppsg_engine {
   // root value sums: descriptors + results staged here (grow-only)
   char* sumStage = nullptr;
   int64_t sumStageCap = 0;
+  char* scratchDev = nullptr;  // this device's address of `scratch` (set at init)
   char* sumHost = nullptr;     // pinned host staging of the root sums (kSumStageBytes, set at init)
 
   Dev dev() const { return Dev{arena.base, ctl}; }
@@ -1496,6 +1529,7 @@ ppsg_engine::~ppsg_engine() {
 
 int ppsg_engine::init(int64_t bytes, int spotsCap) {
   if (bytes <= 0 || spotsCap <= 0 || spotsCap > 65536) return fail("engine: bad capacity");
+  if (scratchAddress(&scratchDev) != 0) return -1;
   {
     void* hp = nullptr;
     HIP_TRY(hipHostMalloc(&hp, kSumStageBytes, hipHostMallocDefault));
@@ -2515,6 +2549,16 @@ int ppsg_batch_root_sums(ppsg_t* h, int tr, int n_slots, const int* slots, doubl
     h->sumStageCap = need;
   }
   return rootSums(h, tr, n_slots, slots, sums, h->sumStage);
+}
+
+int ppsg_batch_root_sums_noalloc(ppsg_t* h, int tr, int n_slots, const int* slots, double* sums) {
+  if (rootSumsArgs(h, tr, n_slots, slots, sums) != 0) return -1;
+  if (n_slots > kRootSumsMax)
+    return fail("root sums (no allocation): at most " + std::to_string(kRootSumsMax) +
+                " spots per call, " + std::to_string(n_slots) + " given");
+  if (n_slots == 0) return 0;
+  if (!h->scratchDev) return fail("root sums (no allocation): scratch address not set");
+  return rootSums(h, tr, n_slots, slots, sums, h->scratchDev);
 }
 
 int ppsg_batch_read(ppsg_t* h, int slot, int buf, int64_t off, int64_t n, float* host) {

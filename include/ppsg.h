@@ -201,6 +201,17 @@ int ppsg_batch_read(ppsg_t* h, int slot, int buf, int64_t off, int64_t n, float*
    host-side reductions of root rows copied back one spot at a time
    (cuda/gpu_cfr.cu:2036-2107, rootValue / explNow). */
 int ppsg_batch_root_sums(ppsg_t* h, int tr, int n_slots, const int* slots, double* sums);
+/* ppsg_batch_root_sums without allocating anything. No reference
+   counterpart: it is for callers that cannot allocate device memory (code
+   under graph capture, tight serving loops, harnesses that forbid it). The
+   staging lives in a fixed device array of this library (128 KiB per GPU,
+   held until the process ends); at most 4096 spots per call (more is an
+   error and changes nothing). The array is shared by every call on a GPU
+   and keeps the last call's contents; each call writes every descriptor
+   and result it reads, so nothing has to start at zero. Calls that overlap
+   on one GPU (from several threads or handles) race on the array. */
+int ppsg_batch_root_sums_noalloc(ppsg_t* h, int tr, int n_slots, const int* slots,
+                                 double* sums);
 int ppsg_batch_write(ppsg_t* h, int slot, int buf, int64_t off, int64_t n, const float* host);
 /* Free arena bytes in total and in the largest block (either may be NULL). */
 int ppsg_batch_free_bytes(ppsg_t* h, int64_t* total, int64_t* largest);
