@@ -2477,6 +2477,63 @@ int ppsg_profile(ppsg_t* h, int iters, double* out, int cap) {
   return 0;
 }
 
+int ppsg_profile_depths(ppsg_t* h, int iters, double* out, int cap) {
+  if (!h || !out) return fail("null argument");
+  if (iters <= 0) return fail("profile: iters must be positive");
+  if (h->prepare() != 0) return -1;
+  if (h->live.empty()) return fail("no live spot");
+  if (h->covers(iters) != 0) return -1;
+  const int D = h->depthCount();
+  if (cap < 1 + 4 * D + 4) return fail("profile: output holds " + std::to_string(cap) +
+                                       " values, needs " + std::to_string(1 + 4 * D + 4));
+  std::vector<double> acc((size_t)(4 * D + 4), 0.0);
+  hipEvent_t a = nullptr, b = nullptr;
+  HIP_TRY(hipEventCreate(&a));
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return fail("profile: event creation failed");
+  }
+  auto timed = [&](auto&& launch, size_t slot) -> int {
+    HIP_TRY(hipEventRecord(a, h->stream));
+    if (launch() != 0) return -1;
+    HIP_TRY(hipEventRecord(b, h->stream));
+    HIP_TRY(hipEventSynchronize(b));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, a, b));
+    acc[slot] += 1e3 * ms;
+    return 0;
+  };
+  int rc = 0;
+  for (int it = 0; it < iters && rc == 0; ++it) {
+    for (int tr = 0; tr < 2 && rc == 0; ++tr) {
+      for (int dpt = 0; dpt < D && rc == 0; ++dpt)
+        if (h->lists[tr][0][dpt].count)
+          rc = timed([&] { return h->launchFwd(tr, dpt, 0, -1, h->stream); },
+                     (size_t)(4 * dpt + 2 * tr));
+      for (int ps = 2; ps < kLists && rc == 0; ++ps)
+        if (!h->lists[tr][ps].empty() && h->lists[tr][ps][0].count)
+          rc = timed([&] { return h->launchTerm(tr, ps, h->stream); },
+                     (size_t)(4 * D + (ps == 2 ? 0 : 2) + tr));
+      for (int dpt = D - 1; dpt >= 0 && rc == 0; --dpt)
+        if (h->lists[tr][1][dpt].count)
+          rc = timed([&] { return h->launchBwd(tr, dpt, PPSG_MODE_CFR, -1, h->stream); },
+                     (size_t)(4 * dpt + 2 * tr + 1));
+    }
+    if (rc == 0) {
+      advanceCounters<<<1, kBlock, 0, h->stream>>>(h->ctl, h->liveDev, (int)h->live.size());
+      if (hipGetLastError() != hipSuccess) rc = fail("profile: counter launch failed");
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (rc != 0) return rc;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  for (int s : h->live) h->spots[s].counter += iters;
+  out[0] = D;
+  for (size_t i = 0; i < acc.size(); ++i) out[1 + i] = acc[i];
+  return 0;
+}
+
 // ---- batches ----
 
 int ppsg_footprint(const ppsg_tree_desc* desc, int64_t* bytes) {

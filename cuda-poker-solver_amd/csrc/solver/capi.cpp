@@ -19,6 +19,7 @@
 #include "pps/postflop_cfr.h"
 #include "pps/range.h"
 #include "pps/server.h"
+#include "solver/solution_file.h"
 
 // A solver of its own, or a spot served by a SpotServer (then queries run
 // between the server's steps and solves are batched jobs of the server).
@@ -583,6 +584,25 @@ int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, 
     for (int k = 0; k < 6; ++k) out[25 + k] = P.kindBytes[k];
     out[31] = P.foldLaunches;
     out[32] = P.showdownLaunches;
+    return 0;
+  });
+}
+
+int pps_solution_inspect(const char* path, pps_solution_info* out) {
+  return guard([&] {
+    if (!path || !out) throw std::runtime_error("null argument");
+    const pps::gpu::Solution s = pps::gpu::readSolution(path, false);
+    std::memset(out, 0, sizeof(*out));
+    out->n_board = s.spot.nBoard;
+    for (int i = 0; i < 5; ++i) out->board[i] = s.spot.board[i];
+    out->pot = s.spot.pot;
+    out->stack = s.spot.stack;
+    out->n_bets = (int32_t)s.cfg.betFracs.size();
+    out->n_raises = (int32_t)s.cfg.raiseMults.size();
+    out->nodes = s.nodes;
+    out->iterations = s.iterations;
+    out->rows = s.rows;
+    std::strncpy(out->algo, s.algo.c_str(), sizeof(out->algo) - 1);
     return 0;
   });
 }

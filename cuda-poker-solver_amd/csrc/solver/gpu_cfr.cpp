@@ -13,6 +13,7 @@
 #include <string>
 
 #include "../host/compile.h"
+#include "solution_file.h"
 #include "ppsg.h"
 
 namespace pps {
@@ -329,9 +330,51 @@ void GpuPostflopSolver::reset() {
   check(ppsg_batch_reset(impl_->eng, impl_->slot));
 }
 
+// GPU_CFR_PROFILE set: up to 64 of the iterations run launch by launch with
+// per-depth timings on stderr instead of the graph replay (the reference's
+// profiling mode, cuda/gpu_cfr.cu:1895-1987); only those iterations count.
+void GpuPostflopSolver::profileDepthsRun(int iterations) {
+  Impl* I = impl_;
+  const int pit = std::min(iterations, 64);
+  const int D = I->T.maxDepth + 1;
+  std::vector<double> out((size_t)(1 + 4 * D + 4), 0.0);
+  check(ppsg_profile_depths(I->eng, pit, out.data(), (int)out.size()));
+  const int Dk = (int)out[0];
+  std::vector<int> kinds((size_t)D * 4, 0);  // per depth: decide, showdown, fold, chance
+  for (int64_t u = 0; u < I->T.numNodes; ++u) {
+    const int k = I->T.kind[u];
+    const int col = k == PPSG_DECIDE ? 0 : k == PPSG_SHOWDOWN ? 1 : k == PPSG_FOLD ? 2 : 3;
+    kinds[(size_t)I->T.depth[u] * 4 + col]++;
+  }
+  double fwd = 0.0, bwd = 0.0;
+  for (int d = 0; d < std::min(D, Dk); ++d) {
+    const double* t = out.data() + 1 + 4 * d;
+    fwd += t[0] + t[2];
+    bwd += t[1] + t[3];
+    std::fprintf(stderr,
+                 "gpu_cfr: d%-2d decide=%-5d show=%-5d fold=%-5d chance=%-3d  fwd %6.1f/%6.1f us"
+                 "  bwd %6.1f/%6.1f us\n",
+                 d, kinds[(size_t)d * 4], kinds[(size_t)d * 4 + 1], kinds[(size_t)d * 4 + 2],
+                 kinds[(size_t)d * 4 + 3], t[0] / pit, t[2] / pit, t[1] / pit, t[3] / pit);
+  }
+  const double* term = out.data() + 1 + 4 * Dk;
+  std::fprintf(stderr, "gpu_cfr: terminals fold %6.1f/%6.1f us  showdown %6.1f/%6.1f us\n",
+               term[0] / pit, term[1] / pit, term[2] / pit, term[3] / pit);
+  bwd += term[0] + term[1] + term[2] + term[3];
+  std::fprintf(stderr, "gpu_cfr: per-iter (serialized) fwd %.1f us  bwd %.1f us over %d iters\n",
+               fwd / pit, bwd / pit, pit);
+  iterationsRun_ = pit;
+  totalIters_ += pit;
+  I->lastMs = (fwd + bwd) / 1e3;
+}
+
 void GpuPostflopSolver::replayChunks(int iterations, double target) {
   Impl* I = impl_;
   I->ownEngine("solve");
+  if (std::getenv("GPU_CFR_PROFILE")) {
+    profileDepthsRun(iterations);
+    return;
+  }
   const int chunk = kCheckEvery;
   int done = 0;
   double ms = 0.0;
@@ -754,103 +797,28 @@ double GpuPostflopSolver::rootValue(int tr, int mode, std::vector<float>* rootCf
 double GpuPostflopSolver::lastSolveDeviceMs() const { return impl_->lastMs; }
 
 // ---------------------------------------------------------------------------
-// Solution files: the reference's self-describing "PPS1" v1 layout
-// (cuda/gpu_cfr.cu:2686-2853), little-endian, byte-compatible. The tree is
-// recompiled from the stored spot + config and validated by the row count.
+// Solution files (solver/solution_file.h): the spot, the bet abstraction and
+// the rows in node order. Loading recompiles the tree from the stored spot
+// and configuration; the row count has to match it.
 // ---------------------------------------------------------------------------
-
-namespace {
-constexpr uint32_t kMagic = 0x31535050;  // "PPS1"
-constexpr uint32_t kVersion = 1;
-
-struct Writer {
-  FILE* f;
-  const std::string& path;
-  void raw(const void* p, size_t n) {
-    if (n && std::fwrite(p, 1, n, f) != n) throw std::runtime_error("write failed: " + path);
-  }
-  template <typename T>
-  void val(T v) {
-    raw(&v, sizeof(T));
-  }
-  void doubles(const std::vector<double>& v) {
-    val<int32_t>((int32_t)v.size());
-    raw(v.data(), v.size() * sizeof(double));
-  }
-};
-
-struct Reader {
-  FILE* f = nullptr;
-  std::string path;
-  explicit Reader(const std::string& p) : path(p) {
-    f = std::fopen(p.c_str(), "rb");
-    if (!f) throw std::runtime_error("cannot open solution file: " + p);
-  }
-  ~Reader() {
-    if (f) std::fclose(f);
-  }
-  void raw(void* p, size_t n) {
-    if (n && std::fread(p, 1, n, f) != n) throw std::runtime_error("solution file truncated: " + path);
-  }
-  template <typename T>
-  T val() {
-    T v;
-    raw(&v, sizeof(T));
-    return v;
-  }
-  std::vector<double> doubles() {
-    const int32_t n = val<int32_t>();
-    if (n < 0 || n > (1 << 24)) throw std::runtime_error("solution file corrupt: bad length: " + path);
-    std::vector<double> v((size_t)n);
-    raw(v.data(), v.size() * sizeof(double));
-    return v;
-  }
-};
-}  // namespace
 
 void GpuPostflopSolver::save(const std::string& path) const {
   const Impl* I = impl_;
-  if (I->T.empty) throw std::runtime_error("cannot save an empty solver");
+  if (I->T.empty) throw std::runtime_error("save: the spot compiled to an empty tree (no rows)");
   if (I->T.suitIso)
-    throw std::runtime_error("cannot save a suit-isomorphic solve: solution files (PPS1 v1) do not "
-                             "record the option, and its tree differs from the file's");
-  FILE* f = std::fopen(path.c_str(), "wb");
-  if (!f) throw std::runtime_error("cannot open for writing: " + path);
-  try {
-    Writer w{f, path};
-    const PostflopSpot& s = I->T.spot;
-    const pf::BetConfig& c = I->T.cfg;
-    w.val<uint32_t>(kMagic);
-    w.val<uint32_t>(kVersion);
-    w.val<int32_t>(s.nBoard);
-    w.raw(s.board, sizeof(s.board));
-    w.val<int64_t>(s.pot);
-    w.val<int64_t>(s.stack);
-    w.raw(s.oop.w, sizeof(s.oop.w));
-    w.raw(s.ip.w, sizeof(s.ip.w));
-    w.doubles(c.betFracs);
-    w.doubles(c.raiseMults);
-    w.val<int8_t>((int8_t)c.betAllIn);
-    w.val<int8_t>((int8_t)c.betGeometric);
-    w.val<int32_t>(c.maxRaises);
-    w.val<double>(c.addAllinThreshold);
-    w.val<double>(c.forceAllinThreshold);
-    w.val<double>(c.mergingThreshold);
-    w.val<int64_t>(I->T.numNodes);
-    w.val<int64_t>(totalIters_);
-    w.val<int32_t>((int32_t)I->lastAlgo.size());
-    w.raw(I->lastAlgo.data(), I->lastAlgo.size());
-    w.val<int64_t>(I->T.regTotal);
-    const std::vector<float> reg = I->readTable(PPSG_REGRET);
-    const std::vector<float> str = I->readTable(PPSG_STRAT);
-    w.raw(reg.data(), reg.size() * 4);
-    w.raw(str.data(), str.size() * 4);
-    if (std::fflush(f) != 0) throw std::runtime_error("write failed: " + path);
-  } catch (...) {
-    std::fclose(f);
-    throw;
-  }
-  std::fclose(f);
+    throw std::runtime_error(
+        "save: a suit-isomorphic solve has its own tree, and a solution file records no "
+        "isomorphism option (it would reload as the full tree)");
+  Solution s;
+  s.spot = I->T.spot;
+  s.cfg = I->T.cfg;
+  s.nodes = I->T.numNodes;
+  s.iterations = totalIters_;
+  s.algo = I->lastAlgo;
+  s.rows = I->T.regTotal;
+  s.regret = I->readTable(PPSG_REGRET);
+  s.strat = I->readTable(PPSG_STRAT);
+  writeSolution(path, s);
 }
 
 GpuPostflopSolver::GpuPostflopSolver(const std::string& solutionPath) : impl_(new Impl()) {
@@ -874,66 +842,29 @@ GpuPostflopSolver::GpuPostflopSolver(Deferred, const std::string& solutionPath) 
 }
 
 void GpuPostflopSolver::loadFrom(const std::string& solutionPath, bool device) {
-  {
-    Reader r(solutionPath);
-    if (r.val<uint32_t>() != kMagic) throw std::runtime_error("not a solution file: " + solutionPath);
-    const uint32_t ver = r.val<uint32_t>();
-    if (ver != kVersion)
-      throw std::runtime_error("solution file version " + std::to_string(ver) +
-                               " unsupported (expected " + std::to_string(kVersion) +
-                               "): " + solutionPath);
-    PostflopSpot spot;
-    spot.nBoard = r.val<int32_t>();
-    if (spot.nBoard < 3 || spot.nBoard > 5)
-      throw std::runtime_error("solution file corrupt: bad board size");
-    r.raw(spot.board, sizeof(spot.board));
-    spot.pot = r.val<int64_t>();
-    spot.stack = r.val<int64_t>();
-    r.raw(spot.oop.w, sizeof(spot.oop.w));
-    r.raw(spot.ip.w, sizeof(spot.ip.w));
-    pf::BetConfig cfg;
-    cfg.betFracs = r.doubles();
-    cfg.raiseMults = r.doubles();
-    cfg.betAllIn = r.val<int8_t>() != 0;
-    cfg.betGeometric = r.val<int8_t>() != 0;
-    cfg.maxRaises = r.val<int32_t>();
-    cfg.addAllinThreshold = r.val<double>();
-    cfg.forceAllinThreshold = r.val<double>();
-    cfg.mergingThreshold = r.val<double>();
-    (void)r.val<int64_t>();  // node count (the row count validates the tree)
-    const int64_t savedIters = r.val<int64_t>();
-    const int32_t algoLen = r.val<int32_t>();
-    if (algoLen < 0 || algoLen > 15) throw std::runtime_error("solution file corrupt: bad algo length");
-    std::string algo((size_t)algoLen, '\0');
-    r.raw(algo.data(), (size_t)algoLen);
-    if (!knownAlgo(algo)) throw std::runtime_error("solution file corrupt: unknown algo '" + algo + "'");
-    init(spot, cfg, device);
-    Impl* I = impl_;
-    if (I->T.empty)
-      throw std::runtime_error(
-          "solution spot compiled to an empty solver (empty or fully board-blocked range)");
-    const int64_t regTotal = r.val<int64_t>();
-    if (regTotal != I->T.regTotal)
-      throw std::runtime_error("solution file does not match this engine's compiled tree (row table " +
-                               std::to_string(regTotal) + " vs " + std::to_string(I->T.regTotal) +
-                               " — different engine version or spot semantics)");
-    if (savedIters < 0 || savedIters > std::numeric_limits<int>::max())
-      throw std::runtime_error("solution file corrupt: bad iteration count");
-    std::vector<float> reg((size_t)regTotal), str((size_t)regTotal);
-    r.raw(reg.data(), reg.size() * 4);
-    r.raw(str.data(), str.size() * 4);
-    if (device) {
-      I->writeTable(PPSG_REGRET, reg.data());
-      I->writeTable(PPSG_STRAT, str.data());
-      I->setCounter(savedIters);
-    } else {
-      I->pendingReg = std::move(reg);
-      I->pendingStr = std::move(str);
-      I->pendingIters = savedIters;
-    }
-    I->lastAlgo = algo;
-    totalIters_ = savedIters;
+  Solution s = readSolution(solutionPath);
+  init(s.spot, s.cfg, device);
+  Impl* I = impl_;
+  if (I->T.empty)
+    throw std::runtime_error("load '" + solutionPath +
+                             "': its spot compiles to an empty tree (a range is empty or "
+                             "blocked by the board)");
+  if (s.rows != I->T.regTotal)
+    throw std::runtime_error("load '" + solutionPath + "': " + std::to_string(s.rows) +
+                             " row entries, but the tree compiled from its spot has " +
+                             std::to_string(I->T.regTotal) +
+                             " (written by a tree builder with other rules or bet abstraction)");
+  if (device) {
+    I->writeTable(PPSG_REGRET, s.regret.data());
+    I->writeTable(PPSG_STRAT, s.strat.data());
+    I->setCounter(s.iterations);
+  } else {
+    I->pendingReg = std::move(s.regret);
+    I->pendingStr = std::move(s.strat);
+    I->pendingIters = s.iterations;
   }
+  I->lastAlgo = s.algo;
+  totalIters_ = s.iterations;
 }
 
 void GpuPostflopSolver::debugDump() {

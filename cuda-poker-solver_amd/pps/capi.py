@@ -72,12 +72,13 @@ CAPI_SYMBOLS = [
     "pps_solver_create_ex", "pps_server_add_ex", "pps_compile_info_ex",
     "pps_server_stats", "pps_solver_start_solve", "pps_solver_start_continue", "pps_solver_job",
     "pps_solver_cancel",
-    "pps_batch_profile",
+    "pps_batch_profile", "pps_solution_inspect",
 ]
 PPSG_SYMBOLS = [
     "ppsg_create", "ppsg_destroy", "ppsg_set_schedule", "ppsg_set_counter", "ppsg_get_counter",
     "ppsg_reset", "ppsg_iterate", "ppsg_half_step", "ppsg_walk", "ppsg_read", "ppsg_write",
-    "ppsg_last_iterate_ms", "ppsg_info", "ppsg_profile", "ppsg_copy_probe", "ppsg_last_error",
+    "ppsg_last_iterate_ms", "ppsg_info", "ppsg_profile", "ppsg_profile_depths",
+    "ppsg_copy_probe", "ppsg_last_error",
     "ppsg_footprint", "ppsg_batch_create", "ppsg_batch_add", "ppsg_batch_remove",
     "ppsg_batch_set_active", "ppsg_batch_set_schedule", "ppsg_batch_set_counter",
     "ppsg_batch_get_counter", "ppsg_batch_reset", "ppsg_batch_step", "ppsg_batch_walk", "ppsg_batch_walk_only",
@@ -468,6 +469,26 @@ class Solver:
     @property
     def last_solve_ms(self):
         return float(lib().pps_solver_last_solve_ms(self.h))
+
+
+class SolutionInfo(ctypes.Structure):
+    _fields_ = [("n_board", ctypes.c_int32), ("board", ctypes.c_uint8 * 5),
+                ("pot", ctypes.c_int64), ("stack", ctypes.c_int64),
+                ("n_bets", ctypes.c_int32), ("n_raises", ctypes.c_int32),
+                ("nodes", ctypes.c_int64), ("iterations", ctypes.c_int64),
+                ("rows", ctypes.c_int64), ("algo", ctypes.c_char * 33)]
+
+
+def solution_info(path):
+    """What a solution file holds (pps_solution_inspect: host only, the rows
+    are checked to be present but not read)."""
+    out = SolutionInfo()
+    lib().pps_solution_inspect.argtypes = [ctypes.c_char_p, ctypes.POINTER(SolutionInfo)]
+    _check(lib().pps_solution_inspect(str(path).encode(), ctypes.byref(out)))
+    return {"n_board": out.n_board, "board": list(out.board)[:out.n_board], "pot": out.pot,
+            "stack": out.stack, "n_bets": out.n_bets, "n_raises": out.n_raises,
+            "nodes": out.nodes, "iterations": out.iterations, "rows": out.rows,
+            "algo": out.algo.decode()}
 
 
 def compile_footprint(spot, cfg):

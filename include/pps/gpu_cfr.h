@@ -205,6 +205,7 @@ class GpuPostflopSolver {
             const SolverOptions& opt = SolverOptions());
   void loadFrom(const std::string& path, bool device);
   void replayChunks(int iterations, double target);
+  void profileDepthsRun(int iterations);
 
   // Server hooks. beginSolve / beginContinue check arguments as solve() /
   // continueSolve() do and set the spot's rows and schedule; advanced(n)

@@ -198,6 +198,21 @@ int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, 
 /* Device bytes the compiled spot takes in an arena (host only). */
 int pps_compile_footprint(const pps_spot* spot, const pps_bet_config* cfg, int64_t* bytes);
 
+/* What a solution file (GpuPostflopSolver::save, PPS1 v1) holds, read and
+   validated on the host without loading its rows or touching a device; an
+   invalid file is an error naming the field (pps_last_error). No reference
+   counterpart (the reference reads files only to load them). */
+typedef struct pps_solution_info {
+  int32_t n_board;
+  uint8_t board[5];
+  int64_t pot, stack;
+  int32_t n_bets, n_raises;   /* bet and raise size list lengths */
+  int64_t nodes, iterations;  /* tree size and schedule index of the rows */
+  int64_t rows;               /* entries in each of the two row tables */
+  char algo[33];
+} pps_solution_info;
+int pps_solution_inspect(const char* path, pps_solution_info* out);
+
 /* ---- spot server (no reference counterpart as a call; it replaces the
    global solve lock of the reference's HTTP layer, python/pps_api.py:165,
    by batching) ----

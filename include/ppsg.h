@@ -141,6 +141,15 @@ int ppsg_info(ppsg_t* h, int64_t* out, int cap);
    (No reference counterpart; the reference's GPU_CFR_PROFILE mode plays
    this role, cuda/gpu_cfr.cu:1895-1987.) */
 int ppsg_profile(ppsg_t* h, int iters, double* out, int cap);
+/* Per-depth timing of `iters` CFR iterations run launch by launch (events on
+   the engine stream, one sync per launch, so launch-bound trees read slower
+   than from the graph; ratios are what to compare). out[0] = D (depths),
+   then per depth d: forward us for traverser 0, backward us for 0, forward
+   for 1, backward for 1 (out[1 + 4d ..]), then fold us for 0 and 1, showdown
+   us (both widths) for 0 and 1; sums over the iterations. cap >= 1 + 4D + 4.
+   Advances the counter like ppsg_iterate (these are real iterations).
+   Replaces the GPU_CFR_PROFILE timing of cuda/gpu_cfr.cu:1895-1987. */
+int ppsg_profile_depths(ppsg_t* h, int iters, double* out, int cap);
 /* Calibration for hardware byte counters (no reference counterpart): a
    plain device copy of `bytes` (multiple of 16), `reps` launches on the
    null stream; mean launch time out. */

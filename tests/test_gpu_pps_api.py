@@ -115,7 +115,7 @@ def test_save_load(solved):
             pps.Solver.load(os.path.join(d, "nonexistent.sol"))
         with open(path, "r+b") as f:  # wrong magic
             f.write(b"XXXX")
-        with pytest.raises(RuntimeError, match="not a solution file"):
+        with pytest.raises(RuntimeError, match="magic.*not a PPS1 solution file"):
             pps.Solver.load(path)
 
 
@@ -259,3 +259,55 @@ def test_cli_protocol():
     assert lines[4].startswith("EXPLOITABILITY ")
     err = out.stderr
     assert "gpu_cfr: nodes=" in err and "gpu_cfr: compile" in err and "iters in" in err
+
+
+def test_file_written_from_the_layout_loads(tmp_path):
+    """A file built here from the PPS1 v1 layout (test_solution_file_cpu.pps1)
+    loads into the product with its rows and iteration count, and the
+    product's own file reads back through pps_solution_inspect."""
+    from pps import capi
+    import spots
+    from test_solution_file_cpu import pps1
+    s = spots.tiny_dict("turn")
+    sp = capi.make_spot(s["board"], s["w0"], s["w1"], s["pot"], s["stack"])
+    cf = capi.make_config(s["bets"], s["raises"], s["allin"], False, s["max_raises"])
+    ci = capi.compile_info(sp, cf)
+    rng = np.random.default_rng(3)
+    rows = rng.uniform(-5, 5, 2 * ci["reg_total"]).astype("<f4")
+    rows[ci["reg_total"]:] = np.abs(rows[ci["reg_total"]:])
+    data = pps1(board=tuple(s["board"]), pot=s["pot"], stack=s["stack"], w0=s["w0"], w1=s["w1"],
+                bets=tuple(s["bets"]), raises=tuple(s["raises"]), allin=1, cap=s["max_raises"],
+                nodes=ci["nodes"], iterations=77, algo=b"dcfr", rows=rows,
+                n_rows=ci["reg_total"])
+    path = tmp_path / "layout.sol"
+    path.write_bytes(data)
+    g = capi.Solver(path=str(path))
+    assert g.total_iterations == 77
+    assert np.array_equal(g.rows(0), rows[:ci["reg_total"]])
+    assert np.array_equal(g.rows(1), rows[ci["reg_total"]:])
+    g.continue_solve(5)
+    assert g.total_iterations == 82
+    own = tmp_path / "own.sol"
+    g.save(str(own))
+    got = capi.solution_info(own)
+    assert got["iterations"] == 82 and got["nodes"] == ci["nodes"]
+    assert got["rows"] == ci["reg_total"] and got["algo"] == "dcfr"
+    assert got["board"] == list(s["board"]) and got["pot"] == s["pot"]
+
+
+def test_cli_profile_mode():
+    """GPU_CFR_PROFILE (cuda/gpu_cfr.cu:1895-1987): at most 64 iterations run
+    launch by launch, with per-depth timings and the serialized per-iteration
+    total on stderr; only those iterations count."""
+    out = _cli("postflop", BOARD, OOP, IP, "200", "500", "0.75,a", "2.5,3", "200",
+               env={"GPU_CFR_PROFILE": "1"})
+    err = out.stderr.splitlines()
+    depth_lines = [l for l in err if l.startswith("gpu_cfr: d")]
+    assert len(depth_lines) >= 7 and depth_lines[0].startswith("gpu_cfr: d0 ")
+    assert "decide=1 " in depth_lines[0] and " fwd " in depth_lines[0] and " bwd " in depth_lines[0]
+    assert any(l.startswith("gpu_cfr: terminals fold") for l in err)
+    per = [l for l in err if "per-iter (serialized)" in l]
+    assert per and per[0].endswith("over 64 iters")
+    assert "gpu_cfr: 64 iters in" in out.stderr
+    ev = [float(l.split()[2]) for l in out.stdout.splitlines() if l.startswith("EV ")]
+    assert abs(sum(ev) - 200) < 1e-2
