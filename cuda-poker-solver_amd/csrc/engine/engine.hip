@@ -1087,8 +1087,12 @@ struct Spot {
   Layout L;
   int64_t regTotal = 0, cfvTotal = 0, reachTotal[2] = {0, 0};
   int64_t rootCfv = 0;  // chunk-relative byte offset of the root cfv row
-  // work items per [traverser][0 forward, 1 backward][depth], chunk-relative
+  // work items per [traverser][0 forward, 1 backward][depth], chunk-relative,
+  // and the node each item serves (subtree walks select by node)
   std::vector<std::vector<Item>> items[2][2];
+  std::vector<std::vector<int32_t>> itemNode[2][2];
+  std::vector<int32_t> termNode;  // node of each terminal entry (group order)
+  int64_t nodes = 0;
   std::vector<TermGroup> groups[2];  // per traverser
   int64_t itemCount = 0;  // listed items (terminal items at the group size in use)
   double fwdBytes = 0.0, bwdBytes = 0.0;
@@ -1169,11 +1173,16 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
   S.rootCfv = L.cfv + 4 * t->cfv_off[0];
 
   const int D = t->max_depth + 1;
+  S.nodes = N;
   for (int tr = 0; tr < 2; ++tr)
-    for (int ps = 0; ps < 2; ++ps) S.items[tr][ps].assign(D, {});
+    for (int ps = 0; ps < 2; ++ps) {
+      S.items[tr][ps].assign(D, {});
+      S.itemNode[tr][ps].assign(D, {});
+    }
   auto tiles = [](int n, int r) { return (n + kWave * r - 1) / (kWave * r); };
   std::vector<int> groupOf[2] = {std::vector<int>((size_t)2 * nc, 0), std::vector<int>((size_t)2 * nc, 0)};  // (class, kind) -> group index + 1
   std::vector<std::vector<TermNode>> members[2];
+  std::vector<std::vector<int32_t>> memberNode[2];
   for (int64_t u = 0; u < N; ++u) {
     const int dep = t->depth[u], cl = t->cls[u], k = t->kind[u];
     int kb, ke;
@@ -1271,6 +1280,7 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
           g.nO = nO;
           S.groups[tr].push_back(g);
           members[tr].emplace_back();
+          memberNode[tr].emplace_back();
           slot = (int)S.groups[tr].size();
         }
         TermNode e{};
@@ -1281,7 +1291,11 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
         e.v1 = tv[1];
         e.v2 = tv[2];
         members[tr][slot - 1].push_back(e);
+        memberNode[tr][slot - 1].push_back((int32_t)u);
       }
+      // the items just added serve node u
+      S.itemNode[tr][0][dep].resize(fwd.size(), (int32_t)u);
+      S.itemNode[tr][1][dep].resize(bwd.size(), (int32_t)u);
     }
   }
   // terminal node entries, contiguous per group, after the rows
@@ -1290,6 +1304,7 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
       S.groups[tr][g].first = (int64_t)T.termNodes.size();
       S.groups[tr][g].count = (int)members[tr][g].size();
       T.termNodes.insert(T.termNodes.end(), members[tr][g].begin(), members[tr][g].end());
+      S.termNode.insert(S.termNode.end(), memberNode[tr][g].begin(), memberNode[tr][g].end());
     }
   L.termNodes = L.add((int64_t)(T.termNodes.size() * sizeof(TermNode)));
   S.bytes = L.bytes;
@@ -1462,6 +1477,8 @@ struct ppsg_engine {
   int64_t sumStageCap = 0;
   char* scratchDev = nullptr;  // this device's address of `scratch` (set at init)
   char* sumHost = nullptr;     // pinned host staging of the root sums (kSumStageBytes, set at init)
+  Item* walkBuf = nullptr;     // items of a subtree walk (grow-only)
+  int64_t walkCap = 0;
 
   Dev dev() const { return Dev{arena.base, ctl}; }
   bool valid(int slot) const {
@@ -1524,6 +1541,7 @@ ppsg_engine::~ppsg_engine() {
   if (ctl) (void)hipFree(ctl);
   if (sumStage) (void)hipFree(sumStage);
   if (sumHost) (void)hipHostFree(sumHost);
+  if (walkBuf) (void)hipFree(walkBuf);
   if (arena.base) (void)hipFree(arena.base);
 }
 
@@ -2134,6 +2152,136 @@ int walk(ppsg_t* h, int tr, int mode, int nSlots, const int* slots, float* const
   return 0;
 }
 
+int walkOnly(ppsg_t* h, int tr, int mode, int nSlots, const int* slots, float* const* rootCfv);
+
+// EV / BR walk of one spot restricted to a subtree: the forward items of
+// the listed ancestors and of nodes in [lo, hi), the subtree's terminal
+// nodes (each group's run of them, cut at the group size), the backward
+// items of [lo, hi). Node ids are pre-order, so [lo, hi) is the subtree of lo when
+// hi is one past its last descendant. Each node's arithmetic is that of the
+// full walk, so the subtree's rows come out the same bit for bit; rows
+// outside it are not touched except the children of the ancestors (their
+// reach).
+int walkRange(ppsg_t* h, int slot, int tr, int mode, int64_t lo, int64_t hi, const int64_t* path,
+              int nPath) {
+  if (!h) return fail("null handle");
+  if (!h->valid(slot)) return fail("walk range: no spot in slot " + std::to_string(slot));
+  if ((tr != 0 && tr != 1) || (mode != PPSG_MODE_EV && mode != PPSG_MODE_BR))
+    return fail("walk range: traverser 0 or 1, mode EV or BR");
+  const Spot& S = h->spots[slot];
+  if (lo < 0 || hi > S.nodes || lo >= hi) return fail("walk range: bad node range");
+  if (nPath < 0 || (nPath > 0 && !path)) return fail("null argument");
+  // a large part of the tree: the resident lists (this spot's items only)
+  // cost less than building and uploading a list of its size (on the large
+  // flop: whole-tree walk ~11 ms per traverser; a subtree list of 40 % of
+  // the nodes took ~100 ms to build and upload)
+  if (8 * (hi - lo) > S.nodes) {
+    float* none[1] = {nullptr};
+    return walkOnly(h, tr, mode, 1, &slot, none);
+  }
+  auto absolute = [&](Item it) {
+    it.a += S.off;
+    it.b += S.off;
+    it.c += S.off;
+    it.d += S.off;
+    it.f = slot;
+    return it;
+  };
+  // items and terminal entries are in node order (per depth, per group), so
+  // a node range is one run of each, found by bisection
+  auto run = [](const std::vector<int32_t>& ids, int64_t a, int64_t b) {
+    const auto x = std::lower_bound(ids.begin(), ids.end(), (int32_t)a);
+    const auto y = std::lower_bound(x, ids.end(), (int32_t)b);
+    return std::make_pair((size_t)(x - ids.begin()), (size_t)(y - ids.begin()));
+  };
+  struct Seg {
+    int64_t first, count;
+    int type;  // 0 forward, 1 fold, 2 showdown, 3 wide showdown, 4 backward
+    size_t lds;
+  };
+  std::vector<Item> all;
+  std::vector<Seg> segs;
+  const int D = S.maxDepth + 1;
+  for (int d = 0; d < D; ++d) {
+    const int64_t first = (int64_t)all.size();
+    const std::vector<Item>& v = S.items[tr][0][d];
+    const std::vector<int32_t>& ids = S.itemNode[tr][0][d];
+    for (int k = 0; k < nPath; ++k) {  // ancestors above the range come first in node order
+      if (path[k] >= lo) continue;
+      const auto r = run(ids, path[k], path[k] + 1);
+      for (size_t q = r.first; q < r.second; ++q) all.push_back(absolute(v[q]));
+    }
+    const auto r = run(ids, lo, hi);
+    for (size_t q = r.first; q < r.second; ++q) all.push_back(absolute(v[q]));
+    if ((int64_t)all.size() > first) segs.push_back({first, (int64_t)all.size() - first, 0, 0});
+  }
+  std::vector<Item> term[3];
+  for (const TermGroup& g : S.groups[tr]) {
+    const std::vector<int32_t> ids(S.termNode.begin() + g.first,
+                                   S.termNode.begin() + g.first + g.count);
+    const auto r = run(ids, lo, hi);
+    for (size_t s0 = r.first; s0 < r.second; s0 += (size_t)h->groupSize) {
+      Item it{};
+      it.kind = g.kind;
+      it.a = S.off + S.L.termNodes + (int64_t)sizeof(TermNode) * (g.first + (int64_t)s0);
+      it.b = S.off;
+      it.c = S.off + g.gather;
+      it.d = S.off + g.meta;
+      it.n = g.nT;
+      it.m = g.nO;
+      it.na = g.rows;
+      it.cnt = (int32_t)std::min<size_t>((size_t)h->groupSize, r.second - s0);
+      it.f = slot;
+      term[g.kind == kBwdFold ? 0 : g.wide ? 2 : 1].push_back(it);
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    if (term[k].empty()) continue;
+    size_t lds = 0;
+    for (const Item& it : term[k]) lds = std::max(lds, terminalLdsOf(it, k == 2 ? kWideBlock : kBlock));
+    segs.push_back({(int64_t)all.size(), (int64_t)term[k].size(), 1 + k, lds});
+    all.insert(all.end(), term[k].begin(), term[k].end());
+  }
+  for (int d = D - 1; d >= 0; --d) {
+    const int64_t first = (int64_t)all.size();
+    const std::vector<Item>& v = S.items[tr][1][d];
+    const auto r = run(S.itemNode[tr][1][d], lo, hi);
+    for (size_t q = r.first; q < r.second; ++q) all.push_back(absolute(v[q]));
+    if ((int64_t)all.size() > first) segs.push_back({first, (int64_t)all.size() - first, 4, 0});
+  }
+  if (all.empty()) return 0;
+  if ((int64_t)all.size() > h->walkCap) {
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (h->walkBuf) (void)hipFree(h->walkBuf);
+    h->walkBuf = nullptr;
+    h->walkCap = 0;
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, all.size() * sizeof(Item)));
+    h->walkBuf = static_cast<Item*>(p);
+    h->walkCap = (int64_t)all.size();
+  }
+  HIP_TRY(hipMemcpyAsync(h->walkBuf, all.data(), all.size() * sizeof(Item), hipMemcpyHostToDevice,
+                         h->stream));
+  const Dev dv = h->dev();
+  for (const Seg& sg : segs) {
+    const Item* buf = h->walkBuf + sg.first;
+    const int c = (int)sg.count;
+    const unsigned waves = (unsigned)((sg.count + kWavesPerBlock - 1) / kWavesPerBlock);
+    switch (sg.type) {
+      case 0: fwdKernel<<<waves, kBlock, 0, h->stream>>>(dv, buf, c, 1, -1, 0); break;
+      case 1: foldKernel<<<(unsigned)c, kBlock, sg.lds, h->stream>>>(dv, buf, 0); break;
+      case 2: showdownKernel<<<(unsigned)c, kBlock, sg.lds, h->stream>>>(dv, buf, 0); break;
+      case 3: showdownWideKernel<<<(unsigned)c, kWideBlock, sg.lds, h->stream>>>(dv, buf, 0); break;
+      default:
+        if (mode == PPSG_MODE_EV) bwdKernel<PPSG_MODE_EV><<<waves, kBlock, 0, h->stream>>>(dv, buf, c, -1, 0);
+        else bwdKernel<PPSG_MODE_BR><<<waves, kBlock, 0, h->stream>>>(dv, buf, c, -1, 0);
+    }
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
 // EV / BR half-step of the listed spots only (live or parked): every other
 // spot's items are masked for the walk, the live set's flags restored after.
 int walkOnly(ppsg_t* h, int tr, int mode, int nSlots, const int* slots, float* const* rootCfv) {
@@ -2588,6 +2736,11 @@ int ppsg_batch_walk(ppsg_t* h, int tr, int mode, int n_slots, const int* slots,
 int ppsg_batch_walk_only(ppsg_t* h, int tr, int mode, int n_slots, const int* slots,
                          float* const* root_cfv) {
   return walkOnly(h, tr, mode, n_slots, slots, root_cfv);
+}
+
+int ppsg_batch_walk_range(ppsg_t* h, int slot, int tr, int mode, int64_t lo, int64_t hi,
+                          const int64_t* path, int n_path) {
+  return walkRange(h, slot, tr, mode, lo, hi, path, n_path);
 }
 
 int ppsg_batch_root_sums(ppsg_t* h, int tr, int n_slots, const int* slots, double* sums) {

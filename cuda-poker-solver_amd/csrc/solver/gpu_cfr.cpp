@@ -56,6 +56,26 @@ struct GpuPostflopSolver::Impl {
   // rows read from a solution file before the spot is attached to an arena
   std::vector<float> pendingReg, pendingStr;
   int64_t pendingIters = -1;
+  std::vector<int32_t> subEnd;  // per node: one past its last descendant (pre-order ids)
+
+  // EV / BR walk of node u's subtree only (its ancestors' forward launches
+  // give it its reach); u's subtree rows equal a full walk's.
+  void walkSubtree(int tr, int mode, int32_t u) {
+    const int64_t N = T.numNodes;
+    if (subEnd.empty()) {
+      subEnd.resize((size_t)N);
+      for (int64_t v = N - 1; v >= 0; --v) {
+        int32_t e = (int32_t)v + 1;
+        const int64_t b = T.childBase[v], end = v + 1 < N ? T.childBase[v + 1] : (int64_t)T.childFlat.size();
+        for (int64_t k = b; k < end; ++k) e = std::max(e, subEnd[(size_t)T.childFlat[k]]);
+        subEnd[(size_t)v] = e;
+      }
+    }
+    std::vector<int64_t> path;
+    for (int32_t a = T.parent[u]; a >= 0; a = T.parent[a]) path.push_back(a);
+    check(ppsg_batch_walk_range(eng, slot, tr, mode, u, subEnd[(size_t)u], path.data(),
+                                (int)path.size()));
+  }
 
   ~Impl() {
     if (!eng) return;
@@ -554,7 +574,7 @@ NodeEv GpuPostflopSolver::nodeEv(int decideIdx) {
   out.actions = I->labels(u);
   out.actionEv.assign(na, {});
   for (int tr = 0; tr < 2; ++tr) {
-    I->walk(tr, PPSG_MODE_EV, nullptr);
+    I->walkSubtree(tr, PPSG_MODE_EV, u);
     NodeEv::Side& S = out.side[tr];
     const int cl = I->T.cls[u];
     const int nC = I->T.clsN[tr][cl];
@@ -660,7 +680,7 @@ RunoutSummary GpuPostflopSolver::runoutSummary(int decideIdx, int actionIdx) {
   out.decider = I->T.owner[fu];
   out.actions = I->labels(fu);
   for (int tr = 0; tr < 2; ++tr) {
-    I->walk(tr, PPSG_MODE_EV, nullptr);
+    I->walkSubtree(tr, PPSG_MODE_EV, ch);
     const std::vector<double>& wSelf = I->T.w[tr];
     for (int b = 0; b < nb; ++b) {
       const int bi = out.branches[b].nextDecide;

@@ -82,7 +82,8 @@ PPSG_SYMBOLS = [
     "ppsg_footprint", "ppsg_batch_create", "ppsg_batch_add", "ppsg_batch_remove",
     "ppsg_batch_set_active", "ppsg_batch_set_schedule", "ppsg_batch_set_counter",
     "ppsg_batch_get_counter", "ppsg_batch_reset", "ppsg_batch_step", "ppsg_batch_walk", "ppsg_batch_walk_only",
-    "ppsg_batch_read", "ppsg_batch_root_sums", "ppsg_batch_root_sums_noalloc",
+    "ppsg_batch_read", "ppsg_batch_walk_range", "ppsg_batch_root_sums",
+    "ppsg_batch_root_sums_noalloc",
     "ppsg_batch_write", "ppsg_batch_free_bytes", "ppsg_batch_compact",
     "ppsg_device_memory", "ppsg_tune", "ppsg_autotune", "ppsg_kind_bytes",
 ]

@@ -199,6 +199,17 @@ int ppsg_batch_walk(ppsg_t* h, int tr, int mode, int n_slots, const int* slots,
 int ppsg_batch_walk_only(ppsg_t* h, int tr, int mode, int n_slots, const int* slots,
                          float* const* root_cfv);
 int ppsg_batch_read(ppsg_t* h, int slot, int buf, int64_t off, int64_t n, float* host);
+/* EV or BR walk of one spot (live or parked) restricted to a subtree:
+   the forward launches of the `path` nodes (the ancestors whose reach
+   feeds it) and of the nodes in [lo, hi), its terminal nodes, and the
+   backward launches of [lo, hi). Node ids are pre-order, so the subtree of
+   node u is [u, one past its last descendant). Afterwards the cfv rows of
+   the subtree and the reach rows of its nodes hold exactly what a full walk
+   (ppsg_batch_walk) leaves there; other spots and other nodes' cfv rows
+   are not touched. Replaces the whole-tree walks behind the node EV and
+   runout queries (cuda/gpu_cfr.cu:2318-2684). */
+int ppsg_batch_walk_range(ppsg_t* h, int slot, int tr, int mode, int64_t lo, int64_t hi,
+                          const int64_t* path, int n_path);
 /* Root value sums of the listed spots (live or parked; a slot may repeat):
    sums[k] = sum over the spot's base list of w_tr[i] * root_cfv[i], in index
    order, in f64 with each multiply and add rounded on its own (the host

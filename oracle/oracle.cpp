@@ -457,6 +457,10 @@ struct orc_solver {
   int build(int cl, int64_t potBase, int64_t sc0, int64_t sc1, int actor, bool afterAllin,
             bool closed, int depth, int64_t pc0, int64_t pc1, int numBets);
   void cfr(int u, int tr, int mode, const float* reach, float* out, const float* coef);
+  void cfrNode(int u, int tr, int mode, const float* reach, float* out, const float* coef);
+  // node whose traverser values and opponent reach a walk records (-1: none)
+  int capNode = -1;
+  std::vector<float> capCfv, capReach;
   void halfStep(int tr, int mode, const float* coef, float* rootOut);
   double pairMass() const;
   double rootValue(int tr, int mode, float* rootCfv);
@@ -627,6 +631,16 @@ int orc_solver::build(int cl, int64_t potBase, int64_t sc0, int64_t sc1, int act
 // over the traverser's list at u.
 void orc_solver::cfr(int u, int tr, int mode, const float* reach, float* out,
                      const float* coef) {
+  cfrNode(u, tr, mode, reach, out, coef);
+  if (u == capNode) {  // one node: at most one thread gets here
+    const BoardClass& bc = cls[nodes[u].cls];
+    capReach.assign(reach, reach + bc.list[1 - tr].size());
+    capCfv.assign(out, out + bc.list[tr].size());
+  }
+}
+
+void orc_solver::cfrNode(int u, int tr, int mode, const float* reach, float* out,
+                         const float* coef) {
   Node& n = nodes[u];
   const BoardClass& bc = cls[n.cls];
   const int opp = 1 - tr;
@@ -1016,6 +1030,32 @@ static int aggregate(orc_solver* s, int node, double* out, int cap) {
 }
 
 int orc_root_strategy(orc_solver* s, double* out, int cap) { return aggregate(s, 0, out, cap); }
+
+int orc_node_values(orc_solver* s, int node, int tr, int mode, float* cfv, float* reach, int cap,
+                    int* n_tr, int* n_opp) {
+  if (node < 0 || node >= (int)s->nodes.size() || (tr != 0 && tr != 1)) return -1;
+  s->capNode = node;
+  s->rootValue(tr, mode, nullptr);
+  s->capNode = -1;
+  const int a = (int)s->capCfv.size(), b = (int)s->capReach.size();
+  if (a > cap || b > cap) return -1;
+  std::copy(s->capCfv.begin(), s->capCfv.end(), cfv);
+  std::copy(s->capReach.begin(), s->capReach.end(), reach);
+  *n_tr = a;
+  *n_opp = b;
+  return 0;
+}
+
+int orc_node_cards(const orc_solver* s, int node, int player, int* cards, int cap) {
+  if (node < 0 || node >= (int)s->nodes.size() || (player != 0 && player != 1)) return -1;
+  const std::vector<int>& lst = s->cls[s->nodes[node].cls].list[player];
+  if ((int)lst.size() * 2 > cap) return -1;
+  for (size_t i = 0; i < lst.size(); ++i) {
+    cards[2 * i] = s->side[player].a[lst[i]];
+    cards[2 * i + 1] = s->side[player].b[lst[i]];
+  }
+  return (int)lst.size();
+}
 
 int orc_node_strategy(orc_solver* s, int idx, double* out, int cap) {
   if (idx < 0 || idx >= (int)s->decide.size()) return -1;

@@ -81,6 +81,12 @@ int orc_stats(orc_solver* s, double out[4]);
 /* root value per walk: tr, mode 1=EV 2=BR, fills the root cfv (f32) */
 double orc_root_value(orc_solver* s, int tr, int mode, float* root_cfv);
 int orc_root_strategy(orc_solver* s, double* out, int cap);
+/* one walk (tr, mode 1=EV 2=BR) recording node's traverser values (its
+   list order) and the opponent reach arriving there; counts out */
+int orc_node_values(orc_solver* s, int node, int tr, int mode, float* cfv, float* reach, int cap,
+                    int* n_tr, int* n_opp);
+/* the cards of each entry of a player's list at node (2 per entry) */
+int orc_node_cards(const orc_solver* s, int node, int player, int* cards, int cap);
 int orc_node_strategy(orc_solver* s, int decide_idx, double* out, int cap);
 /* rows: which 0 = regret, 1 = strategy sum; concatenated over decide
    nodes in creation order, action-major per node */

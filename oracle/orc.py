@@ -199,6 +199,29 @@ class Oracle:
         n = lib().orc_root_strategy(self.h, out, 64)
         return [out[i] for i in range(n)]
 
+    def node_values(self, node, tr, mode):
+        """One walk (mode 1 = EV, 2 = BR) recording node's traverser values and
+        the opponent reach arriving there; with the cards of both lists."""
+        L = lib()
+        L.orc_node_values.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_float),
+                                      ctypes.POINTER(ctypes.c_float), ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.orc_node_cards.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+        cfv, reach = (ctypes.c_float * 1326)(), (ctypes.c_float * 1326)()
+        nt, no = ctypes.c_int(), ctypes.c_int()
+        if L.orc_node_values(self.h, node, tr, mode, cfv, reach, 1326, ctypes.byref(nt),
+                             ctypes.byref(no)) != 0:
+            raise IndexError(node)
+        cards = []
+        for p in (tr, 1 - tr):
+            buf = (ctypes.c_int * 2652)()
+            n = L.orc_node_cards(self.h, node, p, buf, 2652)
+            cards.append([(buf[2 * i], buf[2 * i + 1]) for i in range(n)])
+        return (np.frombuffer(cfv, dtype=np.float32)[:nt.value].copy(),
+                np.frombuffer(reach, dtype=np.float32)[:no.value].copy(), cards[0], cards[1])
+
     def node_strategy(self, idx):
         out = (ctypes.c_double * 64)()
         n = lib().orc_node_strategy(self.h, idx, out, 64)

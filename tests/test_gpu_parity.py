@@ -247,3 +247,51 @@ def test_detailed_profile_leaves_the_rows_as_they_were():
     b = gpu_solver(s)
     b.solve(50, "dcfr")
     assert np.array_equal(a.rows(0), b.rows(0)) and np.array_equal(a.rows(1), b.rows(1))
+
+
+def _pps_from_oracle_rows(s, o, tmp_path):
+    """The product solver loaded from a PPS1 file holding the oracle's
+    current rows (the file's row layout is the oracle's: decide nodes in
+    creation order, action-major)."""
+    import pps
+    from test_solution_file_cpu import pps1
+    reg, strat = o.rows(0), o.rows(1)
+    rows = np.concatenate([reg, strat]).astype("<f4")
+    data = pps1(board=tuple(s["board"]), pot=s["pot"], stack=s["stack"], w0=s["w0"], w1=s["w1"],
+                bets=tuple(s["bets"]), raises=tuple(s["raises"]), allin=1 if s["allin"] else 0,
+                cap=s["max_raises"], nodes=o.counts()["nodes"], iterations=25, algo=b"dcfr",
+                rows=rows, n_rows=len(reg))
+    path = tmp_path / "oracle_rows.sol"
+    path.write_bytes(data)
+    return pps.Solver.load(str(path))
+
+
+@pytest.mark.parametrize("name", ["turn", "river", "blocked", "flop"])
+def test_node_ev_matches_oracle_node_values(name, tmp_path):
+    """node_ev (walks restricted to the node's subtree) against the oracle's
+    own walk on the same rows: per traverser combo, the oracle's value at the
+    node divided by the opponent reach mass arriving there without shared
+    cards, on the root, node 1 and the first decide node past a deal."""
+    s = spots.tiny_dict(name)
+    o = orc.Oracle(spots.orc_spot(s))
+    o.solve(25, "dcfr")
+    g = _pps_from_oracle_rows(s, o, tmp_path)
+    assert g.num_nodes == o.counts()["nodes"] and g.total_iterations == 25
+    nodes = [0, 1]
+    deeper = [i for i in range(o.counts()["decide"])
+              if o.decide_meta(i)["n_board"] > len(s["board"])]
+    nodes += deeper[:1]
+    for idx in nodes:
+        ev = g.node_ev(idx)
+        node = o.decide_meta(idx)["node_id"]
+        for tr, side in ((0, "oop"), (1, "ip")):
+            cfv, reach, tc, oc = o.node_values(node, tr, 1)
+            mass = np.array([sum(r for r, (x, y) in zip(reach, oc) if not {x, y} & {a, b})
+                             for a, b in tc])
+            got = np.asarray(ev["sides"][side]["ev"])
+            assert len(got) == len(cfv), (name, idx, side)
+            live = mass > 0
+            want = cfv[live] / mass[live]
+            err = np.abs(got[live] - want) / np.maximum(1.0, np.abs(want))
+            assert np.all(err <= 1e-4), (name, idx, side, err.max())
+            assert np.allclose(ev["sides"][side]["mass"], mass, rtol=1e-5, atol=1e-6)
