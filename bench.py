@@ -307,6 +307,8 @@ def bench_batched(args, capi, configs):
     log(f"bench: profile {json.dumps(prof)}")
     roof = roofline_batch(prof)
     roof["traffic"] = counter_traffic(roof["kernel"], f"batched-turn-{len(spots)}")
+    if roof["traffic"]:
+        roof["traffic_over_alg"] = round(roof["traffic"] / roof["bytes_per_launch"], 3)
     # a plain copy on this GPU (4 GiB each way, one float4 per lane): the
     # streaming rate the HBM-bound kernels compare against
     copy_bytes = 4 << 30
