@@ -223,6 +223,21 @@ def roofline_of(prof, iters_key="iters"):
     }
 
 
+def shapes_summary(table):
+    """The autotuned per-depth launch shapes (combos per lane) as
+    {"traverser 0": {"fwd_decide": [...], "fwd_chance": [...], "bwd_decide": [...]}, ...}."""
+    if not table:
+        return None
+    depths = table[0]
+    out = {}
+    for tr in range(2):
+        rows = [table[1 + 3 * (tr * depths + d):4 + 3 * (tr * depths + d)] for d in range(depths)]
+        out[f"traverser {tr}"] = {"fwd_decide": [r[0] for r in rows],
+                                  "fwd_chance": [r[1] for r in rows],
+                                  "bwd_decide": [r[2] for r in rows]}
+    return out
+
+
 def serve_stats(res, rep):
     lat = sorted(r["retire_ms"] - r["arrival_ms"] for r in res if r["ok"])
     pick = lambda q: lat[min(len(lat) - 1, int(q * len(lat)))] if lat else None
@@ -261,6 +276,13 @@ def bench_batched(args, capi, configs):
         return 0
     if args.steps <= 0:
         raise SystemExit("--steps must be positive")
+
+    # the timed steps: K batched iterations of the full resident batch after
+    # W untimed ones, then per-launch timing over K more; the per-depth launch
+    # shapes are autotuned on this batch first and every run below uses them
+    prof = capi.batch_profile(S, C, args.steps, warmup=args.warmup)
+    log(f"bench: profile {json.dumps(prof)}")
+    opt["shapes"] = prof["shapes"]
 
     res, rep = capi.serve(S, C, "dcfr", None, **opt)
     bad = [r["error"] for r in res if not r["ok"]]
@@ -301,10 +323,6 @@ def bench_batched(args, capi, configs):
     if args.http_spots > 0:
         http = bench_http(spots[:args.http_spots], args.http_clients, MAX_ITERS)
 
-    # the timed steps: K batched iterations of the full resident batch after
-    # W untimed ones, then per-launch timing over K more
-    prof = capi.batch_profile(S, C, args.steps, warmup=args.warmup)
-    log(f"bench: profile {json.dumps(prof)}")
     roof = roofline_batch(prof)
     roof["traffic"] = counter_traffic(roof["kernel"], f"batched-turn-{len(spots)}")
     if roof["traffic"]:
@@ -354,6 +372,7 @@ def bench_batched(args, capi, configs):
             "serving": serving,
             "http": http,
             "single_spot": single_spot,
+            "launch_shapes": shapes_summary(prof["shapes"]),
             "resident_batch": {
                 "steps": args.steps, "warmup": args.warmup,
                 "graph_ms_per_batched_iter": round(prof["graph_ms_per_iter"], 4),

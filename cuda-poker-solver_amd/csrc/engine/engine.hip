@@ -38,6 +38,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -62,13 +63,19 @@ constexpr int kGatherHeader = 256;  // bytes: 53 int32 segment bounds + pad
 // fold: entries of a card's segment per lane (4 lanes per card)
 constexpr int kFoldPart = 12;
 constexpr int kMaxGroup = 64;  // terminal nodes per work item at most
-// combos per lane (kWave apart) of forward decide, backward decide and
-// forward chance items (chosen by measurement: more loads in flight per lane
-// help the read-mostly forward items; the read-modify-write backward tiles
-// do best with one combo per lane and more lanes busy)
+// Combos per lane (kWave apart) of forward decide, forward chance and
+// backward decide items: a per-depth launch shape. Spots are planned at one
+// combo per lane; a list built for a depth merges each node's tiles to the
+// depth's shape. Defaults (measured over the whole batched stream: more
+// loads in flight per lane help the read-mostly forward items, the
+// read-modify-write backward tiles do best with one combo per lane);
+// ppsg_autotune_shapes picks per depth from the candidates below.
 constexpr int kRDecide = 2;
 constexpr int kRBackward = 1;
 constexpr int kRChance = 4;
+constexpr int kRDecideCand[] = {1, 2, 4};
+constexpr int kRChanceCand[] = {2, 4, 8};
+constexpr int kRBackwardCand[] = {1, 2};
 
 thread_local std::string g_err;
 
@@ -247,28 +254,29 @@ __device__ __forceinline__ void fwdDecide(const Dev& d, const Item& it, int from
   }
 }
 
+template <int RD, int RC>
 __device__ __forceinline__ void fwdItem(const Dev& d, const Item& it, int fromStrat) {
   if (it.kind == kFwdChance) {
-    const int c0 = it.tile * kWave * kRChance + waveLane();
+    const int c0 = it.tile * kWave * RC + waveLane();
     const float* __restrict__ in = at<float>(d, it.b);
     const uint16_t* __restrict__ tp = at<uint16_t>(d, it.a);
     float* __restrict__ out = at<float>(d, it.c);
-    int p[kRChance];
+    int p[RC];
 #pragma unroll
-    for (int k = 0; k < kRChance; ++k) p[k] = c0 + k * kWave < it.n ? tp[c0 + k * kWave] : -1;
+    for (int k = 0; k < RC; ++k) p[k] = c0 + k * kWave < it.n ? tp[c0 + k * kWave] : -1;
 #pragma unroll
-    for (int k = 0; k < kRChance; ++k)
+    for (int k = 0; k < RC; ++k)
       if (p[k] >= 0) out[c0 + k * kWave] = in[p[k]] * it.v0;
     return;
   }
   switch (it.na) {
-    case 1: fwdDecide<1, kRDecide>(d, it, fromStrat); break;
-    case 2: fwdDecide<2, kRDecide>(d, it, fromStrat); break;
-    case 3: fwdDecide<3, kRDecide>(d, it, fromStrat); break;
-    case 4: fwdDecide<4, kRDecide>(d, it, fromStrat); break;
-    case 5: fwdDecide<5, kRDecide>(d, it, fromStrat); break;
-    case 6: fwdDecide<6, kRDecide>(d, it, fromStrat); break;
-    default: fwdDecide<0, kRDecide>(d, it, fromStrat); break;
+    case 1: fwdDecide<1, RD>(d, it, fromStrat); break;
+    case 2: fwdDecide<2, RD>(d, it, fromStrat); break;
+    case 3: fwdDecide<3, RD>(d, it, fromStrat); break;
+    case 4: fwdDecide<4, RD>(d, it, fromStrat); break;
+    case 5: fwdDecide<5, RD>(d, it, fromStrat); break;
+    case 6: fwdDecide<6, RD>(d, it, fromStrat); break;
+    default: fwdDecide<0, RD>(d, it, fromStrat); break;
   }
 }
 
@@ -627,6 +635,7 @@ __global__ __launch_bounds__(kWideBlock) void showdownWideKernel(Dev d,
 
 // Forward launch of one depth: opponent reach of the next depth.
 // kindMask: profiling attribution only (-1 = every kind in real launches).
+template <int RD, int RC>
 __global__ __launch_bounds__(kBlock) void fwdKernel(Dev d, const Item* __restrict__ items,
                                                     int count, int fromStrat, int kindMask,
                                                     int masked) {
@@ -635,7 +644,7 @@ __global__ __launch_bounds__(kBlock) void fwdKernel(Dev d, const Item* __restric
   if (w >= count) return;
   if (!((kindMask >> it.kind) & 1)) return;
   if (masked && !d.ctl[it.f].alive) return;
-  fwdItem(d, it, fromStrat);
+  fwdItem<RD, RC>(d, it, fromStrat);
 }
 
 // ---------------------------------------------------------------------------
@@ -645,8 +654,11 @@ __global__ __launch_bounds__(kBlock) void fwdKernel(Dev d, const Item* __restric
 // Decide tile: every lane handles up to R traverser combos kWave apart
 // (all row loads issued first). NA > 0: action count fixed at compile time
 // (values stay in registers); NA == 0: generic loop.
+// dry (CFR mode, timing only): the update is computed and stored, but the
+// stored values are the old ones, so the tile moves exactly the bytes of a
+// real update and changes no row.
 template <int MODE, int NA, int R>
-__device__ __forceinline__ void decideTile(const Dev& d, const Item& it) {
+__device__ __forceinline__ void decideTile(const Dev& d, const Item& it, int dry) {
   const int c0 = it.tile * kWave * R + waveLane();
   const int n = it.n;
   if (c0 >= n) return;
@@ -715,8 +727,10 @@ __device__ __forceinline__ void decideTile(const Dev& d, const Item& it) {
       if (MODE != PPSG_MODE_CFR) continue;
 #pragma unroll
       for (int a = 0; a < A; ++a) {
-        st[(int64_t)a * n + c] = sm[k][a] * avg + s[a];
-        src[(int64_t)a * n + c] = x[k][a] * (x[k][a] >= 0.0f ? pos : neg) + (v[k][a] - val);
+        const float ns = sm[k][a] * avg + s[a];
+        const float nr = x[k][a] * (x[k][a] >= 0.0f ? pos : neg) + (v[k][a] - val);
+        st[(int64_t)a * n + c] = dry ? sm[k][a] : ns;
+        src[(int64_t)a * n + c] = dry ? x[k][a] : nr;
       }
     }
     return;
@@ -739,8 +753,9 @@ __device__ __forceinline__ void decideTile(const Dev& d, const Item& it) {
     for (int a = 0; a < na; ++a) {
       const float x = src[(int64_t)a * n + c];
       const float sg = sigmaOf(x, acc, uniform, plus);
-      st[(int64_t)a * n + c] = st[(int64_t)a * n + c] * avg + sg;
-      src[(int64_t)a * n + c] = x * (x >= 0.0f ? pos : neg) + (kid[a * stride + c] - val);
+      const float so = st[(int64_t)a * n + c];
+      st[(int64_t)a * n + c] = dry ? so : so * avg + sg;
+      src[(int64_t)a * n + c] = dry ? x : x * (x >= 0.0f ? pos : neg) + (kid[a * stride + c] - val);
     }
   }
 }
@@ -787,9 +802,9 @@ __device__ __forceinline__ void chanceTile(const Dev& d, const Item& it) {
   at<float>(d, it.b)[i] = (float)v;
 }
 
-template <int MODE>
+template <int MODE, int RB>
 __global__ __launch_bounds__(kBlock) void bwdKernel(Dev d, const Item* __restrict__ items,
-                                                    int count, int kindMask, int masked) {
+                                                    int count, int kindMask, int masked, int dry) {
   const int w = waveItem();
   const Item it = fetchItem(d, items, w, count);
   if (w >= count) return;
@@ -800,13 +815,13 @@ __global__ __launch_bounds__(kBlock) void bwdKernel(Dev d, const Item* __restric
     return;
   }
   switch (it.na) {
-    case 1: decideTile<MODE, 1, kRBackward>(d, it); break;
-    case 2: decideTile<MODE, 2, kRBackward>(d, it); break;
-    case 3: decideTile<MODE, 3, kRBackward>(d, it); break;
-    case 4: decideTile<MODE, 4, kRBackward>(d, it); break;
-    case 5: decideTile<MODE, 5, kRBackward>(d, it); break;
-    case 6: decideTile<MODE, 6, kRBackward>(d, it); break;
-    default: decideTile<MODE, 0, kRBackward>(d, it); break;
+    case 1: decideTile<MODE, 1, RB>(d, it, dry); break;
+    case 2: decideTile<MODE, 2, RB>(d, it, dry); break;
+    case 3: decideTile<MODE, 3, RB>(d, it, dry); break;
+    case 4: decideTile<MODE, 4, RB>(d, it, dry); break;
+    case 5: decideTile<MODE, 5, RB>(d, it, dry); break;
+    case 6: decideTile<MODE, 6, RB>(d, it, dry); break;
+    default: decideTile<MODE, 0, RB>(d, it, dry); break;
   }
 }
 
@@ -1196,10 +1211,10 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
         const int na = t->na[u];
         const int first = t->child_flat[kb];
         if (t->owner[u] == op) {
-          for (int q = 0; q < tiles(nO, kRDecide); ++q) {
+          for (int q = 0; q < tiles(nO, 1); ++q) {
             Item it{};
             it.kind = kFwdDecide;
-            it.r = kRDecide;
+            it.r = 1;
             it.a = L.regret + 4 * t->reg_off[u];
             it.d = L.strat + 4 * t->reg_off[u];
             it.b = L.reach[tr] + 4 * t->reach_off[tr][u];
@@ -1211,11 +1226,10 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
           }
         }
         const int64_t stride = na > 1 ? t->cfv_off[t->child_flat[kb + 1]] - t->cfv_off[first] : 0;
-        const int rb = kRBackward;  // must match bwdKernel's dispatch
-        for (int q = 0; q < tiles(nT, rb); ++q) {
+        for (int q = 0; q < tiles(nT, 1); ++q) {
           Item it{};
           it.kind = kBwdDecide;
-          it.r = (uint8_t)rb;
+          it.r = 1;
           it.a = L.regret + 4 * t->reg_off[u];
           it.d = L.strat + 4 * t->reg_off[u];
           it.b = L.cfv + 4 * t->cfv_off[u];
@@ -1233,10 +1247,10 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
         for (int b = kb; b < ke; ++b) {
           const int ch = t->child_flat[b];
           const int ccl = t->cls[ch];
-          for (int q = 0; q < tiles(t->cls_n[op][ccl], kRChance); ++q) {
+          for (int q = 0; q < tiles(t->cls_n[op][ccl], 1); ++q) {
             Item it{};
             it.kind = kFwdChance;
-            it.r = kRChance;
+            it.r = 1;
             it.a = L.toParent[op] + 2 * t->cls_list_off[op][ccl];
             it.b = L.reach[tr] + 4 * t->reach_off[tr][u];
             it.c = L.reach[tr] + 4 * t->reach_off[tr][ch];
@@ -1357,6 +1371,51 @@ struct DevList {
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Launches at a given shape (combos per lane): forward decide RD in {1, 2,
+// 4}, forward chance RC in {1, 2, 4, 8}, backward decide RB in {1, 2}.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+template <int RD>
+void launchFwdRC(int rc, unsigned g, hipStream_t s, Dev d, const Item* buf, int count,
+                 int fromStrat, int kindMask, int masked) {
+  switch (rc) {
+    case 1: fwdKernel<RD, 1><<<g, kBlock, 0, s>>>(d, buf, count, fromStrat, kindMask, masked); break;
+    case 2: fwdKernel<RD, 2><<<g, kBlock, 0, s>>>(d, buf, count, fromStrat, kindMask, masked); break;
+    case 8: fwdKernel<RD, 8><<<g, kBlock, 0, s>>>(d, buf, count, fromStrat, kindMask, masked); break;
+    default: fwdKernel<RD, 4><<<g, kBlock, 0, s>>>(d, buf, count, fromStrat, kindMask, masked); break;
+  }
+}
+
+void launchFwdShape(int rd, int rc, hipStream_t s, Dev d, const Item* buf, int64_t count,
+                    int fromStrat, int kindMask, int masked) {
+  const unsigned g = (unsigned)((count + kWavesPerBlock - 1) / kWavesPerBlock);
+  const int c = (int)count;
+  if (rd == 1) launchFwdRC<1>(rc, g, s, d, buf, c, fromStrat, kindMask, masked);
+  else if (rd == 4) launchFwdRC<4>(rc, g, s, d, buf, c, fromStrat, kindMask, masked);
+  else launchFwdRC<2>(rc, g, s, d, buf, c, fromStrat, kindMask, masked);
+}
+
+template <int MODE>
+void launchBwdRB(int rb, unsigned g, hipStream_t s, Dev d, const Item* buf, int count,
+                 int kindMask, int masked, int dry) {
+  if (rb == 2) bwdKernel<MODE, 2><<<g, kBlock, 0, s>>>(d, buf, count, kindMask, masked, dry);
+  else bwdKernel<MODE, 1><<<g, kBlock, 0, s>>>(d, buf, count, kindMask, masked, dry);
+}
+
+void launchBwdShape(int mode, int rb, hipStream_t s, Dev d, const Item* buf, int64_t count,
+                    int kindMask, int masked, int dry) {
+  const unsigned g = (unsigned)((count + kWavesPerBlock - 1) / kWavesPerBlock);
+  const int c = (int)count;
+  if (mode == PPSG_MODE_CFR) launchBwdRB<PPSG_MODE_CFR>(rb, g, s, d, buf, c, kindMask, masked, dry);
+  else if (mode == PPSG_MODE_EV) launchBwdRB<PPSG_MODE_EV>(rb, g, s, d, buf, c, kindMask, masked, 0);
+  else launchBwdRB<PPSG_MODE_BR>(rb, g, s, d, buf, c, kindMask, masked, 0);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // Root value sums: per listed spot, sum_i w[i] * cfv_root[i] over its base
 // list for one traverser, in index order, in f64 with separately rounded
 // multiplies and adds (the host loop's arithmetic, bit for bit). One 64-lane
@@ -1454,6 +1513,25 @@ struct ppsg_engine {
   // groups, 4 wide showdown groups (one list each)]
   std::vector<DevList> lists[2][kLists];
   int groupSize = 4;  // terminal nodes per work item
+  // per-depth launch shapes (combos per lane), [traverser][depth]: forward
+  // decide, forward chance, backward decide
+  std::vector<uint8_t> shapeFD[2], shapeFC[2], shapeB[2];
+  int shapeOf(int tr, int dep, uint8_t kind) const {
+    const std::vector<uint8_t>* v = kind == kFwdDecide   ? &shapeFD[tr]
+                                    : kind == kFwdChance ? &shapeFC[tr]
+                                    : kind == kBwdDecide ? &shapeB[tr]
+                                                         : nullptr;
+    if (!v) return 1;  // backward chance: one combo per lane
+    if (dep < (int)v->size()) return (*v)[dep];
+    return kind == kFwdDecide ? kRDecide : kind == kFwdChance ? kRChance : kRBackward;
+  }
+  void ensureShapes(int D) {
+    for (int tr = 0; tr < 2; ++tr) {
+      if ((int)shapeFD[tr].size() < D) shapeFD[tr].resize(D, kRDecide);
+      if ((int)shapeFC[tr].size() < D) shapeFC[tr].resize(D, kRChance);
+      if ((int)shapeB[tr].size() < D) shapeB[tr].resize(D, kRBackward);
+    }
+  }
   // device compaction scratch (grow-only)
   CompactJob* cjobs = nullptr;
   int64_t cjobsCap = 0;
@@ -1499,11 +1577,12 @@ struct ppsg_engine {
   int appendSpot(int slot);
   int compact();
   int rebuild();
+  int rebuildList(int tr, int ps, int dep);
   int prepare();
   void dropGraph();
   int launchFwd(int tr, int dpt, int fromStrat, int kindMask, hipStream_t s);
   int launchTerm(int tr, int ps, hipStream_t s);
-  int launchBwd(int tr, int dpt, int mode, int kindMask, hipStream_t s);
+  int launchBwd(int tr, int dpt, int mode, int kindMask, hipStream_t s, int dry = 0);
   int launchHalf(int tr, int mode, hipStream_t s);
   int capture();
   int covers(int64_t iters);
@@ -1596,13 +1675,20 @@ std::vector<Item> ppsg_engine::spotItems(int slot, int tr, int ps, int dep) cons
   std::vector<Item> v;
   if (ps < 2) {
     if (dep > S.maxDepth) return v;
-    v = S.items[tr][ps][dep];
-    for (Item& it : v) {
+    // planned at one combo per lane: merge each node's tiles to the depth's
+    // shape (tile q at shape r covers planned tiles q*r .. q*r + r - 1)
+    for (const Item& p : S.items[tr][ps][dep]) {
+      const int r = shapeOf(tr, dep, p.kind);
+      if (p.tile % r) continue;
+      Item it = p;
+      it.tile = (uint8_t)(p.tile / r);
+      it.r = (uint8_t)r;
       it.a += S.off;
       it.b += S.off;
       it.c += S.off;
       it.d += S.off;
       it.f = slot;
+      v.push_back(it);
     }
     return v;
   }
@@ -1636,7 +1722,9 @@ int64_t ppsg_engine::countItems(int slot) const {
   int64_t n = 0;
   for (int tr = 0; tr < 2; ++tr) {
     for (int ps = 0; ps < 2; ++ps)
-      for (const auto& v : S.items[tr][ps]) n += (int64_t)v.size();
+      for (size_t dep = 0; dep < S.items[tr][ps].size(); ++dep)
+        for (const Item& it : S.items[tr][ps][dep])
+          n += it.tile % shapeOf(tr, (int)dep, it.kind) == 0;
     for (const TermGroup& g : S.groups[tr]) n += (g.count + groupSize - 1) / groupSize;
   }
   return n;
@@ -1671,6 +1759,7 @@ int ppsg_engine::appendSpot(int slot) {
   Spot& S = spots[slot];
   const int D = S.maxDepth + 1;
   dropGraph();
+  ensureShapes(D);
   for (int tr = 0; tr < 2; ++tr)
     for (int ps = 0; ps < kLists; ++ps) {
       std::vector<DevList>& L = lists[tr][ps];
@@ -1772,6 +1861,21 @@ int ppsg_engine::compact() {
 
 // Rebuilds every list from the live spots' items (after a launch-shape
 // change) and frees removed spots.
+// One merged list from the live spots' items at the current shapes (the
+// lists hold no dead items when this runs).
+int ppsg_engine::rebuildList(int tr, int ps, int dep) {
+  dropGraph();
+  std::vector<Item> v;
+  for (int s = 0; s < maxSpots; ++s) {
+    if (spots[s].state != kLive) continue;
+    const std::vector<Item> w = spotItems(s, tr, ps, dep);
+    v.insert(v.end(), w.begin(), w.end());
+  }
+  DevList& l = lists[tr][ps][dep];
+  l.lds = 0;
+  return uploadTo(l, v, false, kBlock);
+}
+
 int ppsg_engine::rebuild() {
   dropGraph();
   listedItems = 0;
@@ -1792,6 +1896,7 @@ int ppsg_engine::rebuild() {
       any = true;
     }
   }
+  ensureShapes(D);
   for (int tr = 0; tr < 2; ++tr)
     for (int ps = 0; ps < kLists; ++ps) {
       std::vector<DevList>& L = lists[tr][ps];
@@ -1975,8 +2080,8 @@ int64_t ppsg_engine::kernelsPerIter() const {
 int ppsg_engine::launchFwd(int tr, int dpt, int fromStrat, int kindMask, hipStream_t s) {
   const DevList& l = lists[tr][0][dpt];
   if (l.count == 0) return 0;
-  const unsigned g = (unsigned)((l.count + kWavesPerBlock - 1) / kWavesPerBlock);
-  fwdKernel<<<g, kBlock, 0, s>>>(dev(), l.buf, (int)l.count, fromStrat, kindMask, masked());
+  launchFwdShape(shapeOf(tr, dpt, kFwdDecide), shapeOf(tr, dpt, kFwdChance), s, dev(), l.buf,
+                 l.count, fromStrat, kindMask, masked());
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -1992,14 +2097,11 @@ int ppsg_engine::launchTerm(int tr, int ps, hipStream_t s) {
   return 0;
 }
 
-int ppsg_engine::launchBwd(int tr, int dpt, int mode, int kindMask, hipStream_t s) {
+int ppsg_engine::launchBwd(int tr, int dpt, int mode, int kindMask, hipStream_t s, int dry) {
   const DevList& l = lists[tr][1][dpt];
   if (l.count == 0) return 0;
-  const unsigned g = (unsigned)((l.count + kWavesPerBlock - 1) / kWavesPerBlock);
-  const int c = (int)l.count, mk = masked();
-  if (mode == PPSG_MODE_CFR) bwdKernel<PPSG_MODE_CFR><<<g, kBlock, 0, s>>>(dev(), l.buf, c, kindMask, mk);
-  else if (mode == PPSG_MODE_EV) bwdKernel<PPSG_MODE_EV><<<g, kBlock, 0, s>>>(dev(), l.buf, c, kindMask, mk);
-  else bwdKernel<PPSG_MODE_BR><<<g, kBlock, 0, s>>>(dev(), l.buf, c, kindMask, mk);
+  launchBwdShape(mode, shapeOf(tr, dpt, kBwdDecide), s, dev(), l.buf, l.count, kindMask, masked(),
+                 dry);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -2266,15 +2368,12 @@ int walkRange(ppsg_t* h, int slot, int tr, int mode, int64_t lo, int64_t hi, con
   for (const Seg& sg : segs) {
     const Item* buf = h->walkBuf + sg.first;
     const int c = (int)sg.count;
-    const unsigned waves = (unsigned)((sg.count + kWavesPerBlock - 1) / kWavesPerBlock);
     switch (sg.type) {
-      case 0: fwdKernel<<<waves, kBlock, 0, h->stream>>>(dv, buf, c, 1, -1, 0); break;
+      case 0: launchFwdShape(1, 1, h->stream, dv, buf, sg.count, 1, -1, 0); break;
       case 1: foldKernel<<<(unsigned)c, kBlock, sg.lds, h->stream>>>(dv, buf, 0); break;
       case 2: showdownKernel<<<(unsigned)c, kBlock, sg.lds, h->stream>>>(dv, buf, 0); break;
       case 3: showdownWideKernel<<<(unsigned)c, kWideBlock, sg.lds, h->stream>>>(dv, buf, 0); break;
-      default:
-        if (mode == PPSG_MODE_EV) bwdKernel<PPSG_MODE_EV><<<waves, kBlock, 0, h->stream>>>(dv, buf, c, -1, 0);
-        else bwdKernel<PPSG_MODE_BR><<<waves, kBlock, 0, h->stream>>>(dv, buf, c, -1, 0);
+      default: launchBwdShape(mode, 1, h->stream, dv, buf, sg.count, -1, 0, 0);
     }
     HIP_TRY(hipGetLastError());
   }
@@ -2841,6 +2940,121 @@ int ppsg_autotune(ppsg_t* h, const int* candidates, int n, int reps, int* chosen
   if (ppsg_tune(h, PPSG_TUNE_GROUP, best) != 0) return -1;
   if (chosen) *chosen = best;
   return 0;
+}
+
+int ppsg_autotune_shapes(ppsg_t* h, int reps, int* table, int cap) {
+  if (!h) return fail("null handle");
+  if (reps <= 0) return fail("autotune: reps must be positive");
+  if (h->prepare() != 0) return -1;
+  if (h->live.empty()) return fail("no live spot");
+  if (h->deadItems > 0 && h->compact() != 0) return -1;
+  const int D = h->depthCount();
+  if (table && cap < 1 + 6 * D) return fail("autotune: the table needs " + std::to_string(1 + 6 * D) + " entries");
+  h->ensureShapes(D);
+  hipEvent_t a = nullptr, b = nullptr;
+  HIP_TRY(hipEventCreate(&a));
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return fail("autotune: event creation failed");
+  }
+  auto mean = [&](auto&& launch) -> double {  // one untimed launch, then reps timed
+    if (launch() != 0) return -1.0;
+    if (hipEventRecord(a, h->stream) != hipSuccess) return -1.0;
+    for (int r = 0; r < reps; ++r)
+      if (launch() != 0) return -1.0;
+    if (hipEventRecord(b, h->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess) return -1.0;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return -1.0;
+    return ms / reps;
+  };
+  // picks the fastest candidate for one shape of one list; the forward
+  // launches rewrite reach rows only, the backward ones run dry (CFR
+  // traffic, no row changed), so timing leaves every spot as it was
+  auto pick = [&](std::vector<uint8_t>& shape, int tr, int ps, int d, const int* cand, int nc,
+                  const std::function<int()>& launch) -> int {
+    double best = -1.0;
+    int bestR = shape[d];
+    for (int k = 0; k < nc; ++k) {
+      shape[d] = (uint8_t)cand[k];
+      if (h->rebuildList(tr, ps, d) != 0) return -1;
+      const double t = mean(launch);
+      if (t < 0.0) return fail("autotune: timed launch failed");
+      if (best < 0.0 || t < best) {
+        best = t;
+        bestR = cand[k];
+      }
+    }
+    shape[d] = (uint8_t)bestR;
+    return h->rebuildList(tr, ps, d);
+  };
+  int rc = 0;
+  for (int tr = 0; tr < 2 && rc == 0; ++tr)
+    for (int d = 0; d < D && rc == 0; ++d) {
+      bool hasDec = false, hasCh = false, hasBwd = false;
+      for (int s : h->live) {
+        const Spot& S = h->spots[s];
+        if (d > S.maxDepth) continue;
+        for (const Item& it : S.items[tr][0][d]) (it.kind == kFwdDecide ? hasDec : hasCh) = true;
+        for (const Item& it : S.items[tr][1][d]) hasBwd = hasBwd || it.kind == kBwdDecide;
+      }
+      if (hasDec)
+        rc = pick(h->shapeFD[tr], tr, 0, d, kRDecideCand, 3,
+                  [&] { return h->launchFwd(tr, d, 0, 1 << kFwdDecide, h->stream); });
+      if (rc == 0 && hasCh)
+        rc = pick(h->shapeFC[tr], tr, 0, d, kRChanceCand, 3,
+                  [&] { return h->launchFwd(tr, d, 0, 1 << kFwdChance, h->stream); });
+      if (rc == 0 && hasBwd)
+        rc = pick(h->shapeB[tr], tr, 1, d, kRBackwardCand, 2, [&] {
+          return h->launchBwd(tr, d, PPSG_MODE_CFR, 1 << kBwdDecide, h->stream, 1);
+        });
+    }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (rc != 0) return rc;
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->listedItems = 0;
+  for (int s : h->live) {
+    h->spots[s].itemCount = h->countItems(s);
+    h->listedItems += h->spots[s].itemCount;
+  }
+  if (table) {
+    table[0] = D;
+    for (int tr = 0; tr < 2; ++tr)
+      for (int d = 0; d < D; ++d) {
+        int* e = table + 1 + 3 * (tr * D + d);
+        e[0] = h->shapeFD[tr][d];
+        e[1] = h->shapeFC[tr][d];
+        e[2] = h->shapeB[tr][d];
+      }
+  }
+  return 0;
+}
+
+int ppsg_set_shapes(ppsg_t* h, const int* table, int n) {
+  if (!h || !table) return fail("null argument");
+  const int D = n >= 1 ? table[0] : -1;
+  if (D < 0 || D > 1024 || n < 1 + 6 * D) return fail("set shapes: malformed table");
+  auto in = [](int r, const int* cand, int nc) {
+    for (int k = 0; k < nc; ++k)
+      if (cand[k] == r) return true;
+    return false;
+  };
+  h->ensureShapes(D);
+  for (int tr = 0; tr < 2; ++tr)
+    for (int d = 0; d < D; ++d) {
+      const int* e = table + 1 + 3 * (tr * D + d);
+      if (!in(e[0], kRDecideCand, 3) || !in(e[1], kRChanceCand, 3) || !in(e[2], kRBackwardCand, 2))
+        return fail("set shapes: a shape outside the candidates (decide 1/2/4, chance 2/4/8, "
+                    "backward 1/2)");
+    }
+  for (int tr = 0; tr < 2; ++tr)
+    for (int d = 0; d < D; ++d) {
+      const int* e = table + 1 + 3 * (tr * D + d);
+      h->shapeFD[tr][d] = (uint8_t)e[0];
+      h->shapeFC[tr][d] = (uint8_t)e[1];
+      h->shapeB[tr][d] = (uint8_t)e[2];
+    }
+  return h->rebuild();
 }
 
 int ppsg_device_memory(int64_t* free_bytes, int64_t* total_bytes) {

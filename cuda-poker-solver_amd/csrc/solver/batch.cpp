@@ -111,6 +111,8 @@ std::vector<SpotResult> serve(const std::vector<SpotRequest>& reqs, const ServeO
   }
   Engine E;
   check(ppsg_batch_create(arena, opt.maxSpots, &E.h), "batch create");
+  if (!opt.shapes.empty())
+    check(ppsg_set_shapes(E.h, opt.shapes.data(), (int)opt.shapes.size()), "launch shapes");
   ppsg_t* h = E.h;
   // Terminal group size: a batch admitted up front shares each launch among
   // many spots, so larger groups (fewer, longer workgroups) pay off; a
@@ -401,6 +403,9 @@ BatchProfile profileBatch(const std::vector<SpotRequest>& reqs, const std::strin
   P.tuneMs.assign(kNumGroupCandidates, 0.0);
   check(ppsg_autotune(E.h, kGroupCandidates, kNumGroupCandidates, 2, &P.groupSize, P.tuneMs.data()),
         "profile: autotune");
+  P.shapes.assign(1 + 6 * 1024, 0);
+  check(ppsg_autotune_shapes(E.h, 3, P.shapes.data(), (int)P.shapes.size()), "profile: shapes");
+  P.shapes.resize(1 + 6 * (size_t)P.shapes[0]);
   if (warmup > 0) check(ppsg_batch_step(E.h, warmup), "profile: warmup");
   check(ppsg_batch_step(E.h, iters), "profile: replay");
   double ms = 0.0;

@@ -486,6 +486,8 @@ void pps_serve_defaults(pps_serve_options* opt) {
   opt->real_time = d.realTime ? 1 : 0;
   opt->prefill = d.prefill ? 1 : 0;
   opt->flags = 0;
+  opt->shapes = nullptr;
+  opt->n_shapes = 0;
 }
 
 int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const double* arrival_ms,
@@ -507,6 +509,7 @@ int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const do
       o.compileThreads = opt->compile_threads;
       o.realTime = opt->real_time != 0;
       o.prefill = opt->prefill != 0;
+      if (opt->shapes && opt->n_shapes > 0) o.shapes.assign(opt->shapes, opt->shapes + opt->n_shapes);
     }
     if (algo) o.algo = algo;
     std::vector<pps::gpu::SpotRequest> reqs(n);
@@ -561,7 +564,8 @@ int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const do
 }
 
 int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, const char* algo,
-                      int iters, int warmup, int64_t arena_bytes, int detail, double out[33]) {
+                      int iters, int warmup, int64_t arena_bytes, int detail, double out[33],
+                      int32_t* shapes, int32_t shapes_cap) {
   return guard([&] {
     if (n <= 0 || !spots || !cfgs || !out) throw std::runtime_error("null argument");
     std::vector<pps::gpu::SpotRequest> reqs(n);
@@ -584,6 +588,12 @@ int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, 
     for (int k = 0; k < 6; ++k) out[25 + k] = P.kindBytes[k];
     out[31] = P.foldLaunches;
     out[32] = P.showdownLaunches;
+    if (shapes) {
+      if ((size_t)shapes_cap < P.shapes.size())
+        throw std::runtime_error("profile: the shape table needs " + std::to_string(P.shapes.size()) +
+                                 " entries");
+      std::copy(P.shapes.begin(), P.shapes.end(), shapes);
+    }
     return 0;
   });
 }

@@ -31,7 +31,8 @@ class ServeOptions(ctypes.Structure):
                 ("max_iters", ctypes.c_int32), ("arena_bytes", ctypes.c_int64),
                 ("max_spots", ctypes.c_int32), ("max_live", ctypes.c_int32),
                 ("compile_threads", ctypes.c_int32), ("real_time", ctypes.c_int32),
-                ("prefill", ctypes.c_int32), ("flags", ctypes.c_int32)]
+                ("prefill", ctypes.c_int32), ("flags", ctypes.c_int32),
+                ("shapes", ctypes.POINTER(ctypes.c_int32)), ("n_shapes", ctypes.c_int32)]
 
 
 class ServeResult(ctypes.Structure):
@@ -85,7 +86,8 @@ PPSG_SYMBOLS = [
     "ppsg_batch_read", "ppsg_batch_walk_range", "ppsg_batch_root_sums",
     "ppsg_batch_root_sums_noalloc",
     "ppsg_batch_write", "ppsg_batch_free_bytes", "ppsg_batch_compact",
-    "ppsg_device_memory", "ppsg_tune", "ppsg_autotune", "ppsg_kind_bytes",
+    "ppsg_device_memory", "ppsg_tune", "ppsg_autotune", "ppsg_autotune_shapes",
+    "ppsg_set_shapes", "ppsg_kind_bytes",
 ]
 
 _lib = None
@@ -155,7 +157,7 @@ def lib():
         L.pps_compile_footprint.argtypes = [P(Spot), P(BetConfig), P(ctypes.c_int64)]
         L.pps_batch_profile.argtypes = [ctypes.c_int, P(Spot), P(BetConfig), ctypes.c_char_p,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int,
-                                        P(ctypes.c_double)]
+                                        P(ctypes.c_double), P(ctypes.c_int32), ctypes.c_int32]
         L.pps_server_create.argtypes = [P(ServerOptions), P(ctypes.c_void_p)]
         L.pps_server_destroy.restype = None
         L.pps_server_destroy.argtypes = [ctypes.c_void_p]
@@ -513,8 +515,14 @@ def serve(spots, cfgs, algo="dcfr", arrival_ms=None, **options):
     if len(cfgs) != n:
         raise ValueError("one bet config per spot")
     o = serve_defaults()
+    shapes = options.pop("shapes", None)
+    keep = None
+    if shapes:
+        keep = (ctypes.c_int32 * len(shapes))(*shapes)
+        o.shapes = ctypes.cast(keep, ctypes.POINTER(ctypes.c_int32))
+        o.n_shapes = len(shapes)
     for k, v in options.items():
-        if not hasattr(o, k):
+        if not hasattr(o, k) or k in ("shapes", "n_shapes"):
             raise ValueError(f"unknown serve option {k}")
         setattr(o, k, v)
     S = (Spot * n)(*spots)
@@ -523,6 +531,7 @@ def serve(spots, cfgs, algo="dcfr", arrival_ms=None, **options):
     R = (ServeResult * n)()
     rep = ServeReport()
     _check(lib().pps_serve(n, S, C, A, algo.encode(), ctypes.byref(o), R, ctypes.byref(rep)))
+    del keep
     out = []
     for r in R:
         out.append({
@@ -545,9 +554,11 @@ def batch_profile(spots, cfgs, iters, algo="dcfr", arena_bytes=0, detail=False, 
     S = (Spot * n)(*spots)
     C = (BetConfig * n)(*cfgs)
     out = (ctypes.c_double * 33)()
+    cap = 1 + 6 * 1024
+    shapes = (ctypes.c_int32 * cap)()
     _check(lib().pps_batch_profile(n, S, C, algo.encode(), int(iters), int(warmup),
                                    int(arena_bytes),
-                                   1 if detail else 0, out))
+                                   1 if detail else 0, out, shapes, cap))
     keys = ["spots", "iters", "graph_ms_per_iter", "fwd_ms", "fwd_launches", "bwd_ms",
             "bwd_launches", "fwd_bytes", "bwd_bytes", "kernels_per_iter", "items_per_iter",
             "arena_bytes", "decide_ms", "chance_ms", "fold_ms", "showdown_ms", "group_size"]
@@ -559,6 +570,7 @@ def batch_profile(spots, cfgs, iters, algo="dcfr", arena_bytes=0, detail=False, 
                                 "showdown"], list(out)[25:31]))
     d["fold_launches"], d["showdown_launches"] = out[31], out[32]
     d["term_launches"] = out[31] + out[32]
+    d["shapes"] = list(shapes[:1 + 6 * shapes[0]])
     if not detail:
         for k in ("decide_ms", "chance_ms", "fwd_decide_ms", "fwd_chance_ms"):
             del d[k]

@@ -52,6 +52,9 @@ struct ServeOptions {
   int compileThreads = 8;
   bool realTime = true;       // wait for arrivalMs (false: all arrive at 0)
   bool prefill = false;       // admit every request that fits before the first step
+  // per-depth launch shapes for the engine (a ppsg_autotune_shapes table);
+  // empty: the defaults
+  std::vector<int> shapes;
 };
 
 struct ServeReport {
@@ -101,6 +104,8 @@ struct BatchProfile {
   // per both-traverser walk) of each candidate in kGroupCandidates order
   int groupSize = 0;
   std::vector<double> tuneMs;
+  // per-depth launch shapes the autotuner chose (ppsg_autotune_shapes table)
+  std::vector<int> shapes;
 };
 
 // Terminal group sizes the autotuner tries.

@@ -147,6 +147,9 @@ typedef struct pps_serve_options {
   int32_t real_time;        /* 1: honour arrival_ms; 0: all arrive at once */
   int32_t prefill;          /* 1: admit all that fit before the first step */
   int32_t flags;            /* option flags of every request (PPS_SUIT_ISOMORPHISM) */
+  const int32_t* shapes;    /* per-depth launch shapes (a ppsg_autotune_shapes
+                               table, e.g. from pps_batch_profile) or NULL */
+  int32_t n_shapes;         /* entries in shapes */
 } pps_serve_options;
 
 typedef struct pps_serve_result {
@@ -192,9 +195,13 @@ int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const do
    (detail) forward decide and forward chance ms per iteration run alone,
    the algorithmic bytes per batched iteration of forward decide, forward
    chance, backward decide, backward chance, fold, showdown, and the fold
-   and showdown launch counts over the timed iterations]. */
+   and showdown launch counts over the timed iterations]. The per-depth
+   launch shapes are autotuned first (ppsg_autotune_shapes); their table goes
+   to `shapes` (if not NULL and it holds shapes_cap >= 1 + 6 * depths
+   entries) for pps_serve_options.shapes. */
 int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, const char* algo,
-                      int iters, int warmup, int64_t arena_bytes, int detail, double out[33]);
+                      int iters, int warmup, int64_t arena_bytes, int detail, double out[33],
+                      int32_t* shapes, int32_t shapes_cap);
 /* Device bytes the compiled spot takes in an arena (host only). */
 int pps_compile_footprint(const pps_spot* spot, const pps_bet_config* cfg, int64_t* bytes);
 

@@ -253,6 +253,19 @@ int ppsg_tune(ppsg_t* h, int param, int value);
    traversers each, which change no rows), keeps the fastest; *chosen and
    ms[i] (per rep) may be NULL. */
 int ppsg_autotune(ppsg_t* h, const int* candidates, int n, int reps, int* chosen, double* ms);
+/* Per-depth launch shapes (combos per lane of forward decide items from
+   {1, 2, 4}, forward chance items from {2, 4, 8}, backward decide items from
+   {1, 2}), picked for the live spots by timing each depth's launch of each
+   kind at every candidate: the forward launches rewrite reach rows only and
+   the backward ones run a dry update (the CFR update's loads and stores,
+   writing the old values back), so no spot's rows change. table (if not
+   NULL, cap >= 1 + 6 * depths) receives [depths, then per traverser and
+   depth: decide, chance, backward]. Keeps the chosen shapes. No reference
+   counterpart (the reference launches one block per node). */
+int ppsg_autotune_shapes(ppsg_t* h, int reps, int* table, int cap);
+/* Sets the shapes from such a table (n entries; depths beyond it keep the
+   defaults 2 / 4 / 1) and rebuilds the work lists. */
+int ppsg_set_shapes(ppsg_t* h, const int* table, int n);
 /* Free and total memory of the current device (for sizing an arena). */
 int ppsg_device_memory(int64_t* free_bytes, int64_t* total_bytes);
 

@@ -127,3 +127,20 @@ def test_admission_by_footprint():
     ref = [serve([s], **opt)[0][0] for s in S[:2]]
     same(res[0], ref[0])
     same(res[1], ref[1])
+
+
+def test_tuned_launch_shapes_give_the_same_results():
+    """A batch profiled with autotuned per-depth shapes, then served with
+    that table, retires every spot at the iteration and with the values it
+    has at the default shapes."""
+    S = mix()
+    pairs = [spot_cfg(s) for s in S]
+    prof = capi.batch_profile([p[0] for p in pairs], [p[1] for p in pairs], 2)
+    assert prof["shapes"][0] >= 7 and len(prof["shapes"]) == 1 + 6 * prof["shapes"][0]
+    opt = dict(target_frac=0.005, check_every=16, max_iters=512)
+    base = serve(S, **opt)[0]
+    tuned = serve(S, shapes=prof["shapes"], **opt)[0]
+    for x, y in zip(base, tuned):
+        assert x["ok"] and y["ok"]
+        assert (x["iterations"], x["ev0"], x["ev1"], x["expl"]) == \
+            (y["iterations"], y["ev0"], y["ev1"], y["expl"])

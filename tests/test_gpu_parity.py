@@ -295,3 +295,45 @@ def test_node_ev_matches_oracle_node_values(name, tmp_path):
             err = np.abs(got[live] - want) / np.maximum(1.0, np.abs(want))
             assert np.all(err <= 1e-4), (name, idx, side, err.max())
             assert np.allclose(ev["sides"][side]["mass"], mass, rtol=1e-5, atol=1e-6)
+
+
+def test_shape_autotuner_changes_no_rows_and_no_results():
+    """ppsg_autotune_shapes times every depth's launches at each candidate
+    shape (forward launches rewrite reach only, backward ones run a dry
+    update): the rows stay bitwise the same, and a solve continued at the
+    tuned shapes equals one at the default shapes (the shape only decides
+    which lane takes which combo)."""
+    import ctypes
+    L = capi.lib()
+    L.pps_solver_engine.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                    ctypes.POINTER(ctypes.c_int)]
+    L.ppsg_autotune_shapes.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    L.ppsg_set_shapes.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    s = configs.large_turn()
+    a = gpu_solver(s)
+    a.solve(30, "dcfr")
+    r0, r1 = a.rows(0), a.rows(1)
+    eng, slot = ctypes.c_void_p(), ctypes.c_int()
+    assert L.pps_solver_engine(a.h, ctypes.byref(eng), ctypes.byref(slot)) == 0
+    table = (ctypes.c_int * 1024)()
+    assert L.ppsg_autotune_shapes(eng, 2, table, 1024) == 0
+    depths = table[0]
+    assert depths >= 8
+    shapes = [table[1 + k] for k in range(6 * depths)]
+    assert set(shapes[0::3]) <= {1, 2, 4} and set(shapes[1::3]) <= {2, 4, 8}
+    assert set(shapes[2::3]) <= {1, 2}
+    assert np.array_equal(a.rows(0), r0) and np.array_equal(a.rows(1), r1)
+    a.continue_solve(20)
+    b = gpu_solver(s)
+    b.solve(50, "dcfr")
+    assert np.array_equal(a.rows(0), b.rows(0)) and np.array_equal(a.rows(1), b.rows(1))
+    # every candidate shape at every depth: still the same rows
+    for rd, rc, rb in ((1, 2, 2), (4, 8, 1)):
+        t = [depths] + [rd, rc, rb] * (2 * depths)
+        arr = (ctypes.c_int * len(t))(*t)
+        assert L.ppsg_set_shapes(eng, arr, len(t)) == 0
+        c = gpu_solver(s)
+        c.solve(50, "dcfr")
+        a.solve(50, "dcfr")
+        assert np.array_equal(a.rows(0), c.rows(0)) and np.array_equal(a.rows(1), c.rows(1))
