@@ -417,13 +417,21 @@ __device__ void foldGroup(const Dev& d, const Item& it, char* lds) {
 
 // Lane-permuted copy of a double within a wave through DPP moves (VALU, no
 // LDS traffic). CTRL is the DPP control: quad_perm (0x00-0xff), row_shr:n
-// (0x110 + n), row_bcast:15 (0x142), row_bcast:31 (0x143); lanes without a
-// source, or in rows outside ROW_MASK, read 0.
+// (0x110 + n), row_bcast:15 (0x142), row_bcast:31 (0x143). With every row
+// enabled, lanes whose source lies outside their row read 0 (the bound
+// control bit), so no zeroed register has to be prepared; lanes in rows
+// outside ROW_MASK are not written and keep the 0 they start from.
 template <int CTRL, int ROW_MASK = 0xf>
 __device__ __forceinline__ double dppD(double x) {
   const uint64_t u = __builtin_bit_cast(uint64_t, x);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROW_MASK, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROW_MASK, 0xf, false);
+  int lo, hi;
+  if constexpr (ROW_MASK == 0xf) {
+    lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)u, CTRL, 0xf, 0xf, true);
+    hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), CTRL, 0xf, 0xf, true);
+  } else {
+    lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROW_MASK, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROW_MASK, 0xf, false);
+  }
   return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint64_t)(uint32_t)lo);
 }
 
@@ -469,7 +477,8 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
   __shared__ double waveTot[NT / 64];
   __shared__ float segTot[52];
   __shared__ TermNode grp[kMaxGroup];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar tests
   const int nO = it.m, nT = it.n, cnt = it.cnt;
   float* raw = reinterpret_cast<float*>(lds);
   double* Xs = reinterpret_cast<double*>(lds + sdRawBytes(nO));
@@ -535,7 +544,11 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
     }
     __syncthreads();
     double run = inc - own;
-    for (int w = 0; w < wave; ++w) run += waveTot[w];
+#pragma unroll
+    for (int w = 0; w < NT / 64 - 1; ++w) {
+      const double t = waveTot[w];
+      run += w < wave ? t : 0.0;  // +0.0 past this wave: the same sum, no loop
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       Xs[k * NT + tid] = run;
@@ -563,7 +576,10 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
     // 4. per traverser combo
     auto xs = [&](int p) -> double { return Xs[(p & 3) * NT + (p >> 2)]; };
     pend = reinterpret_cast<float*>(d.arena + it.b + nd.out);
-    const double vWin = nd.v0, vTie = nd.v1, vLose = nd.v2;
+    // value = vWin win + vTie (leq - win) + vLose (all - leq)
+    //       = kWin win + kLeq leq + vLose all
+    const double vLose = nd.v2;
+    const double kWin = (double)nd.v0 - (double)nd.v1, kLeq = (double)nd.v1 - vLose;
     const double total = xs(nO);
 #pragma unroll
     for (int q = 0; q < ROWS; ++q) {
@@ -582,16 +598,14 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
       const float t1 = segTot[c1], t2 = segTot[c2];
       __builtin_amdgcn_sched_group_barrier(0x0100, 9, 0);  // LDS reads
       __builtin_amdgcn_sched_group_barrier(0x0002, 64, 0);  // then VALU
+      // each card pair is summed in f32 first: the sum is commutative, so a
+      // combo and its suit-swapped twin get the same bits whichever card
+      // comes first; masses below 0 (rounding) count as 0
       const double rs = (double)fs;
-      double win = xl - (double)y1 - (double)y2;
-      const double leq = xt - (double)y3 - (double)y4 + rs;
-      const double all = total - (double)t1 - (double)t2 + rs;
-      if (win < 0) win = 0;
-      double tie = leq - win;
-      if (tie < 0) tie = 0;
-      double lose = all - leq;
-      if (lose < 0) lose = 0;
-      res[q] = (float)(win * vWin + tie * vTie + lose * vLose);
+      const double win = fmax(xl - (double)(y1 + y2), 0.0);
+      const double leq = fmax(xt - (double)(y3 + y4) + rs, 0.0);
+      const double all = fmax(total - (double)(t1 + t2) + rs, 0.0);
+      res[q] = (float)(kWin * win + (kLeq * leq + vLose * all));
     }
   };
   float rv[ROWS];
