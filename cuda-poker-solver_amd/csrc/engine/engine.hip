@@ -115,16 +115,17 @@ enum ItemKind : uint8_t {
 
 // One work item; a, b, c, d are byte offsets into the arena; f is the
 // owning spot's slot (its control block).
-//   kFwdDecide   a regret row, d strategy row, b node reach row, c children's
-//                reach block (stride n), n opponent combos, na actions
+//   kFwdDecide   a regret row, d strategy row (action rows ld apart), b node
+//                reach row, c children's reach block, m child stride
+//                (floats), n opponent combos, na actions
 //   kFwdChance   a to-parent map (u16), b parent reach row, c child reach
 //                row, n child opponent combos, v0 = 1/(52-board-4)
 //   (decide and chance items are one wave's work: combos [tile * kWave * r,
 //   +kWave * r), lane t taking t, t + kWave, ... so that r loads per row
 //   are in flight)
-//   kBwdDecide   a regret row, d strategy row, b node cfv row, c children's
-//                cfv block, m child stride (floats), n traverser combos,
-//                e = actor is the traverser
+//   kBwdDecide   a regret row, d strategy row (action rows ld apart), b node
+//                cfv row, c children's cfv block, m child stride (floats), n
+//                traverser combos, e = actor is the traverser
 //   kBwdChance   a dealt-card x parent-slot map (i16: child slot or -1), b
 //                node cfv row, c children's cfv block, d the child serving
 //                each dealt card (u8), m child stride, n traverser combos
@@ -146,7 +147,7 @@ struct alignas(16) Item {
   uint8_t r;     // decide / chance: combos per lane (kWave apart)
   float v0;
   int32_t cnt;   // terminal groups: nodes in the group
-  int32_t pad;
+  int32_t ld;    // decide: distance between action rows (floats)
 };
 static_assert(sizeof(Item) == 64, "work item must be one 64-byte line");
 
@@ -209,11 +210,17 @@ __device__ __forceinline__ float sigmaOf(float x, float acc, float uniform, bool
 // Decide (opponent acts): child reach a = reach * sigma_a. NA > 0: action
 // count fixed at compile time (the row values stay in registers); every
 // lane handles up to R combos kWave apart, all loads issued first.
+// Tiles are whole: a row of n combos is padded to PPSG_ROW_STRIDE(n), a
+// multiple of kWave, and the lanes past n load and store its padding (no
+// one reads it back), so every store covers whole cache lines. A store
+// that covers part of a line costs far more than its share: a launch
+// whose nodes end in part-filled tiles runs ~15 % slower.
 template <int NA, int R>
 __device__ __forceinline__ void fwdDecide(const Dev& d, const Item& it, int fromStrat) {
   const int c0 = it.tile * kWave * R + waveLane();
-  if (c0 >= it.n) return;
-  const int n = it.n, na = NA > 0 ? NA : it.na;
+  const int n = (int)PPSG_ROW_STRIDE(it.n), na = NA > 0 ? NA : it.na;
+  if (c0 >= n) return;
+  const int64_t ld = it.ld, cs = it.m;
   const float* __restrict__ reach = at<float>(d, it.b);
   const float* __restrict__ src = at<float>(d, fromStrat ? it.d : it.a);
   float* __restrict__ out = at<float>(d, it.c);
@@ -226,7 +233,7 @@ __device__ __forceinline__ void fwdDecide(const Dev& d, const Item& it, int from
       const int c = c0 + k * kWave;
       r[k] = c < n ? reach[c] : 0.0f;
 #pragma unroll
-      for (int a = 0; a < NA; ++a) x[k][a] = c < n ? src[(int64_t)a * n + c] : 0.0f;
+      for (int a = 0; a < NA; ++a) x[k][a] = c < n ? src[a * ld + c] : 0.0f;
     }
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -236,7 +243,7 @@ __device__ __forceinline__ void fwdDecide(const Dev& d, const Item& it, int from
 #pragma unroll
       for (int a = 0; a < NA; ++a) acc += plus ? (x[k][a] > 0.0f ? x[k][a] : 0.0f) : x[k][a];
 #pragma unroll
-      for (int a = 0; a < NA; ++a) out[(int64_t)a * n + c] = r[k] * sigmaOf(x[k][a], acc, uniform, plus);
+      for (int a = 0; a < NA; ++a) out[a * cs + c] = r[k] * sigmaOf(x[k][a], acc, uniform, plus);
     }
     return;
   }
@@ -246,11 +253,10 @@ __device__ __forceinline__ void fwdDecide(const Dev& d, const Item& it, int from
     const float rr = reach[c];
     float acc = 0.0f;
     for (int a = 0; a < na; ++a) {
-      const float x = src[(int64_t)a * n + c];
+      const float x = src[a * ld + c];
       acc += plus ? (x > 0.0f ? x : 0.0f) : x;
     }
-    for (int a = 0; a < na; ++a)
-      out[(int64_t)a * n + c] = rr * sigmaOf(src[(int64_t)a * n + c], acc, uniform, plus);
+    for (int a = 0; a < na; ++a) out[a * cs + c] = rr * sigmaOf(src[a * ld + c], acc, uniform, plus);
   }
 }
 
@@ -261,12 +267,14 @@ __device__ __forceinline__ void fwdItem(const Dev& d, const Item& it, int fromSt
     const float* __restrict__ in = at<float>(d, it.b);
     const uint16_t* __restrict__ tp = at<uint16_t>(d, it.a);
     float* __restrict__ out = at<float>(d, it.c);
+    // whole tiles: zeros into the row's padding
+    const int n = it.n, nw = (int)PPSG_ROW_STRIDE(n);
     int p[RC];
 #pragma unroll
-    for (int k = 0; k < RC; ++k) p[k] = c0 + k * kWave < it.n ? tp[c0 + k * kWave] : -1;
+    for (int k = 0; k < RC; ++k) p[k] = c0 + k * kWave < n ? tp[c0 + k * kWave] : -1;
 #pragma unroll
     for (int k = 0; k < RC; ++k)
-      if (p[k] >= 0) out[c0 + k * kWave] = in[p[k]] * it.v0;
+      if (c0 + k * kWave < nw) out[c0 + k * kWave] = p[k] >= 0 ? in[p[k]] * it.v0 : 0.0f;
     return;
   }
   switch (it.na) {
@@ -321,15 +329,18 @@ __device__ __forceinline__ const TermNode* stageNodes(TermNode* grp, const TermN
 
 // A node's results, stored once the next node's reach is staged: its
 // stores are then older than the next reach request, so the wait for that
-// reach (which cannot count conditional stores and waits for all vector
-// memory operations) never waits for them.
+// reach (which waits for all vector memory operations) never waits for
+// them. The row is padded to PPSG_ROW_STRIDE(nT) floats, a multiple of 64,
+// so waveBase + q * NT < padT holds for a whole wave or for none of it:
+// waves past the padded row skip with a scalar test, and the lanes in the
+// padding store values that nothing reads, so every store covers whole
+// cache lines and none needs an exec mask of its own.
 template <int ROWS, int NT>
-__device__ __forceinline__ void storeRows(float* out, const float (&res)[ROWS], int nT) {
+__device__ __forceinline__ void storePadded(float* out, const float (&res)[ROWS], int waveBase,
+                                            int padT) {
 #pragma unroll
-  for (int q = 0; q < ROWS; ++q) {
-    const int i = (int)threadIdx.x + q * NT;
-    if (i < nT) out[i] = res[q];
-  }
+  for (int q = 0; q < ROWS; ++q)
+    if (waveBase + q * NT < padT) out[(int)threadIdx.x + q * NT] = res[q];
 }
 
 // Fold group: mass(i) = total - cardSum[c1] - cardSum[c2] + r[same(i)].
@@ -343,7 +354,9 @@ __device__ void foldGroup(const Dev& d, const Item& it, char* lds) {
   __shared__ double cs[53];
   __shared__ TermNode grp[kMaxGroup];
   const int tid = threadIdx.x, lane = tid & 63;
+  const int waveBase = __builtin_amdgcn_readfirstlane(tid >> 6) * 64;  // uniform
   const int nO = it.m, nT = it.n, cnt = it.cnt;
+  const int padT = (int)PPSG_ROW_STRIDE(nT);
   float* raw = reinterpret_cast<float*>(lds);
   const TermNode* nodes = stageNodes(grp, at<TermNode>(d, it.a), cnt);
   const char* base = d.arena + it.b;
@@ -379,7 +392,7 @@ __device__ void foldGroup(const Dev& d, const Item& it, char* lds) {
       if (j < nO) raw[j] = rv[q];
     }
     if (tid == 0) raw[nO] = 0.0f;
-    if (g > 0) storeRows<ROWS, kBlock>(pend, res, nT);
+    if (g > 0) storePadded<ROWS, kBlock>(pend, res, waveBase, padT);
     __syncthreads();
     loadReach<ROWS>(rv, base, nodes, g + 1, cnt, nO);  // the next node's reach
     double part = 0.0;
@@ -412,7 +425,7 @@ __device__ void foldGroup(const Dev& d, const Item& it, char* lds) {
   float rv[ROWS];
   loadReach<ROWS>(rv, base, nodes, 0, cnt, nO);
   for (int g = 0; g < cnt; ++g) node(g, rv);
-  storeRows<ROWS, kBlock>(pend, res, nT);
+  storePadded<ROWS, kBlock>(pend, res, waveBase, padT);
 }
 
 // Lane-permuted copy of a double within a wave through DPP moves (VALU, no
@@ -448,100 +461,167 @@ __device__ __forceinline__ double waveInclusiveSum(double x) {
   return x;
 }
 
-// LDS of a showdown group: reach (+ sentinel) | strength-order prefixes
-// (f64, 4 per lane) | per-card segment prefixes (f32, one extra slot per
-// card holding its total).
-__host__ __device__ constexpr size_t sdRawBytes(int nO) { return (size_t)(((nO + 1) * 4 + 15) & ~15); }
-__host__ __device__ constexpr size_t sdLdsBytes(int nO, int lanes) {
-  return sdRawBytes(nO) + (size_t)lanes * 4 * sizeof(double) + (size_t)(2 * nO + 52) * 4;
+// LDS of a fold group: the staged reach row and its zero sentinel slot nO.
+__host__ __device__ constexpr size_t foldLdsBytes(int nO) { return (size_t)(((nO + 1) * 4 + 15) & ~15); }
+
+// LDS of a showdown group of NT lanes (kSdRows reach entries per lane), all
+// of it dynamic so that the block starts at LDS address 0 and a lookup's
+// byte offset is its address:
+//   raw  the staged reach row: kSdRows * NT slots (zeros past the opponent
+//        list) + the zero sentinel slot kSdRows * NT
+//   Xs   strength-order prefixes, f64, 4 per lane, entry-major (position p
+//        at (p % 4) * NT + p / 4: a wave's stores of one entry are
+//        consecutive)
+//   Y    per-card segment prefixes, f32: the lane of card c's part k (lane
+//        4c + k, of the first kBlock lanes) owns kYSlots slots from
+//        (4c + k) * kYSlots; card c's total sits in slot 12 of its part 3.
+//        The odd stride puts the 64 lanes of a store, and the same slot of
+//        different cards, in different banks.
+//   the waves' scan totals (f64) and the group's node descriptors
+constexpr int kSdRows = 3;
+constexpr int kYSlots = 13;
+constexpr int kYTotal = 3 * kYSlots + 12;  // offset of the total from (4c) * kYSlots
+__host__ __device__ constexpr size_t sdRawBytes(int lanes) {
+  return ((size_t)(kSdRows * lanes + 1) * 4 + 15) & ~(size_t)15;
+}
+__host__ __device__ constexpr size_t sdXsBytes(int lanes) { return (size_t)lanes * 4 * sizeof(double); }
+__host__ __device__ constexpr size_t sdYBytes() { return ((size_t)kBlock * kYSlots * 4 + 15) & ~(size_t)15; }
+__host__ __device__ constexpr size_t sdTotBytes() { return (size_t)(kWideBlock / 64) * sizeof(double); }
+__host__ __device__ constexpr size_t sdLdsBytes(int lanes) {
+  return sdRawBytes(lanes) + sdXsBytes(lanes) + sdYBytes() + sdTotBytes() +
+         (size_t)kMaxGroup * sizeof(TermNode);
+}
+
+// LDS byte offset of the f32 segment prefix of card c at position a (0..L,
+// L the segment's length): the part holding position a < L is the last k
+// with (L * k) / 4 <= a (the parts split the segment at those bounds), and
+// position L is the card's total.
+__device__ __forceinline__ uint32_t segPrefixAt(uint32_t yBase, int c, int a, int L) {
+  a = max(a, 0);
+  if (a >= L) return yBase + 4u * (uint32_t)(4 * c * kYSlots + kYTotal);
+  int k = 0;
+#pragma unroll
+  for (int j = 1; j < 4; ++j) k = (L * j) / 4 <= a ? j : k;
+  return yBase + 4u * (uint32_t)((4 * c + k) * kYSlots + a - (L * k) / 4);
 }
 
 // Showdown group. Per node: opponent reach is staged in LDS, then
 //  - the class's opponent slots in strength order (4 per lane) are gathered
-//    and scanned in f64 into the prefixes Xs (entry-major: position p at
-//    Xs[(p % 4) * NT + p / 4], so a wave's stores of one entry are
-//    consecutive);
+//    and scanned in f64 into the prefixes Xs;
 //  - each card's segment (its opponent slots in strength order) is gathered
-//    by 4 lanes, and the segment-relative prefixes are stored in f32 (Y,
-//    segment c at offset c past its position so each segment has a slot
-//    for its total); segments hold at most 48 slots, so their f32 prefixes
-//    keep about 2^-24 of a segment total, not of the whole range.
+//    by 4 lanes, and the segment-relative prefixes are stored in f32 (Y);
+//    segments hold at most 48 slots, so their f32 prefixes keep about
+//    2^-24 of a segment total, not of the whole range.
 // Per traverser combo the weaker / weaker-or-equal / all masses are
 //   win = Xs[le] - Y1[a1] - Y2[a2], leq = Xs[te] - Y1[b1] - Y2[b2] + same,
 //   all = total - t1 - t2 + same
 // with the compile-time cut points le, te (strength list) and a, b (card
-// segments). The next node's reach row is requested as soon as this one is
+// segments). The LDS byte offsets of a combo's nine lookups depend only on
+// the class, so every lane works them out once per group and keeps them in
+// registers; every per-node store is unconditional or skipped by whole
+// waves. The next node's reach row is requested as soon as this one is
 // staged.
-template <int NT, int ROWS>
+template <int NT>
 __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
-  __shared__ double waveTot[NT / 64];
-  __shared__ float segTot[52];
-  __shared__ TermNode grp[kMaxGroup];
-  const int tid = threadIdx.x, lane = tid & 63;
+  constexpr int ROWS = kSdRows;
+  constexpr int kRaw = ROWS * NT;  // sentinel slot
+  const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar tests
+  const int waveBase = wave * 64;
   const int nO = it.m, nT = it.n, cnt = it.cnt;
+  const int padT = (int)PPSG_ROW_STRIDE(nT);
   float* raw = reinterpret_cast<float*>(lds);
-  double* Xs = reinterpret_cast<double*>(lds + sdRawBytes(nO));
-  float* Y = reinterpret_cast<float*>(lds + sdRawBytes(nO) + (size_t)NT * 4 * sizeof(double));
+  constexpr uint32_t xsBase = (uint32_t)sdRawBytes(NT);
+  constexpr uint32_t yBase = xsBase + (uint32_t)sdXsBytes(NT);
+  constexpr uint32_t totBase = yBase + (uint32_t)sdYBytes();
+  double* Xs = reinterpret_cast<double*>(lds + xsBase);
+  float* Y = reinterpret_cast<float*>(lds + yBase);
+  double* waveTot = reinterpret_cast<double*>(lds + totBase);
+  TermNode* grp = reinterpret_cast<TermNode*>(lds + totBase + sdTotBytes());
+  if (tid == 0) raw[kRaw] = 0.0f;
   const TermNode* nodes = stageNodes(grp, at<TermNode>(d, it.a), cnt);
   const char* base = d.arena + it.b;
 
   // class data for the whole group: 4 strength-order slots, this lane's part
-  // of one card segment, the per-combo bounds
-  const uint2 gs = at<uint2>(d, it.c + kGatherHeader)[tid];
-  const int sc = tid >> 2, part = tid & 3;
-  int segLo = 0, segLen = 0;
-  if (sc < 52) {
-    const int32_t* so = at<int32_t>(d, it.c);
-    const int s0 = so[sc], sl = so[sc + 1] - s0;
-    segLo = s0 + (sl * part) / 4;
-    segLen = s0 + (sl * (part + 1)) / 4 - segLo;
-  }
-  uint16_t idx[kFoldPart];
+  // of one card segment, the nine lookup offsets of each traverser combo
+  const int32_t* so = at<int32_t>(d, it.c);
+  uint32_t gA[4];
   {
+    const uint2 gs = at<uint2>(d, it.c + kGatherHeader)[tid];
+    gA[0] = 4u * (gs.x & 0xffffu);
+    gA[1] = 4u * (gs.x >> 16);
+    gA[2] = 4u * (gs.y & 0xffffu);
+    gA[3] = 4u * (gs.y >> 16);
+  }
+  const int sc = tid >> 2, part = tid & 3;
+  int segLen = 0;
+  uint32_t idx[kFoldPart];
+  {
+    int segLo = 0;
+    if (sc < 52) {
+      const int s0 = so[sc], sl = so[sc + 1] - s0;
+      segLo = s0 + (sl * part) / 4;
+      segLen = s0 + (sl * (part + 1)) / 4 - segLo;
+    }
     const uint16_t* seg = at<uint16_t>(d, it.c + kGatherHeader + (int64_t)NT * 4 * 2);
 #pragma unroll
-    for (int k = 0; k < kFoldPart; ++k) idx[k] = k < segLen ? seg[segLo + k] : (uint16_t)nO;
+    for (int k = 0; k < kFoldPart; ++k) idx[k] = 4u * (k < segLen ? seg[segLo + k] : (uint32_t)nO);
   }
-  uint4 meta[ROWS];
+  // per combo: reach of the same two cards, strength prefixes at le / te,
+  // the four segment prefixes, the two card totals
+  uint32_t fsA[ROWS], xlA[ROWS], xtA[ROWS], y1A[ROWS], y2A[ROWS], y3A[ROWS], y4A[ROWS],
+      t1A[ROWS], t2A[ROWS];
 #pragma unroll
   for (int q = 0; q < ROWS; ++q) {
     const int i = tid + q * NT;
-    meta[q] = i < nT ? at<uint4>(d, it.d)[i] : make_uint4(0, 0, 0, 0);
+    const uint4 m = i < nT ? at<uint4>(d, it.d)[i] : make_uint4(0, 0, 0, 0);
+    const int le = m.x & 0xffff, te = m.x >> 16;
+    const int c1 = m.w & 0xff, c2 = (m.w >> 8) & 0xff;
+    const int same = (int)(m.w >> 16) - 1;
+    const int o1 = so[c1], L1 = so[c1 + 1] - o1, o2 = so[c2], L2 = so[c2 + 1] - o2;
+    fsA[q] = 4u * (uint32_t)(same >= 0 ? same : kRaw);
+    xlA[q] = xsBase + 8u * (uint32_t)((le & 3) * NT + (le >> 2));
+    xtA[q] = xsBase + 8u * (uint32_t)((te & 3) * NT + (te >> 2));
+    y1A[q] = segPrefixAt(yBase, c1, (int)(m.y & 0xffff) - o1, L1);
+    y3A[q] = segPrefixAt(yBase, c1, (int)(m.y >> 16) - o1, L1);
+    y2A[q] = segPrefixAt(yBase, c2, (int)(m.z & 0xffff) - o2, L2);
+    y4A[q] = segPrefixAt(yBase, c2, (int)(m.z >> 16) - o2, L2);
+    t1A[q] = yBase + 4u * (uint32_t)(4 * c1 * kYSlots + kYTotal);
+    t2A[q] = yBase + 4u * (uint32_t)(4 * c2 * kYSlots + kYTotal);
   }
+  auto ldF = [&](uint32_t off) { return *reinterpret_cast<const float*>(lds + off); };
+  auto ldD = [&](uint32_t off) { return *reinterpret_cast<const double*>(lds + off); };
 
   float res[ROWS];
   float* pend = nullptr;  // the previous node's output row
   auto node = [&](int g, float(&rv)[ROWS]) {
     const TermNode nd = nodes[g];
     if (g > 0) __syncthreads();  // the previous node's queries are done with raw / Xs / Y
-    // 1. stage reach (raw[nO] = 0 is the padding sentinel)
+    // 1. stage reach, zeros past the list
 #pragma unroll
     for (int q = 0; q < ROWS; ++q) {
       const int j = tid + q * NT;
-      if (j < nO) raw[j] = rv[q];
+      raw[j] = j < nO ? rv[q] : 0.0f;
     }
-    if (tid == 0) raw[nO] = 0.0f;
-    if (g > 0) storeRows<ROWS, NT>(pend, res, nT);
+    if (g > 0) storePadded<ROWS, NT>(pend, res, waveBase, padT);
     __syncthreads();
     loadReach<ROWS, NT>(rv, base, nodes, g + 1, cnt, nO);  // the next node's reach
 
     // 2. gathers; strength-order scan across the block
     float gv[4];
-    gv[0] = raw[gs.x & 0xffffu];
-    gv[1] = raw[gs.x >> 16];
-    gv[2] = raw[gs.y & 0xffffu];
-    gv[3] = raw[gs.y >> 16];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) gv[k] = ldF(gA[k]);
     const double own = (((double)gv[0] + (double)gv[1]) + (double)gv[2]) + (double)gv[3];
     const double inc = waveInclusiveSum(own);
-    if (lane == 63) waveTot[wave] = inc;
+    if ((tid & 63) == 63) waveTot[wave] = inc;
     float sv[kFoldPart];
-    double segSum = 0.0;
 #pragma unroll
-    for (int k = 0; k < kFoldPart; ++k) {
-      sv[k] = raw[idx[k]];
-      segSum += (double)sv[k];
-    }
+    for (int k = 0; k < kFoldPart; ++k) sv[k] = ldF(idx[k]);
+    // (reach values are never -0.0, so starting from the first term is
+    // starting from 0.0)
+    double segSum = (double)sv[0];
+#pragma unroll
+    for (int k = 1; k < kFoldPart; ++k) segSum += (double)sv[k];
     __syncthreads();
     double run = inc - own;
 #pragma unroll
@@ -554,48 +634,42 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
       Xs[k * NT + tid] = run;
       run += (double)gv[k];
     }
-    // 3. segment-relative prefixes: the quad's four parts, in order
+    // 3. segment-relative prefixes: the quad's four parts, in order; every
+    // lane of the first kBlock writes its 12 slots (past its part's length
+    // they repeat the part's end and are never read)
     const double p0 = dppD<0x00>(segSum), p1 = dppD<0x55>(segSum);
     const double p2 = dppD<0xaa>(segSum), p3 = dppD<0xff>(segSum);
-    if (sc < 52) {
-      double y = part == 0 ? 0.0 : part == 1 ? p0 : part == 2 ? p0 + p1 : (p0 + p1) + p2;
-      float* yo = Y + segLo + sc;
+    const double p01 = p0 + p1, p012 = p01 + p2;
+    if (waveBase < kBlock) {
+      double y = part == 0 ? 0.0 : part == 1 ? p0 : part == 2 ? p01 : p012;
+      float yv[kFoldPart];
 #pragma unroll
       for (int k = 0; k < kFoldPart; ++k) {
-        if (k < segLen) yo[k] = (float)y;
+        yv[k] = (float)y;
         y += (double)sv[k];
       }
-      if (part == 3) {
-        const float t = (float)(((p0 + p1) + p2) + p3);
-        yo[segLen] = t;
-        segTot[sc] = t;
-      }
+      float* yo = Y + tid * kYSlots;
+#pragma unroll
+      for (int k = 0; k < kFoldPart; ++k) yo[k] = yv[k];
+      yo[12] = (float)(p012 + p3);
     }
     __syncthreads();
 
     // 4. per traverser combo
-    auto xs = [&](int p) -> double { return Xs[(p & 3) * NT + (p >> 2)]; };
     pend = reinterpret_cast<float*>(d.arena + it.b + nd.out);
     // value = vWin win + vTie (leq - win) + vLose (all - leq)
     //       = kWin win + kLeq leq + vLose all
     const double vLose = nd.v2;
     const double kWin = (double)nd.v0 - (double)nd.v1, kLeq = (double)nd.v1 - vLose;
-    const double total = xs(nO);
+    const double total = Xs[(nO & 3) * NT + (nO >> 2)];
 #pragma unroll
     for (int q = 0; q < ROWS; ++q) {
-      if ((int)tid + q * NT >= nT) continue;  // whole waves past the list skip
-      const uint4 m = meta[q];
-      const int le = m.x & 0xffff, te = m.x >> 16;
-      const int c1 = m.w & 0xff, c2 = (m.w >> 8) & 0xff;
-      const int a1 = (m.y & 0xffff) + c1, b1 = (m.y >> 16) + c1;
-      const int a2 = (m.z & 0xffff) + c2, b2 = (m.z >> 16) + c2;
-      const int same = (int)(m.w >> 16) - 1;
-      // the nine lookups are issued together (no branch on `same`: the
-      // sentinel raw[nO] is 0), then the arithmetic
-      const float fs = raw[same >= 0 ? same : nO];
-      const double xl = xs(le), xt = xs(te);
-      const float y1 = Y[a1], y2 = Y[a2], y3 = Y[b1], y4 = Y[b2];
-      const float t1 = segTot[c1], t2 = segTot[c2];
+      if (waveBase + q * NT >= nT) continue;  // whole waves past the list skip
+      // the nine lookups are issued together, then the arithmetic
+      const float fs = ldF(fsA[q]);
+      const double xl = ldD(xlA[q]), xt = ldD(xtA[q]);
+      const float y1 = ldF(y1A[q]), y2 = ldF(y2A[q]), y3 = ldF(y3A[q]), y4 = ldF(y4A[q]);
+      const float t1 = ldF(t1A[q]), t2 = ldF(t2A[q]);
       __builtin_amdgcn_sched_group_barrier(0x0100, 9, 0);  // LDS reads
       __builtin_amdgcn_sched_group_barrier(0x0002, 64, 0);  // then VALU
       // each card pair is summed in f32 first: the sum is commutative, so a
@@ -611,7 +685,7 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
   float rv[ROWS];
   loadReach<ROWS, NT>(rv, base, nodes, 0, cnt, nO);
   for (int g = 0; g < cnt; ++g) node(g, rv);
-  storeRows<ROWS, NT>(pend, res, nT);
+  storePadded<ROWS, NT>(pend, res, waveBase, padT);
 }
 
 // Terminal launches of a half-step (after the forward pass has produced all
@@ -634,7 +708,7 @@ __global__ __launch_bounds__(kBlock) void showdownKernel(Dev d, const Item* __re
   extern __shared__ __align__(16) char lds[];
   const Item it = items[blockIdx.x];
   if (masked && !d.ctl[it.f].alive) return;
-  showdownGroup<kBlock, 3>(d, it, lds);
+  showdownGroup<kBlock>(d, it, lds);
 }
 
 // The larger classes (up to 1326 combos per side): 512 lanes.
@@ -644,7 +718,7 @@ __global__ __launch_bounds__(kWideBlock) void showdownWideKernel(Dev d,
   extern __shared__ __align__(16) char lds[];
   const Item it = items[blockIdx.x];
   if (masked && !d.ctl[it.f].alive) return;
-  showdownGroup<kWideBlock, 3>(d, it, lds);
+  showdownGroup<kWideBlock>(d, it, lds);
 }
 
 // Forward launch of one depth: opponent reach of the next depth.
@@ -671,13 +745,15 @@ __global__ __launch_bounds__(kBlock) void fwdKernel(Dev d, const Item* __restric
 // dry (CFR mode, timing only): the update is computed and stored, but the
 // stored values are the old ones, so the tile moves exactly the bytes of a
 // real update and changes no row.
+// Whole tiles, as in the forward pass: the lanes past the n traverser
+// combos work on the rows' padding.
 template <int MODE, int NA, int R>
 __device__ __forceinline__ void decideTile(const Dev& d, const Item& it, int dry) {
   const int c0 = it.tile * kWave * R + waveLane();
-  const int n = it.n;
+  const int n = (int)PPSG_ROW_STRIDE(it.n);
   if (c0 >= n) return;
   const int na = NA > 0 ? NA : it.na;
-  const int64_t stride = it.m;
+  const int64_t stride = it.m, ld = it.ld;
   const float* __restrict__ kid = at<float>(d, it.c);
   float* __restrict__ out = at<float>(d, it.b);
   if (!it.e || MODE == PPSG_MODE_BR) {
@@ -712,9 +788,9 @@ __device__ __forceinline__ void decideTile(const Dev& d, const Item& it, int dry
       const int c = c0 + k * kWave;
 #pragma unroll
       for (int a = 0; a < A; ++a) {
-        x[k][a] = c < n ? src[(int64_t)a * n + c] : 0.0f;
+        x[k][a] = c < n ? src[a * ld + c] : 0.0f;
         v[k][a] = c < n ? kid[a * stride + c] : 0.0f;
-        sm[k][a] = plus && c < n ? st[(int64_t)a * n + c] : 0.0f;
+        sm[k][a] = plus && c < n ? st[a * ld + c] : 0.0f;
       }
     }
     float pos = 0.0f, neg = 0.0f, avg = 0.0f;
@@ -743,8 +819,8 @@ __device__ __forceinline__ void decideTile(const Dev& d, const Item& it, int dry
       for (int a = 0; a < A; ++a) {
         const float ns = sm[k][a] * avg + s[a];
         const float nr = x[k][a] * (x[k][a] >= 0.0f ? pos : neg) + (v[k][a] - val);
-        st[(int64_t)a * n + c] = dry ? sm[k][a] : ns;
-        src[(int64_t)a * n + c] = dry ? x[k][a] : nr;
+        st[a * ld + c] = dry ? sm[k][a] : ns;
+        src[a * ld + c] = dry ? x[k][a] : nr;
       }
     }
     return;
@@ -754,22 +830,22 @@ __device__ __forceinline__ void decideTile(const Dev& d, const Item& it, int dry
     if (c >= n) return;
     float acc = 0.0f;
     for (int a = 0; a < na; ++a) {
-      const float x = src[(int64_t)a * n + c];
+      const float x = src[a * ld + c];
       acc += plus ? (x > 0.0f ? x : 0.0f) : x;
     }
     float val = 0.0f;
     for (int a = 0; a < na; ++a)
-      val += sigmaOf(src[(int64_t)a * n + c], acc, uniform, plus) * kid[a * stride + c];
+      val += sigmaOf(src[a * ld + c], acc, uniform, plus) * kid[a * stride + c];
     out[c] = val;
     if (MODE != PPSG_MODE_CFR) continue;
     const Ctl& ctl = d.ctl[it.f];
     const float pos = ctl.cur[0], neg = ctl.cur[1], avg = ctl.cur[2];
     for (int a = 0; a < na; ++a) {
-      const float x = src[(int64_t)a * n + c];
+      const float x = src[a * ld + c];
       const float sg = sigmaOf(x, acc, uniform, plus);
-      const float so = st[(int64_t)a * n + c];
-      st[(int64_t)a * n + c] = dry ? so : so * avg + sg;
-      src[(int64_t)a * n + c] = dry ? x : x * (x >= 0.0f ? pos : neg) + (kid[a * stride + c] - val);
+      const float so = st[a * ld + c];
+      st[a * ld + c] = dry ? so : so * avg + sg;
+      src[a * ld + c] = dry ? x : x * (x >= 0.0f ? pos : neg) + (kid[a * stride + c] - val);
     }
   }
 }
@@ -786,7 +862,8 @@ __device__ __forceinline__ void decideTile(const Dev& d, const Item& it, int dry
 __device__ __forceinline__ void chanceTile(const Dev& d, const Item& it) {
   const int i = it.tile * kWave + waveLane();
   const int n = it.n;
-  if (i >= n) return;
+  if (i >= (int)PPSG_ROW_STRIDE(n)) return;  // whole tiles: lanes past n store 0
+  const bool in = i < n;
   // 32-bit unsigned byte offsets from uniform bases (a branch block is at
   // most 255 x 1326 floats): one VGPR per address instead of two
   const char* __restrict__ map = at<char>(d, it.a);
@@ -801,7 +878,7 @@ __device__ __forceinline__ void chanceTile(const Dev& d, const Item& it) {
     float x[kChunk];
 #pragma unroll
     for (int k = 0; k < kChunk; ++k)
-      slot[k] = b0 + k < nb
+      slot[k] = in && b0 + k < nb
                     ? *reinterpret_cast<const int16_t*>(map + 2u * ((uint32_t)(b0 + k) * (uint32_t)n + (uint32_t)i))
                     : -1;
 #pragma unroll
@@ -1055,7 +1132,7 @@ bool wideShowdown(int nT, int nO) { return std::max(nT, nO) > 3 * kBlock; }
 // Dynamic LDS of a terminal item: reach (+ sentinel), and for showdown the
 // prefixes.
 size_t terminalLdsOf(const Item& it, int lanes) {
-  return it.kind == kBwdShowdown ? sdLdsBytes(it.m, lanes) : sdRawBytes(it.m);
+  return it.kind == kBwdShowdown ? sdLdsBytes(lanes) : foldLdsBytes(it.m);
 }
 
 // Host-side tables of one spot, built from its description.
@@ -1141,6 +1218,16 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
     b = t->child_base[u];
     e = u + 1 < N ? t->child_base[u + 1] : (int)t->n_child;
   };
+  // rows on 64-float lines; a decide node's action rows PPSG_ROW_STRIDE apart
+  for (int64_t u = 0; u < N; ++u) {
+    if (t->cfv_off[u] % 64 || t->reach_off[0][u] % 64 || t->reach_off[1][u] % 64)
+      return fail("tree description: a row does not start on a 64-float line");
+    if (t->kind[u] == PPSG_DECIDE &&
+        (t->reg_off[u] % 64 ||
+         t->reg_off[u] + t->na[u] * PPSG_ROW_STRIDE(t->cls_n[t->owner[u]][t->cls[u]]) >
+             t->reg_total))
+      return fail("tree description: regret rows not PPSG_ROW_STRIDE apart within reg_total");
+  }
   for (int p = 0; p < 2; ++p) {
     T.sdOff[p].assign(nc, -1);
     T.foldOff[p].assign(nc, -1);
@@ -1225,6 +1312,8 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
         const int na = t->na[u];
         const int first = t->child_flat[kb];
         if (t->owner[u] == op) {
+          const int64_t rs =
+              na > 1 ? t->reach_off[tr][t->child_flat[kb + 1]] - t->reach_off[tr][first] : 0;
           for (int q = 0; q < tiles(nO, 1); ++q) {
             Item it{};
             it.kind = kFwdDecide;
@@ -1233,6 +1322,8 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
             it.d = L.strat + 4 * t->reg_off[u];
             it.b = L.reach[tr] + 4 * t->reach_off[tr][u];
             it.c = L.reach[tr] + 4 * t->reach_off[tr][first];
+            it.m = (int32_t)rs;
+            it.ld = (int32_t)PPSG_ROW_STRIDE(nO);
             it.n = nO;
             it.na = (uint8_t)na;
             it.tile = (uint8_t)q;
@@ -1249,6 +1340,7 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
           it.b = L.cfv + 4 * t->cfv_off[u];
           it.c = L.cfv + 4 * t->cfv_off[first];
           it.m = (int32_t)stride;
+          it.ld = (int32_t)PPSG_ROW_STRIDE(t->cls_n[t->owner[u]][cl]);
           it.n = nT;
           it.na = (uint8_t)na;
           it.e = t->owner[u] == tr;

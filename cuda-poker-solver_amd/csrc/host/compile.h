@@ -69,10 +69,14 @@ struct CompiledTree {
   std::vector<uint8_t> na;
   std::vector<int32_t> depth, cls, childBase, childFlat, parent;
   std::vector<int64_t> potBase, sc0, sc1, pc0, pc1;
-  // device row offsets (floats), placed breadth-first: by depth, then node id
+  // device row offsets (floats), placed breadth-first: by depth, then node
+  // id; rows padded to PPSG_ROW_STRIDE (a decide node's action rows are
+  // regStride(u) apart, its block na * regStride(u) long)
   std::vector<int64_t> regOff, cfvOff, reachOff[2];
   std::vector<float> termVal;
-  int64_t regTotal = 0, cfvTotal = 0, reachTotal[2] = {0, 0};
+  int64_t cfvTotal = 0, reachTotal[2] = {0, 0};
+  int64_t regDevTotal = 0;  // device regret / strategy table, padded rows
+  int64_t regTotal = 0;     // the same table in node order, unpadded (API, files)
   std::vector<float> rootReach[2];
   std::vector<pf::TreeAction> actFlat;
   std::vector<int32_t> actBase;
@@ -80,6 +84,8 @@ struct CompiledTree {
   std::vector<int32_t> decideOf;     // node id -> decide index or -1
 
   int listSize(int p, int node) const { return clsN[p][cls[node]]; }
+  // distance between a decide node's action rows on the device
+  int64_t regStride(int64_t u) const { return PPSG_ROW_STRIDE(listSize(owner[u], (int)u)); }
   ppsg_tree_desc desc() const;
   // Algorithmic f32 row bytes of one iteration (both half-steps).
   double iterationBytes() const;

@@ -326,9 +326,14 @@ std::vector<SpotResult> serve(const std::vector<SpotRequest>& reqs, const ServeO
         res.converged = expl[i] <= opt.targetFrac * L.pot;
         res.ev0 = ev0[i];
         res.ev1 = ev1[i];
-        std::vector<float> rows((size_t)L.na0 * L.nC0);
+        // the root's action rows sit PPSG_ROW_STRIDE(nC0) apart
+        const int64_t ld = PPSG_ROW_STRIDE(L.nC0);
+        std::vector<float> rows((size_t)((L.na0 - 1) * ld + L.nC0));
         check(ppsg_batch_read(h, L.slot, PPSG_STRAT, L.regOff0, (int64_t)rows.size(), rows.data()),
               "strategy read");
+        for (int a = 1; a < L.na0; ++a)
+          std::copy(rows.begin() + a * ld, rows.begin() + a * ld + L.nC0, rows.begin() + a * L.nC0);
+        rows.resize((size_t)L.na0 * L.nC0);
         res.rootStrategy = rangeFrequencies(rows, L.w[0], L.na0, L.nC0);
         check(ppsg_batch_remove(h, L.slot), "batch remove");
         res.retireMs = msNow();

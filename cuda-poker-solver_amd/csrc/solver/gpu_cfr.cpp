@@ -92,29 +92,41 @@ struct GpuPostflopSolver::Impl {
     check(ppsg_batch_write(eng, slot, buf, off, n, v));
   }
   // Whole regret / strategy tables in node order: each decide node's
-  // action-major block after the blocks of all lower node ids (the row
-  // layout of the reference's tables and of solution files). On the device
-  // the blocks sit breadth-first (T.regOff).
+  // action-major block (na rows of nC floats, back to back) after the
+  // blocks of all lower node ids (the row layout of the reference's tables
+  // and of solution files). On the device the blocks sit breadth-first
+  // (T.regOff) with each action row padded to T.regStride(u).
   std::vector<float> readTable(int buf) const {
-    const std::vector<float> dev = read(buf, 0, T.regTotal);
+    const std::vector<float> dev = read(buf, 0, T.regDevTotal);
     std::vector<float> out((size_t)T.regTotal);
     int64_t at = 0;
     for (const int32_t u : T.decideNodes) {
-      const int64_t len = (int64_t)T.na[u] * T.listSize(T.owner[u], u);
-      std::copy(dev.begin() + T.regOff[u], dev.begin() + T.regOff[u] + len, out.begin() + at);
-      at += len;
+      const int64_t nC = T.listSize(T.owner[u], u), ld = T.regStride(u);
+      for (int a = 0; a < T.na[u]; ++a, at += nC)
+        std::copy(dev.begin() + T.regOff[u] + a * ld, dev.begin() + T.regOff[u] + a * ld + nC,
+                  out.begin() + at);
     }
     return out;
   }
   void writeTable(int buf, const float* v) const {
-    std::vector<float> dev((size_t)T.regTotal);
+    std::vector<float> dev((size_t)T.regDevTotal, 0.0f);
     int64_t at = 0;
     for (const int32_t u : T.decideNodes) {
-      const int64_t len = (int64_t)T.na[u] * T.listSize(T.owner[u], u);
-      std::copy(v + at, v + at + len, dev.begin() + T.regOff[u]);
-      at += len;
+      const int64_t nC = T.listSize(T.owner[u], u), ld = T.regStride(u);
+      for (int a = 0; a < T.na[u]; ++a, at += nC)
+        std::copy(v + at, v + at + nC, dev.begin() + T.regOff[u] + a * ld);
     }
-    write(buf, 0, T.regTotal, dev.data());
+    write(buf, 0, T.regDevTotal, dev.data());
+  }
+  // One decide node's block, action-major with nC floats per action.
+  std::vector<float> readBlock(int buf, int64_t u) const {
+    const int na = T.na[u];
+    const int64_t nC = T.listSize(T.owner[u], (int)u), ld = T.regStride(u);
+    std::vector<float> dev = read(buf, T.regOff[u], (na - 1) * ld + nC);
+    for (int a = 1; a < na; ++a)
+      std::copy(dev.begin() + a * ld, dev.begin() + a * ld + nC, dev.begin() + a * nC);
+    dev.resize((size_t)(na * nC));
+    return dev;
   }
   // EV / BR walk of this spot alone; cfv (n[tr] floats) may be null.
   void walk(int tr, int mode, float* cfv) const {
@@ -450,7 +462,7 @@ pf::NodeStats GpuPostflopSolver::stats() {
   if (const char* sm = std::getenv("GPU_CFR_SEED_STRAT")) {
     // probe profile of cuda/gpu_cfr.cu:2062-2097
     const int mode = std::atoi(sm);
-    std::vector<float> rows((size_t)std::max<int64_t>(I->T.regTotal, 0), 0.0f);
+    std::vector<float> rows((size_t)std::max<int64_t>(I->T.regDevTotal, 0), 0.0f);
     for (size_t mi = 0; mi < I->T.decideNodes.size(); ++mi) {
       const int u = I->T.decideNodes[mi];
       const int dpt = I->T.depth[u];
@@ -472,7 +484,7 @@ pf::NodeStats GpuPostflopSolver::stats() {
       for (int c = 0; c < nC; ++c) {
         const float p0 = (float)(0.15 + 0.7 * ((c % 8) / 7.0));
         rows[I->T.regOff[u] + c] = p0;
-        rows[I->T.regOff[u] + nC + c] = 1.0f - p0;
+        rows[I->T.regOff[u] + I->T.regStride(u) + c] = 1.0f - p0;
       }
     }
     if (!rows.empty()) I->write(PPSG_STRAT, 0, (int64_t)rows.size(), rows.data());
@@ -493,14 +505,14 @@ std::vector<double> GpuPostflopSolver::nodeStrategy(int decideIdx) {
   const int u = I->T.decideNodes[decideIdx];
   if (I->T.clsNBoard[I->T.cls[u]] != I->T.spot.nBoard) return {};
   const int p = I->T.owner[u], na = I->T.na[u], nC = I->T.listSize(p, u);
-  return rangeFrequencies(I->read(PPSG_STRAT, I->T.regOff[u], (int64_t)na * nC), I->T.w[p], na, nC);
+  return rangeFrequencies(I->readBlock(PPSG_STRAT, u), I->T.w[p], na, nC);
 }
 
 std::vector<double> GpuPostflopSolver::rootStrategy() {
   Impl* I = impl_;
   if (I->T.empty) return {};
   const int na = I->T.na[0], nC = I->T.listSize(0, 0);
-  return rangeFrequencies(I->read(PPSG_STRAT, I->T.regOff[0], (int64_t)na * nC), I->T.w[0], na, nC);
+  return rangeFrequencies(I->readBlock(PPSG_STRAT, 0), I->T.w[0], na, nC);
 }
 
 int GpuPostflopSolver::numDecideNodes() const { return (int)impl_->T.decideNodes.size(); }
@@ -528,7 +540,7 @@ ComboStrategy GpuPostflopSolver::comboStrategy(int decideIdx) {
   out.actions = I->labels(u);
   const int64_t off = I->T.clsListOff[p][I->T.cls[u]];
   out.combos.assign(I->T.listBase[p].begin() + off, I->T.listBase[p].begin() + off + nC);
-  const std::vector<float> rows = I->read(PPSG_STRAT, I->T.regOff[u], (int64_t)na * nC);
+  const std::vector<float> rows = I->readBlock(PPSG_STRAT, u);
   out.freqs.assign((size_t)na * nC, 0.0);
   for (int c = 0; c < nC; ++c) {
     double s = 0.0;
@@ -586,7 +598,7 @@ NodeEv GpuPostflopSolver::nodeEv(int decideIdx) {
     for (int i = 0; i < nC; ++i) S.ev[i] = S.mass[i] > 0.0 ? (double)row[i] / S.mass[i] : 0.0;
     const std::vector<double>& wSelf = I->T.w[tr];
     if (tr == dec) {
-      const std::vector<float> srows = I->read(PPSG_STRAT, I->T.regOff[u], (int64_t)na * nC);
+      const std::vector<float> srows = I->readBlock(PPSG_STRAT, u);
       for (int a = 0; a < na; ++a) {
         const int ch = I->T.childFlat[I->T.childBase[u] + a];
         const int nChild = I->T.listSize(tr, ch);
@@ -702,7 +714,7 @@ RunoutSummary GpuPostflopSolver::runoutSummary(int decideIdx, int actionIdx) {
       out.branches[b].aggEv[tr] = den > 0.0 ? num / den : 0.0;
       const int bna = I->T.na[f];
       if (I->T.owner[f] != tr && bna == (int)out.actions.size()) {
-        const std::vector<float> srows = I->read(PPSG_STRAT, I->T.regOff[f], (int64_t)bna * nOpp);
+        const std::vector<float> srows = I->readBlock(PPSG_STRAT, f);
         std::vector<double>& freq = out.branches[b].freq;
         freq.assign(bna, 0.0);
         double z = 0.0;
@@ -891,7 +903,8 @@ void GpuPostflopSolver::debugDump() {
   Impl* I = impl_;
   if (I->T.empty) return;
   for (int which = 0; which < 2; ++which) {
-    const std::vector<float> v = I->read(which == 0 ? PPSG_STRAT : PPSG_REGRET, 0, I->T.regTotal);
+    // node order: the device table's row padding holds values no one reads
+    const std::vector<float> v = I->readTable(which == 0 ? PPSG_STRAT : PPSG_REGRET);
     long long bad = 0;
     for (float x : v) bad += !std::isfinite(x);
     std::fprintf(stderr, "%s nonfinite=%lld of %lld\n", which == 0 ? "strat" : "regret", bad,

@@ -30,6 +30,11 @@ enum { PPSG_MODE_CFR = 0, PPSG_MODE_EV = 1, PPSG_MODE_BR = 2 };
 enum { PPSG_REGRET = 0, PPSG_STRAT = 1, PPSG_REACH0 = 2, PPSG_REACH1 = 3, PPSG_CFV = 4 };
 enum { PPSG_TUNE_GROUP = 0 };
 
+/* Row offsets (reg_off, cfv_off, reach_off, in floats) are multiples of
+   64: every f32 row starts on a 256-byte line and is padded to
+   PPSG_ROW_STRIDE(n) floats, which the engine checks. */
+#define PPSG_ROW_STRIDE(n) (((int64_t)(n) + 63) / 64 * 64)
+
 typedef struct ppsg_tree_desc {
   int32_t n_base[2];               /* base list sizes (root class lists) */
   /* board classes */
@@ -65,9 +70,13 @@ typedef struct ppsg_tree_desc {
   const int32_t* child_base;       /* into child_flat */
   int64_t n_child;
   const int32_t* child_flat;
-  const int64_t* reg_off;          /* decide: regret/strategy row start */
+  const int64_t* reg_off;          /* decide: regret/strategy block start;
+                                      action a's row at reg_off + a *
+                                      PPSG_ROW_STRIDE(n_actor) */
   const int64_t* cfv_off;
-  const int64_t* reach_off[2];     /* per traverser layout (aliased rows) */
+  const int64_t* reach_off[2];     /* per traverser layout (aliased rows);
+                                      the children of an opponent-acting
+                                      decide node PPSG_ROW_STRIDE(n) apart */
   const float* term_val;           /* 8 per node: [tr][win, tie, lose, -] */
   int64_t reg_total, cfv_total, reach_total[2];
   const float* root_reach[2];      /* layout tr root row: opponent weights */
