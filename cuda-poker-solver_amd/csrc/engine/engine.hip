@@ -464,164 +464,101 @@ __device__ __forceinline__ double waveInclusiveSum(double x) {
 // LDS of a fold group: the staged reach row and its zero sentinel slot nO.
 __host__ __device__ constexpr size_t foldLdsBytes(int nO) { return (size_t)(((nO + 1) * 4 + 15) & ~15); }
 
-// LDS of a showdown group of NT lanes (kSdRows reach entries per lane), all
-// of it dynamic so that the block starts at LDS address 0 and a lookup's
-// byte offset is its address:
-//   raw  the staged reach row: kSdRows * NT slots (zeros past the opponent
-//        list) + the zero sentinel slot kSdRows * NT
-//   Xs   strength-order prefixes, f64, 4 per lane, entry-major (position p
-//        at (p % 4) * NT + p / 4: a wave's stores of one entry are
-//        consecutive)
-//   Y    per-card segment prefixes, f32: the lane of card c's part k (lane
-//        4c + k, of the first kBlock lanes) owns kYSlots slots from
-//        (4c + k) * kYSlots; card c's total sits in slot 12 of its part 3.
-//        The odd stride puts the 64 lanes of a store, and the same slot of
-//        different cards, in different banks.
-//   the waves' scan totals (f64) and the group's node descriptors
-constexpr int kSdRows = 3;
-constexpr int kYSlots = 13;
-constexpr int kYTotal = 3 * kYSlots + 12;  // offset of the total from (4c) * kYSlots
-__host__ __device__ constexpr size_t sdRawBytes(int lanes) {
-  return ((size_t)(kSdRows * lanes + 1) * 4 + 15) & ~(size_t)15;
-}
-__host__ __device__ constexpr size_t sdXsBytes(int lanes) { return (size_t)lanes * 4 * sizeof(double); }
-__host__ __device__ constexpr size_t sdYBytes() { return ((size_t)kBlock * kYSlots * 4 + 15) & ~(size_t)15; }
-__host__ __device__ constexpr size_t sdTotBytes() { return (size_t)(kWideBlock / 64) * sizeof(double); }
-__host__ __device__ constexpr size_t sdLdsBytes(int lanes) {
-  return sdRawBytes(lanes) + sdXsBytes(lanes) + sdYBytes() + sdTotBytes() +
-         (size_t)kMaxGroup * sizeof(TermNode);
-}
-
-// LDS byte offset of the f32 segment prefix of card c at position a (0..L,
-// L the segment's length): the part holding position a < L is the last k
-// with (L * k) / 4 <= a (the parts split the segment at those bounds), and
-// position L is the card's total.
-__device__ __forceinline__ uint32_t segPrefixAt(uint32_t yBase, int c, int a, int L) {
-  a = max(a, 0);
-  if (a >= L) return yBase + 4u * (uint32_t)(4 * c * kYSlots + kYTotal);
-  int k = 0;
-#pragma unroll
-  for (int j = 1; j < 4; ++j) k = (L * j) / 4 <= a ? j : k;
-  return yBase + 4u * (uint32_t)((4 * c + k) * kYSlots + a - (L * k) / 4);
+// LDS of a showdown group: reach (+ sentinel) | strength-order prefixes
+// (f64, 4 per lane) | per-card segment prefixes (f32, one extra slot per
+// card holding its total).
+__host__ __device__ constexpr size_t sdRawBytes(int nO) { return (size_t)(((nO + 1) * 4 + 15) & ~15); }
+__host__ __device__ constexpr size_t sdLdsBytes(int nO, int lanes) {
+  return sdRawBytes(nO) + (size_t)lanes * 4 * sizeof(double) + (size_t)(2 * nO + 52) * 4;
 }
 
 // Showdown group. Per node: opponent reach is staged in LDS, then
 //  - the class's opponent slots in strength order (4 per lane) are gathered
-//    and scanned in f64 into the prefixes Xs;
+//    and scanned in f64 into the prefixes Xs (entry-major: position p at
+//    Xs[(p % 4) * NT + p / 4], so a wave's stores of one entry are
+//    consecutive);
 //  - each card's segment (its opponent slots in strength order) is gathered
-//    by 4 lanes, and the segment-relative prefixes are stored in f32 (Y);
-//    segments hold at most 48 slots, so their f32 prefixes keep about
-//    2^-24 of a segment total, not of the whole range.
+//    by 4 lanes, and the segment-relative prefixes are stored in f32 (Y,
+//    segment c at offset c past its position so each segment has a slot
+//    for its total); segments hold at most 48 slots, so their f32 prefixes
+//    keep about 2^-24 of a segment total, not of the whole range.
 // Per traverser combo the weaker / weaker-or-equal / all masses are
 //   win = Xs[le] - Y1[a1] - Y2[a2], leq = Xs[te] - Y1[b1] - Y2[b2] + same,
 //   all = total - t1 - t2 + same
 // with the compile-time cut points le, te (strength list) and a, b (card
-// segments). The LDS byte offsets of a combo's nine lookups depend only on
-// the class, so every lane works them out once per group and keeps them in
-// registers; every per-node store is unconditional or skipped by whole
-// waves. The next node's reach row is requested as soon as this one is
+// segments). The next node's reach row is requested as soon as this one is
 // staged.
-template <int NT>
+template <int NT, int ROWS>
 __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
-  constexpr int ROWS = kSdRows;
-  constexpr int kRaw = ROWS * NT;  // sentinel slot
-  const int tid = threadIdx.x;
+  __shared__ double waveTot[NT / 64];
+  __shared__ float segTot[52];
+  __shared__ TermNode grp[kMaxGroup];
+  const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar tests
-  const int waveBase = wave * 64;
   const int nO = it.m, nT = it.n, cnt = it.cnt;
-  const int padT = (int)PPSG_ROW_STRIDE(nT);
+  const int waveBase = wave * 64, padT = (int)PPSG_ROW_STRIDE(nT);
   float* raw = reinterpret_cast<float*>(lds);
-  constexpr uint32_t xsBase = (uint32_t)sdRawBytes(NT);
-  constexpr uint32_t yBase = xsBase + (uint32_t)sdXsBytes(NT);
-  constexpr uint32_t totBase = yBase + (uint32_t)sdYBytes();
-  double* Xs = reinterpret_cast<double*>(lds + xsBase);
-  float* Y = reinterpret_cast<float*>(lds + yBase);
-  double* waveTot = reinterpret_cast<double*>(lds + totBase);
-  TermNode* grp = reinterpret_cast<TermNode*>(lds + totBase + sdTotBytes());
-  if (tid == 0) raw[kRaw] = 0.0f;
+  double* Xs = reinterpret_cast<double*>(lds + sdRawBytes(nO));
+  float* Y = reinterpret_cast<float*>(lds + sdRawBytes(nO) + (size_t)NT * 4 * sizeof(double));
   const TermNode* nodes = stageNodes(grp, at<TermNode>(d, it.a), cnt);
   const char* base = d.arena + it.b;
 
   // class data for the whole group: 4 strength-order slots, this lane's part
-  // of one card segment, the nine lookup offsets of each traverser combo
-  const int32_t* so = at<int32_t>(d, it.c);
-  uint32_t gA[4];
-  {
-    const uint2 gs = at<uint2>(d, it.c + kGatherHeader)[tid];
-    gA[0] = 4u * (gs.x & 0xffffu);
-    gA[1] = 4u * (gs.x >> 16);
-    gA[2] = 4u * (gs.y & 0xffffu);
-    gA[3] = 4u * (gs.y >> 16);
-  }
+  // of one card segment, the per-combo bounds
+  const uint2 gs = at<uint2>(d, it.c + kGatherHeader)[tid];
   const int sc = tid >> 2, part = tid & 3;
-  int segLen = 0;
-  uint32_t idx[kFoldPart];
+  int segLo = 0, segLen = 0;
+  if (sc < 52) {
+    const int32_t* so = at<int32_t>(d, it.c);
+    const int s0 = so[sc], sl = so[sc + 1] - s0;
+    segLo = s0 + (sl * part) / 4;
+    segLen = s0 + (sl * (part + 1)) / 4 - segLo;
+  }
+  uint16_t idx[kFoldPart];
   {
-    int segLo = 0;
-    if (sc < 52) {
-      const int s0 = so[sc], sl = so[sc + 1] - s0;
-      segLo = s0 + (sl * part) / 4;
-      segLen = s0 + (sl * (part + 1)) / 4 - segLo;
-    }
     const uint16_t* seg = at<uint16_t>(d, it.c + kGatherHeader + (int64_t)NT * 4 * 2);
 #pragma unroll
-    for (int k = 0; k < kFoldPart; ++k) idx[k] = 4u * (k < segLen ? seg[segLo + k] : (uint32_t)nO);
+    for (int k = 0; k < kFoldPart; ++k) idx[k] = k < segLen ? seg[segLo + k] : (uint16_t)nO;
   }
-  // per combo: reach of the same two cards, strength prefixes at le / te,
-  // the four segment prefixes, the two card totals
-  uint32_t fsA[ROWS], xlA[ROWS], xtA[ROWS], y1A[ROWS], y2A[ROWS], y3A[ROWS], y4A[ROWS],
-      t1A[ROWS], t2A[ROWS];
+  uint4 meta[ROWS];
 #pragma unroll
   for (int q = 0; q < ROWS; ++q) {
     const int i = tid + q * NT;
-    const uint4 m = i < nT ? at<uint4>(d, it.d)[i] : make_uint4(0, 0, 0, 0);
-    const int le = m.x & 0xffff, te = m.x >> 16;
-    const int c1 = m.w & 0xff, c2 = (m.w >> 8) & 0xff;
-    const int same = (int)(m.w >> 16) - 1;
-    const int o1 = so[c1], L1 = so[c1 + 1] - o1, o2 = so[c2], L2 = so[c2 + 1] - o2;
-    fsA[q] = 4u * (uint32_t)(same >= 0 ? same : kRaw);
-    xlA[q] = xsBase + 8u * (uint32_t)((le & 3) * NT + (le >> 2));
-    xtA[q] = xsBase + 8u * (uint32_t)((te & 3) * NT + (te >> 2));
-    y1A[q] = segPrefixAt(yBase, c1, (int)(m.y & 0xffff) - o1, L1);
-    y3A[q] = segPrefixAt(yBase, c1, (int)(m.y >> 16) - o1, L1);
-    y2A[q] = segPrefixAt(yBase, c2, (int)(m.z & 0xffff) - o2, L2);
-    y4A[q] = segPrefixAt(yBase, c2, (int)(m.z >> 16) - o2, L2);
-    t1A[q] = yBase + 4u * (uint32_t)(4 * c1 * kYSlots + kYTotal);
-    t2A[q] = yBase + 4u * (uint32_t)(4 * c2 * kYSlots + kYTotal);
+    meta[q] = i < nT ? at<uint4>(d, it.d)[i] : make_uint4(0, 0, 0, 0);
   }
-  auto ldF = [&](uint32_t off) { return *reinterpret_cast<const float*>(lds + off); };
-  auto ldD = [&](uint32_t off) { return *reinterpret_cast<const double*>(lds + off); };
 
-  float res[ROWS];
+  float res[ROWS] = {};  // lanes past the list keep 0 (stored into the row's padding)
   float* pend = nullptr;  // the previous node's output row
   auto node = [&](int g, float(&rv)[ROWS]) {
     const TermNode nd = nodes[g];
     if (g > 0) __syncthreads();  // the previous node's queries are done with raw / Xs / Y
-    // 1. stage reach, zeros past the list
+    // 1. stage reach (raw[nO] = 0 is the padding sentinel)
 #pragma unroll
     for (int q = 0; q < ROWS; ++q) {
       const int j = tid + q * NT;
-      raw[j] = j < nO ? rv[q] : 0.0f;
+      if (j < nO) raw[j] = rv[q];
     }
+    if (tid == 0) raw[nO] = 0.0f;
     if (g > 0) storePadded<ROWS, NT>(pend, res, waveBase, padT);
     __syncthreads();
     loadReach<ROWS, NT>(rv, base, nodes, g + 1, cnt, nO);  // the next node's reach
 
     // 2. gathers; strength-order scan across the block
     float gv[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) gv[k] = ldF(gA[k]);
+    gv[0] = raw[gs.x & 0xffffu];
+    gv[1] = raw[gs.x >> 16];
+    gv[2] = raw[gs.y & 0xffffu];
+    gv[3] = raw[gs.y >> 16];
     const double own = (((double)gv[0] + (double)gv[1]) + (double)gv[2]) + (double)gv[3];
     const double inc = waveInclusiveSum(own);
-    if ((tid & 63) == 63) waveTot[wave] = inc;
+    if (lane == 63) waveTot[wave] = inc;
     float sv[kFoldPart];
+    double segSum = 0.0;
 #pragma unroll
-    for (int k = 0; k < kFoldPart; ++k) sv[k] = ldF(idx[k]);
-    // (reach values are never -0.0, so starting from the first term is
-    // starting from 0.0)
-    double segSum = (double)sv[0];
-#pragma unroll
-    for (int k = 1; k < kFoldPart; ++k) segSum += (double)sv[k];
+    for (int k = 0; k < kFoldPart; ++k) {
+      sv[k] = raw[idx[k]];
+      segSum += (double)sv[k];
+    }
     __syncthreads();
     double run = inc - own;
 #pragma unroll
@@ -634,42 +571,48 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
       Xs[k * NT + tid] = run;
       run += (double)gv[k];
     }
-    // 3. segment-relative prefixes: the quad's four parts, in order; every
-    // lane of the first kBlock writes its 12 slots (past its part's length
-    // they repeat the part's end and are never read)
+    // 3. segment-relative prefixes: the quad's four parts, in order
     const double p0 = dppD<0x00>(segSum), p1 = dppD<0x55>(segSum);
     const double p2 = dppD<0xaa>(segSum), p3 = dppD<0xff>(segSum);
-    const double p01 = p0 + p1, p012 = p01 + p2;
-    if (waveBase < kBlock) {
-      double y = part == 0 ? 0.0 : part == 1 ? p0 : part == 2 ? p01 : p012;
-      float yv[kFoldPart];
+    if (sc < 52) {
+      double y = part == 0 ? 0.0 : part == 1 ? p0 : part == 2 ? p0 + p1 : (p0 + p1) + p2;
+      float* yo = Y + segLo + sc;
 #pragma unroll
       for (int k = 0; k < kFoldPart; ++k) {
-        yv[k] = (float)y;
+        if (k < segLen) yo[k] = (float)y;
         y += (double)sv[k];
       }
-      float* yo = Y + tid * kYSlots;
-#pragma unroll
-      for (int k = 0; k < kFoldPart; ++k) yo[k] = yv[k];
-      yo[12] = (float)(p012 + p3);
+      if (part == 3) {
+        const float t = (float)(((p0 + p1) + p2) + p3);
+        yo[segLen] = t;
+        segTot[sc] = t;
+      }
     }
     __syncthreads();
 
     // 4. per traverser combo
+    auto xs = [&](int p) -> double { return Xs[(p & 3) * NT + (p >> 2)]; };
     pend = reinterpret_cast<float*>(d.arena + it.b + nd.out);
     // value = vWin win + vTie (leq - win) + vLose (all - leq)
     //       = kWin win + kLeq leq + vLose all
     const double vLose = nd.v2;
     const double kWin = (double)nd.v0 - (double)nd.v1, kLeq = (double)nd.v1 - vLose;
-    const double total = Xs[(nO & 3) * NT + (nO >> 2)];
+    const double total = xs(nO);
 #pragma unroll
     for (int q = 0; q < ROWS; ++q) {
-      if (waveBase + q * NT >= nT) continue;  // whole waves past the list skip
-      // the nine lookups are issued together, then the arithmetic
-      const float fs = ldF(fsA[q]);
-      const double xl = ldD(xlA[q]), xt = ldD(xtA[q]);
-      const float y1 = ldF(y1A[q]), y2 = ldF(y2A[q]), y3 = ldF(y3A[q]), y4 = ldF(y4A[q]);
-      const float t1 = ldF(t1A[q]), t2 = ldF(t2A[q]);
+      if ((int)tid + q * NT >= nT) continue;  // whole waves past the list skip
+      const uint4 m = meta[q];
+      const int le = m.x & 0xffff, te = m.x >> 16;
+      const int c1 = m.w & 0xff, c2 = (m.w >> 8) & 0xff;
+      const int a1 = (m.y & 0xffff) + c1, b1 = (m.y >> 16) + c1;
+      const int a2 = (m.z & 0xffff) + c2, b2 = (m.z >> 16) + c2;
+      const int same = (int)(m.w >> 16) - 1;
+      // the nine lookups are issued together (no branch on `same`: the
+      // sentinel raw[nO] is 0), then the arithmetic
+      const float fs = raw[same >= 0 ? same : nO];
+      const double xl = xs(le), xt = xs(te);
+      const float y1 = Y[a1], y2 = Y[a2], y3 = Y[b1], y4 = Y[b2];
+      const float t1 = segTot[c1], t2 = segTot[c2];
       __builtin_amdgcn_sched_group_barrier(0x0100, 9, 0);  // LDS reads
       __builtin_amdgcn_sched_group_barrier(0x0002, 64, 0);  // then VALU
       // each card pair is summed in f32 first: the sum is commutative, so a
@@ -708,7 +651,7 @@ __global__ __launch_bounds__(kBlock) void showdownKernel(Dev d, const Item* __re
   extern __shared__ __align__(16) char lds[];
   const Item it = items[blockIdx.x];
   if (masked && !d.ctl[it.f].alive) return;
-  showdownGroup<kBlock>(d, it, lds);
+  showdownGroup<kBlock, 3>(d, it, lds);
 }
 
 // The larger classes (up to 1326 combos per side): 512 lanes.
@@ -718,7 +661,7 @@ __global__ __launch_bounds__(kWideBlock) void showdownWideKernel(Dev d,
   extern __shared__ __align__(16) char lds[];
   const Item it = items[blockIdx.x];
   if (masked && !d.ctl[it.f].alive) return;
-  showdownGroup<kWideBlock>(d, it, lds);
+  showdownGroup<kWideBlock, 3>(d, it, lds);
 }
 
 // Forward launch of one depth: opponent reach of the next depth.
@@ -1132,7 +1075,7 @@ bool wideShowdown(int nT, int nO) { return std::max(nT, nO) > 3 * kBlock; }
 // Dynamic LDS of a terminal item: reach (+ sentinel), and for showdown the
 // prefixes.
 size_t terminalLdsOf(const Item& it, int lanes) {
-  return it.kind == kBwdShowdown ? sdLdsBytes(lanes) : foldLdsBytes(it.m);
+  return it.kind == kBwdShowdown ? sdLdsBytes(it.m, lanes) : foldLdsBytes(it.m);
 }
 
 // Host-side tables of one spot, built from its description.
