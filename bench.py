@@ -33,8 +33,10 @@ the line reports how many converged.
                   run for the iteration count the GPU needed, ~15 s
 
   single_spot     the plain turn, large turn and large flop spots, each
-                  alone (500 / 2000 / --flop-iters iterations from the
-                  captured graph), with their own roofline
+                  alone (500 / 2000 / --flop-iters = 1000 iterations from
+                  the captured graph), with their own roofline; the large
+                  flop also with the CPU oracle on the same tree (a bounded
+                  sample of whole iterations, --flop-cpu-seconds)
 
 --workload large-turn / plain-turn / large-flop measures one spot alone
 (2000 / 500 / 1000 iterations from the captured graph) instead.
@@ -336,9 +338,10 @@ def bench_batched(args, capi, configs):
 
     single_spot = None
     if not args.no_single_spot:
-        single_spot = {w: single_spot_record(capi, configs, w, n, args.warmup)
-                       for w, n in (("plain-turn", 500), ("large-turn", 2000),
-                                    ("large-flop", args.flop_iters))}
+        single_spot = {w: single_spot_record(capi, configs, w, n, args.warmup, cs)
+                       for w, n, cs in (("plain-turn", 500, 0.0), ("large-turn", 2000, 0.0),
+                                        ("large-flop", args.flop_iters,
+                                         0.0 if args.no_cpu else args.flop_cpu_seconds))}
 
     n_conv = {k: (v["converged"], v["spots"]) for k, v in
               (("batched", batched), ("single", single), ("serving", serving)) if v}
@@ -462,10 +465,11 @@ def bench_http(spots, clients, max_iters):
     return out
 
 
-def single_spot_record(capi, configs, workload, iters, warmup):
+def single_spot_record(capi, configs, workload, iters, warmup, cpu_seconds=0.0):
     """One spot alone: `iters` iterations replayed from the captured graph
     after `warmup` (graph capture, caches), with the forward/backward
-    roofline of its launches."""
+    roofline of its launches; with cpu_seconds > 0 also the CPU oracle on
+    the same spot (cpu_baseline_single)."""
     spot = {"large-turn": configs.large_turn, "plain-turn": configs.plain_turn,
             "large-flop": configs.large_flop}[workload]()
     sp, cf = spot_cfg(capi, spot)
@@ -491,6 +495,8 @@ def single_spot_record(capi, configs, workload, iters, warmup):
            "roofline": roofline_of(prof)}
     rec["roofline"]["traffic"] = counter_traffic(rec["roofline"]["kernel"].split()[0], workload)
     solver.close()
+    if cpu_seconds > 0:
+        rec["cpu_baseline"] = cpu_baseline_single(spot, cpu_seconds)
     log(f"bench: single spot {workload} {json.dumps(rec)}")
     return rec
 
@@ -553,8 +559,10 @@ def main():
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-single-spot", action="store_true",
                     help="skip the plain turn / large turn / large flop records")
-    ap.add_argument("--flop-iters", type=int, default=200,
+    ap.add_argument("--flop-iters", type=int, default=1000,
                     help="timed iterations of the large flop record")
+    ap.add_argument("--flop-cpu-seconds", type=float, default=8.0,
+                    help="CPU oracle sample on the large flop record (whole iterations)")
     ap.add_argument("--http-spots", type=int, default=64,
                     help="spots of the batched stream sent through the HTTP API (0: skip)")
     ap.add_argument("--http-clients", type=int, default=16)
