@@ -129,7 +129,8 @@ enum ItemKind : uint8_t {
 //   kBwdChance   a dealt-card x parent-slot map (i16: child slot or -1), b
 //                node cfv row, c children's cfv block, d the child serving
 //                each dealt card (u8), m child stride, n traverser combos
-//                (parent), na dealt cards
+//                (parent), na dealt cards, e = every card is served by its
+//                own child (d is not read)
 //   kBwdFold /   a group of fold (or showdown) nodes of one board class:
 //   kBwdShowdown a the group's first TermNode, cnt nodes in the group, b the
 //                spot's chunk (TermNode offsets are relative to it), c the
@@ -267,14 +268,16 @@ __device__ __forceinline__ void fwdItem(const Dev& d, const Item& it, int fromSt
     const float* __restrict__ in = at<float>(d, it.b);
     const uint16_t* __restrict__ tp = at<uint16_t>(d, it.a);
     float* __restrict__ out = at<float>(d, it.c);
-    // whole tiles: zeros into the row's padding
+    // whole tiles: zeros into the row's padding; the loads are issued for
+    // every lane (index clamped into the list) and the lanes past n drop
+    // the value, so no load sits behind a per-lane branch
     const int n = it.n, nw = (int)PPSG_ROW_STRIDE(n);
-    int p[RC];
+    float x[RC];
 #pragma unroll
-    for (int k = 0; k < RC; ++k) p[k] = c0 + k * kWave < n ? tp[c0 + k * kWave] : -1;
+    for (int k = 0; k < RC; ++k) x[k] = in[tp[min(c0 + k * kWave, n - 1)]];
 #pragma unroll
     for (int k = 0; k < RC; ++k)
-      if (c0 + k * kWave < nw) out[c0 + k * kWave] = p[k] >= 0 ? in[p[k]] * it.v0 : 0.0f;
+      if (c0 + k * kWave < nw) out[c0 + k * kWave] = c0 + k * kWave < n ? x[k] * it.v0 : 0.0f;
     return;
   }
   switch (it.na) {
@@ -360,26 +363,35 @@ __device__ void foldGroup(const Dev& d, const Item& it, char* lds) {
   float* raw = reinterpret_cast<float*>(lds);
   const TermNode* nodes = stageNodes(grp, at<TermNode>(d, it.a), cnt);
   const char* base = d.arena + it.b;
+  // class data for the whole group; every load is issued by every lane
+  // (indices clamped into the tables) and the lanes that need no value drop
+  // it: a load under a per-lane condition is compiled to a branch with its
+  // own wait, which turned these 15 loads into 15 round trips per group
   uint32_t tc[ROWS];
+  {
+    const uint32_t* cards = at<uint32_t>(d, it.d);
 #pragma unroll
-  for (int q = 0; q < ROWS; ++q) {
-    const int i = tid + q * kBlock;
-    tc[q] = i < nT ? at<uint32_t>(d, it.d)[i] : 0u;
+    for (int q = 0; q < ROWS; ++q) {
+      const int i = tid + q * kBlock;
+      const uint32_t v = cards[min(i, max(nT - 1, 0))];
+      tc[q] = i < nT ? v : 0u;
+    }
   }
   // this lane's part of its card's segment
   uint16_t idx[kFoldPart];
   {
-    const int c = tid >> 2, part = tid & 3;
-    int lo = 0, len = 0;
-    if (c < 52) {
-      const int32_t* so = at<int32_t>(d, it.c);
-      const int s0 = so[c], sl = so[c + 1] - s0;
-      lo = s0 + (sl * part) / 4;
-      len = s0 + (sl * (part + 1)) / 4 - lo;
-    }
+    const int c = min(tid >> 2, 51), part = tid & 3;
+    const int32_t* so = at<int32_t>(d, it.c);
+    const int s0 = so[c], sl = so[c + 1] - s0;
+    const int lo = s0 + (sl * part) / 4;
+    const int len = (tid >> 2) < 52 ? s0 + (sl * (part + 1)) / 4 - lo : 0;
     const uint16_t* list = at<uint16_t>(d, it.c + kGatherHeader);
+    const int last = max(2 * nO - 1, 0);  // the per-card lists hold 2 nO slots
 #pragma unroll
-    for (int k = 0; k < kFoldPart; ++k) idx[k] = k < len ? list[lo + k] : (uint16_t)nO;
+    for (int k = 0; k < kFoldPart; ++k) {
+      const uint16_t e = list[min(lo + k, last)];
+      idx[k] = k < len ? e : (uint16_t)nO;
+    }
   }
   float res[ROWS];
   float* pend = nullptr;  // the previous node's output row
@@ -504,27 +516,39 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
   const char* base = d.arena + it.b;
 
   // class data for the whole group: 4 strength-order slots, this lane's part
-  // of one card segment, the per-combo bounds
+  // of one card segment, the per-combo bounds. Every load is issued by every
+  // lane (indices clamped into the tables) and the lanes that need no value
+  // drop it: a load under a per-lane condition is compiled to a branch with
+  // its own wait, one round trip after another
   const uint2 gs = at<uint2>(d, it.c + kGatherHeader)[tid];
-  const int sc = tid >> 2, part = tid & 3;
-  int segLo = 0, segLen = 0;
-  if (sc < 52) {
+  const int sc = tid >> 2, part = tid & 3;  // lanes of cards 52.. have no segment
+  int segLo, segLen;
+  {
     const int32_t* so = at<int32_t>(d, it.c);
-    const int s0 = so[sc], sl = so[sc + 1] - s0;
-    segLo = s0 + (sl * part) / 4;
-    segLen = s0 + (sl * (part + 1)) / 4 - segLo;
+    const int cl = min(sc, 51);  // load index only
+    const int s0 = so[cl], sl = so[cl + 1] - s0;
+    segLo = sc < 52 ? s0 + (sl * part) / 4 : 0;
+    segLen = sc < 52 ? s0 + (sl * (part + 1)) / 4 - segLo : 0;
   }
   uint16_t idx[kFoldPart];
   {
     const uint16_t* seg = at<uint16_t>(d, it.c + kGatherHeader + (int64_t)NT * 4 * 2);
+    const int last = max(2 * nO - 1, 0);  // the per-card lists hold 2 nO slots
 #pragma unroll
-    for (int k = 0; k < kFoldPart; ++k) idx[k] = k < segLen ? seg[segLo + k] : (uint16_t)nO;
+    for (int k = 0; k < kFoldPart; ++k) {
+      const uint16_t e = seg[min(segLo + k, last)];
+      idx[k] = k < segLen ? e : (uint16_t)nO;
+    }
   }
   uint4 meta[ROWS];
+  {
+    const uint4* md = at<uint4>(d, it.d);
 #pragma unroll
-  for (int q = 0; q < ROWS; ++q) {
-    const int i = tid + q * NT;
-    meta[q] = i < nT ? at<uint4>(d, it.d)[i] : make_uint4(0, 0, 0, 0);
+    for (int q = 0; q < ROWS; ++q) {
+      const int i = tid + q * NT;
+      const uint4 v = md[min(i, max(nT - 1, 0))];
+      meta[q] = i < nT ? v : make_uint4(0, 0, 0, 0);
+    }
   }
 
   float res[ROWS] = {};  // lanes past the list keep 0 (stored into the row's padding)
@@ -801,12 +825,21 @@ __device__ __forceinline__ void decideTile(const Dev& d, const Item& it, int dry
 // bit for bit, which a suit-isomorphic tree relies on (f32 rounding
 // differences there grow under regret matching and break the symmetry the
 // deal tables assume). Branches go in chunks: all map loads of a chunk,
-// then all value loads, then the adds (keeps many loads in flight).
+// then all value loads, then the adds. Every load is issued whatever its
+// lane needs (indices clamped into range) and the conditions are applied
+// to the loaded values: a load under a per-lane condition is compiled to
+// its own branch with a wait after it, which made each lane's 48 value
+// loads one dependent round trip after another (29 us for a launch with a
+// chance node of a spot alone, 9 us now). A dropped term adds +0.0, which
+// leaves the sum's bits as they are (a sum that starts at +0.0 never
+// becomes -0.0). it.e: the deal table serves every card by its own child
+// (no suit isomorphism), so no child index is loaded.
 __device__ __forceinline__ void chanceTile(const Dev& d, const Item& it) {
   const int i = it.tile * kWave + waveLane();
   const int n = it.n;
   if (i >= (int)PPSG_ROW_STRIDE(n)) return;  // whole tiles: lanes past n store 0
   const bool in = i < n;
+  const uint32_t ic = (uint32_t)min(i, n - 1);  // lanes past n read slot n-1 (then drop it)
   // 32-bit unsigned byte offsets from uniform bases (a branch block is at
   // most 255 x 1326 floats): one VGPR per address instead of two
   const char* __restrict__ map = at<char>(d, it.a);
@@ -814,24 +847,25 @@ __device__ __forceinline__ void chanceTile(const Dev& d, const Item& it) {
   const uint8_t* __restrict__ sib = at<uint8_t>(d, it.d);  // child serving each dealt card
   const uint32_t stride = (uint32_t)it.m;
   const int nb = it.na;
+  const bool ident = it.e != 0;
   constexpr int kChunk = 16;
   double v = 0.0;
   for (int b0 = 0; b0 < nb; b0 += kChunk) {
     int slot[kChunk];
+    uint32_t child[kChunk];
     float x[kChunk];
 #pragma unroll
-    for (int k = 0; k < kChunk; ++k)
-      slot[k] = in && b0 + k < nb
-                    ? *reinterpret_cast<const int16_t*>(map + 2u * ((uint32_t)(b0 + k) * (uint32_t)n + (uint32_t)i))
-                    : -1;
+    for (int k = 0; k < kChunk; ++k) {
+      const uint32_t b = (uint32_t)min(b0 + k, nb - 1);
+      child[k] = ident ? b : (uint32_t)sib[b];
+      slot[k] = *reinterpret_cast<const int16_t*>(map + 2u * (b * (uint32_t)n + ic));
+    }
 #pragma unroll
     for (int k = 0; k < kChunk; ++k)
-      x[k] = slot[k] >= 0
-                 ? *reinterpret_cast<const float*>(kid + 4u * ((uint32_t)sib[b0 + k] * stride + (uint32_t)slot[k]))
-                 : 0.0f;
+      x[k] = *reinterpret_cast<const float*>(kid + 4u * (child[k] * stride + (uint32_t)max(slot[k], 0)));
 #pragma unroll
     for (int k = 0; k < kChunk; ++k)
-      if (slot[k] >= 0) v += (double)x[k];
+      v += (double)(in && b0 + k < nb && slot[k] >= 0 ? x[k] : 0.0f);
   }
   at<float>(d, it.b)[i] = (float)v;
 }
@@ -1311,6 +1345,9 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
         }
         const int first = t->child_flat[kb];
         const int64_t stride = nb > 1 ? t->cfv_off[t->child_flat[kb + 1]] - t->cfv_off[first] : 0;
+        int identDeal = 1;  // every dealt card served by its own child
+        for (int32_t k = 0; k < t->deal_n[cl]; ++k)
+          identDeal &= t->deal_sib[t->deal_off[cl] + k] == k;
         for (int q = 0; q < tiles(nT, 1); ++q) {
           Item it{};
           it.kind = kBwdChance;
@@ -1322,6 +1359,7 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
           it.m = (int32_t)stride;
           it.n = nT;
           it.na = (uint8_t)t->deal_n[cl];
+          it.e = identDeal;
           it.tile = (uint8_t)q;
           bwd.push_back(it);
         }
