@@ -724,20 +724,36 @@ __device__ __forceinline__ void decideTile(const Dev& d, const Item& it, int dry
   const float* __restrict__ kid = at<float>(d, it.c);
   float* __restrict__ out = at<float>(d, it.b);
   if (!it.e || MODE == PPSG_MODE_BR) {
-    // opponent acts: the child values add up; best response: their maximum
+    // opponent acts: the child values add up (in action order); best
+    // response: their maximum. With the action count fixed at compile time
+    // every child load is issued before the first add.
     const bool sum = !it.e;
     float v[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) v[k] = sum ? 0.0f : -1e30f;
-    for (int a = 0; a < na; ++a)
+    if (NA > 0) {
+      constexpr int A = NA > 0 ? NA : 1;
+      float x[R][A];
 #pragma unroll
-      for (int k = 0; k < R; ++k) {
-        const int c = c0 + k * kWave;
-        if (c < n) {
-          const float x = kid[a * stride + c];
-          v[k] = sum ? v[k] + x : fmaxf(v[k], x);
+      for (int k = 0; k < R; ++k)
+#pragma unroll
+        for (int a = 0; a < A; ++a)
+          x[k][a] = c0 + k * kWave < n ? kid[a * stride + c0 + k * kWave] : 0.0f;
+#pragma unroll
+      for (int k = 0; k < R; ++k)
+#pragma unroll
+        for (int a = 0; a < A; ++a) v[k] = sum ? v[k] + x[k][a] : fmaxf(v[k], x[k][a]);
+    } else {
+      for (int a = 0; a < na; ++a)
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          const int c = c0 + k * kWave;
+          if (c < n) {
+            const float x = kid[a * stride + c];
+            v[k] = sum ? v[k] + x : fmaxf(v[k], x);
+          }
         }
-      }
+    }
 #pragma unroll
     for (int k = 0; k < R; ++k)
       if (c0 + k * kWave < n) out[c0 + k * kWave] = v[k];
