@@ -51,8 +51,9 @@ namespace {
 constexpr int kBlock = 256;  // 4 wave64s
 constexpr int kWideBlock = 512;  // wide showdown groups: 8 wave64s
 // Decide and chance work items are per wave: a 256-lane block runs four
-// consecutive items, one per wave, so a node's last partial chunk wastes at
-// most 63 lanes (not 255) and the waves of a block may serve different nodes.
+// consecutive items, one per wave, so a node's last tile spans at most 63
+// lanes of row padding (not 255) and the waves of a block may serve
+// different nodes.
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kMaxList = 1326;
