@@ -2635,31 +2635,32 @@ int ppsg_copy_probe(int64_t bytes, int reps, double* ms_per_launch) {
 
 // ---- one spot: a batch of one, in slot 0 ----
 
-// Terminal group size for an engine holding one spot: the largest of 1, 2,
-// ..., 32 nodes per work item that still leaves at least kGroupItems work
-// items in the spot's largest terminal launch (a traverser's fold, showdown
-// or wide-showdown groups). More nodes per item spread a class's table
-// loads over more nodes; fewer keep the launch wide enough to fill the GPU
-// (measured per iteration, sizes 1-64: plain turn best at 1-2, large turn
-// at 3-6, large flop at 32-64, which this rule picks).
-constexpr int64_t kGroupItems = 6 * 256;
+// Terminal group size for an engine holding one spot, from the mean number
+// of nodes per board class in its largest terminal launch (a traverser's
+// fold, showdown or wide-showdown groups): more nodes per work item spread
+// a class's table loads over more nodes, fewer keep the launch wide. Per
+// iteration, sizes 1-64 measured on one spot each: 17 nodes per class
+// (plain turn) best at 1-2, 50 (batched-turn spots) at 4, 181 (large turn)
+// at 3-6, 553 (large flop) at 32-64.
 static int singleSpotGroupSize(const Spot& S) {
-  auto items = [&](int g) {
-    int64_t most = 0;
-    for (int tr = 0; tr < 2; ++tr)
-      for (int kind = 0; kind < 3; ++kind) {  // fold, showdown, wide showdown
-        int64_t n = 0;
-        for (const TermGroup& G : S.groups[tr]) {
-          const int k = G.kind == kBwdFold ? 0 : G.wide ? 2 : 1;
-          if (k == kind) n += (G.count + g - 1) / g;
-        }
-        most = std::max(most, n);
+  int64_t nodes = 0, groups = 0;
+  for (int tr = 0; tr < 2; ++tr)
+    for (int kind = 0; kind < 3; ++kind) {  // fold, showdown, wide showdown
+      int64_t n = 0, g = 0;
+      for (const TermGroup& G : S.groups[tr]) {
+        const int k = G.kind == kBwdFold ? 0 : G.wide ? 2 : 1;
+        if (k != kind) continue;
+        n += G.count;
+        ++g;
       }
-    return most;
-  };
-  int g = 1;
-  while (g < 32 && items(2 * g) >= kGroupItems) g *= 2;
-  return g;
+      if (n > nodes) {
+        nodes = n;
+        groups = g;
+      }
+    }
+  if (groups == 0) return 4;
+  const int64_t perClass = nodes / groups;
+  return perClass >= 400 ? 32 : perClass < 24 ? 2 : 4;
 }
 
 int ppsg_create(const ppsg_tree_desc* t, ppsg_t** out) {

@@ -254,9 +254,9 @@ int ppsg_batch_compact(ppsg_t* h, double max_dead_fraction, int* compacted);
 int ppsg_kind_bytes(ppsg_t* h, double out[6]);
 /* Launch shape (no reference counterpart; results do not depend on it).
    PPSG_TUNE_GROUP: fold/showdown nodes of one board class evaluated by one
-   workgroup (1..64; default 4 for a batch; ppsg_create picks the largest
-   power of two up to 32 that leaves 1,536 work items in the spot's largest
-   terminal launch). Larger groups load a class's gather list and per-combo
+   workgroup (1..64; default 4 for a batch; ppsg_create picks 2, 4 or 32
+   from the spot's mean terminal nodes per board class: below 24, up to
+   400, above). Larger groups load a class's gather list and per-combo
    metadata once for more nodes and launch fewer workgroups; smaller groups
    spread a small batch over more CUs. */
 int ppsg_tune(ppsg_t* h, int param, int value);
