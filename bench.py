@@ -22,7 +22,11 @@ the line reports how many converged.
                   (a batch of one), same definition of time
   serving         a Poisson stream (1000 requests at 40/s, 70% turn spots as
                   above, 30% river spots): p50/p99 arrival-to-retirement
-                  latency and aggregate iters/s over the stream's wall time
+                  latency and aggregate iters/s over the stream's wall time;
+                  serving.capacity: the same requests arriving at 1000/s
+                  (throughput with the GPU kept full; latency is then
+                  queueing), serving.one_at_a_time: the same requests one at
+                  a time through the same engine, and their ratio
   roofline        the dominant kernel of the resident 256-spot batch (the
                   one with the most time per iteration), timed launch by
                   launch with HIP events on the engine's stream: algorithmic
@@ -320,6 +324,26 @@ def bench_batched(args, capi, configs):
         serving["qps"] = args.qps
         serving["mix"] = "70% batched-turn spots, 30% river spots"
         log(f"bench: serving {json.dumps(serving)}")
+        if args.capacity_qps > 0:
+            # the same requests arriving at capacity_qps (the generator draws
+            # the same spots for any rate; only the gaps scale): the loop's
+            # throughput when the GPU is kept full, latency then queueing
+            arr = [s["arrival_ms"] * args.qps / args.capacity_qps for s in stream]
+            res3, rep3 = capi.serve([p[0] for p in sp], [p[1] for p in sp], "dcfr", arr,
+                                    **dict(opt, real_time=1, prefill=0))
+            cap = serve_stats(res3, rep3)
+            cap["qps"] = args.capacity_qps
+            # and the same requests one at a time through the same engine
+            res4, rep4 = capi.serve([p[0] for p in sp], [p[1] for p in sp], "dcfr", None,
+                                    **dict(opt, max_live=1))
+            one = serve_stats(res4, rep4)
+            serving["capacity"] = cap
+            serving["one_at_a_time"] = {k: one[k] for k in
+                                        ("converged", "iterations", "busy_s", "iters_per_s_busy",
+                                         "p50_ms", "p99_ms")}
+            serving["capacity_vs_one_at_a_time"] = round(
+                cap["iters_per_s_busy"] / one["iters_per_s_busy"], 3)
+            log(f"bench: serving capacity {json.dumps(cap)}; one at a time {json.dumps(one)}")
 
     http = None
     if args.http_spots > 0:
@@ -556,6 +580,9 @@ def main():
     ap.add_argument("--spots", type=int, default=256)
     ap.add_argument("--serve-spots", type=int, default=1000)
     ap.add_argument("--qps", type=float, default=40.0)
+    ap.add_argument("--capacity-qps", type=float, default=1000.0,
+                    help="the serving stream again at this arrival rate (its throughput when "
+                         "the GPU is kept full) and one at a time; 0: skip")
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-single-spot", action="store_true",
                     help="skip the plain turn / large turn / large flop records")
