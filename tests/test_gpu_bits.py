@@ -1,0 +1,58 @@
+"""Bit-for-bit pins of the engine's results on fixed solves.
+
+The engine's sums have a fixed order, so a solve is a pure function of its
+inputs: these tests record the SHA-256 of both node-order tables (regret,
+then strategy sums, f32) after fixed solves of the plain and large turn
+spots, and the exact exploitability and EV of a served batch. A change that
+only moves work around (launch shapes, tiles, memory layout, load order)
+must leave them as they are; a change to the arithmetic shows up here
+first and must pass the oracle parity tests (test_gpu_parity.py) before the
+recorded values are updated.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+from pps import capi, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def spot_cfg(s):
+    return (capi.make_spot(s["board"], s["w0"], s["w1"], s["pot"], s["stack"]),
+            capi.make_config(s["bets"], s["raises"], s["allin"], False, s["max_raises"]))
+
+
+@pytest.mark.parametrize("name,iters,digest,ev_oop,expl", [
+    ("plain-turn", 100, "cc053de370756b001519e4519520f7ba39ddf1e6bde4cc15caf237a787b228ab",
+     47.81108402089765, 0.08781354679636877),
+    ("large-turn", 60, "5e709bc7ce5aa73f9315be6b257c3ff8bcb4066639aba12724dc15b1b1964006",
+     26.22252828607608, 0.8788157595749126),
+])
+def test_solve_tables_are_pinned(name, iters, digest, ev_oop, expl):
+    spot = {"plain-turn": configs.plain_turn, "large-turn": configs.large_turn}[name]()
+    s = capi.Solver(*spot_cfg(spot))
+    try:
+        s.solve(iters, spot["algo"])
+        h = hashlib.sha256()
+        for which in (0, 1):
+            h.update(np.asarray(s.rows(which), dtype=np.float32).tobytes())
+        st = s.stats()
+    finally:
+        s.close()
+    assert h.hexdigest() == digest
+    assert st["ev_oop"] == ev_oop and st["exploitability"] == expl
+
+
+def test_served_batch_is_pinned():
+    spots = configs.batch_stream(4)
+    pairs = [spot_cfg(s) for s in spots]
+    res, _ = capi.serve([p[0] for p in pairs], [p[1] for p in pairs], "dcfr", None,
+                        target_frac=0.005, check_every=32, max_iters=4096, real_time=0,
+                        prefill=1)
+    got = [(r["iterations"], r["expl"], r["ev0"]) for r in res]
+    assert got == [(128, 0.5087439806226541, 71.6570791858915),
+                   (64, 0.28165960617813823, 54.26376348232555),
+                   (64, 0.5142293226740549, 58.04267755971645),
+                   (64, 0.5652470131910121, 56.25281028438216)]
