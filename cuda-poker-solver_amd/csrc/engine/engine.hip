@@ -63,6 +63,9 @@ constexpr int kLists = 5;  // per traverser: forward, backward, fold, showdown, 
 constexpr int kGatherHeader = 256;  // bytes: 53 int32 segment bounds + pad
 // fold: entries of a card's segment per lane (4 lanes per card)
 constexpr int kFoldPart = 12;
+// a lane's part is gathered in chunks of this many entries; a wave skips the
+// chunks past its longest part
+constexpr int kPartChunk = 3;
 constexpr int kMaxGroup = 64;  // terminal nodes per work item at most
 // Combos per lane (kWave apart) of forward decide, forward chance and
 // backward decide items: a per-depth launch shape. Spots are planned at one
@@ -316,6 +319,26 @@ __device__ __forceinline__ void loadReach(float (&rv)[ROWS], const char* base,
   for (int q = 0; q < ROWS; ++q) rv[q] = r[min((int)threadIdx.x + q * NT, nO - 1)];
 }
 
+// The largest x over the wave, as a wave-uniform (scalar) value.
+__device__ __forceinline__ int waveMax(int x) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) x = max(x, __shfl_xor(x, off, 64));
+  return __builtin_amdgcn_readfirstlane(x);
+}
+
+// Gathers raw[idx[k]] for the chunks of a lane's card-segment part that the
+// wave's longest part (partMax, uniform) reaches. The entries past a lane's
+// own part hold the zero sentinel slot, so the skipped chunks are terms of
+// +0.0 and every sum over sv is the same with or without them.
+__device__ __forceinline__ void gatherPart(float (&sv)[kFoldPart], const float* raw,
+                                           const uint16_t (&idx)[kFoldPart], int partMax) {
+#pragma unroll
+  for (int k = 0; k < kFoldPart; ++k) {
+    if (k % kPartChunk == 0 && k >= partMax) break;
+    sv[k] = raw[idx[k]];
+  }
+}
+
 // A group's node descriptors, copied into LDS once: reading them from the
 // arena inside the node loop would be vector loads ordered behind every
 // output store in flight (the arena is also written), so the next node's
@@ -380,6 +403,7 @@ __device__ void foldGroup(const Dev& d, const Item& it, char* lds) {
   }
   // this lane's part of its card's segment
   uint16_t idx[kFoldPart];
+  int partMax;  // the wave's longest part
   {
     const int c = min(tid >> 2, 51), part = tid & 3;
     const int32_t* so = at<int32_t>(d, it.c);
@@ -393,6 +417,7 @@ __device__ void foldGroup(const Dev& d, const Item& it, char* lds) {
       const uint16_t e = list[min(lo + k, last)];
       idx[k] = k < len ? e : (uint16_t)nO;
     }
+    partMax = waveMax(len);
   }
   float res[ROWS];
   float* pend = nullptr;  // the previous node's output row
@@ -408,9 +433,14 @@ __device__ void foldGroup(const Dev& d, const Item& it, char* lds) {
     if (g > 0) storePadded<ROWS, kBlock>(pend, res, waveBase, padT);
     __syncthreads();
     loadReach<ROWS>(rv, base, nodes, g + 1, cnt, nO);  // the next node's reach
+    float sv[kFoldPart];
+    gatherPart(sv, raw, idx, partMax);
     double part = 0.0;
 #pragma unroll
-    for (int k = 0; k < kFoldPart; ++k) part += (double)raw[idx[k]];
+    for (int k = 0; k < kFoldPart; ++k) {
+      if (k % kPartChunk == 0 && k >= partMax) break;
+      part += (double)sv[k];
+    }
     part += __shfl_xor(part, 1, 64);
     part += __shfl_xor(part, 2, 64);
     if ((tid & 3) == 0 && (tid >> 2) < 52) cs[tid >> 2] = part;
@@ -541,6 +571,10 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
       idx[k] = k < segLen ? e : (uint16_t)nO;
     }
   }
+  const int partMax = waveMax(segLen);  // the wave's longest part
+  // the wave's strength-order positions (256 w .. 256 w + 255) reach the
+  // list's end nO: waves past it have nothing to gather or scan
+  const bool scanWave = 4 * waveBase <= nO;
   uint4 meta[ROWS];
   {
     const uint4* md = at<uint4>(d, it.d);
@@ -569,32 +603,38 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
     loadReach<ROWS, NT>(rv, base, nodes, g + 1, cnt, nO);  // the next node's reach
 
     // 2. gathers; strength-order scan across the block
-    float gv[4];
-    gv[0] = raw[gs.x & 0xffffu];
-    gv[1] = raw[gs.x >> 16];
-    gv[2] = raw[gs.y & 0xffffu];
-    gv[3] = raw[gs.y >> 16];
-    const double own = (((double)gv[0] + (double)gv[1]) + (double)gv[2]) + (double)gv[3];
-    const double inc = waveInclusiveSum(own);
-    if (lane == 63) waveTot[wave] = inc;
+    float gv[4] = {};
+    double own = 0.0, inc = 0.0;
+    if (scanWave) {
+      gv[0] = raw[gs.x & 0xffffu];
+      gv[1] = raw[gs.x >> 16];
+      gv[2] = raw[gs.y & 0xffffu];
+      gv[3] = raw[gs.y >> 16];
+      own = (((double)gv[0] + (double)gv[1]) + (double)gv[2]) + (double)gv[3];
+      inc = waveInclusiveSum(own);
+      if (lane == 63) waveTot[wave] = inc;
+    }
     float sv[kFoldPart];
+    gatherPart(sv, raw, idx, partMax);
     double segSum = 0.0;
 #pragma unroll
     for (int k = 0; k < kFoldPart; ++k) {
-      sv[k] = raw[idx[k]];
+      if (k % kPartChunk == 0 && k >= partMax) break;
       segSum += (double)sv[k];
     }
     __syncthreads();
-    double run = inc - own;
+    if (scanWave) {  // the waves below a scanning wave scan too
+      double run = inc - own;
 #pragma unroll
-    for (int w = 0; w < NT / 64 - 1; ++w) {
-      const double t = waveTot[w];
-      run += w < wave ? t : 0.0;  // +0.0 past this wave: the same sum, no loop
-    }
+      for (int w = 0; w < NT / 64 - 1; ++w) {
+        const double t = waveTot[w];
+        run += w < wave ? t : 0.0;  // +0.0 past this wave: the same sum, no loop
+      }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      Xs[k * NT + tid] = run;
-      run += (double)gv[k];
+      for (int k = 0; k < 4; ++k) {
+        Xs[k * NT + tid] = run;
+        run += (double)gv[k];
+      }
     }
     // 3. segment-relative prefixes: the quad's four parts, in order
     const double p0 = dppD<0x00>(segSum), p1 = dppD<0x55>(segSum);
@@ -604,6 +644,7 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
       float* yo = Y + segLo + sc;
 #pragma unroll
       for (int k = 0; k < kFoldPart; ++k) {
+        if (k % kPartChunk == 0 && k >= partMax) break;
         if (k < segLen) yo[k] = (float)y;
         y += (double)sv[k];
       }
