@@ -290,7 +290,9 @@ def bench_batched(args, capi, configs):
     log(f"bench: profile {json.dumps(prof)}")
     opt["shapes"] = prof["shapes"]
 
-    res, rep = capi.serve(S, C, "dcfr", None, **opt)
+    # the batch runs with the terminal split tuned on it; the other runs keep
+    # one stream (few spots live: launch latency, not HBM, bounds them)
+    res, rep = capi.serve(S, C, "dcfr", None, **dict(opt, term_split=prof["term_split"]))
     bad = [r["error"] for r in res if not r["ok"]]
     if bad:
         raise RuntimeError(f"batched run rejected spots: {bad[:3]}")

@@ -1233,6 +1233,8 @@ struct Spot {
   std::vector<std::vector<Item>> items[2][2];
   std::vector<std::vector<int32_t>> itemNode[2][2];
   std::vector<int32_t> termNode;  // node of each terminal entry (group order)
+  // depth of each terminal entry; a group's entries run by depth, then node
+  std::vector<int16_t> termDepth;
   int64_t nodes = 0;
   std::vector<TermGroup> groups[2];  // per traverser
   int64_t itemCount = 0;  // listed items (terminal items at the group size in use)
@@ -1253,8 +1255,11 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
     b = t->child_base[u];
     e = u + 1 < N ? t->child_base[u + 1] : (int)t->n_child;
   };
+  if (t->max_depth < 0 || t->max_depth > 32767) return fail("tree description: max_depth out of range");
   // rows on 64-float lines; a decide node's action rows PPSG_ROW_STRIDE apart
   for (int64_t u = 0; u < N; ++u) {
+    if (t->depth[u] < 0 || t->depth[u] > t->max_depth)
+      return fail("tree description: node depth out of range");
     if (t->cfv_off[u] % 64 || t->reach_off[0][u] % 64 || t->reach_off[1][u] % 64)
       return fail("tree description: a row does not start on a 64-float line");
     if (t->kind[u] == PPSG_DECIDE &&
@@ -1457,13 +1462,23 @@ int planSpot(const ppsg_tree_desc* t, Spot& S, SpotTables& T) {
       S.itemNode[tr][1][dep].resize(bwd.size(), (int32_t)u);
     }
   }
-  // terminal node entries, contiguous per group, after the rows
+  // terminal node entries, contiguous per group, after the rows; within a
+  // group by depth, then node, so the entries at depth <= k are a prefix of
+  // it for every k (the split terminal lists)
   for (int tr = 0; tr < 2; ++tr)
     for (size_t g = 0; g < S.groups[tr].size(); ++g) {
+      const std::vector<int32_t>& ids = memberNode[tr][g];
+      std::vector<int32_t> ord(ids.size());
+      for (size_t k = 0; k < ord.size(); ++k) ord[k] = (int32_t)k;
+      std::stable_sort(ord.begin(), ord.end(),
+                       [&](int32_t x, int32_t y) { return t->depth[ids[x]] < t->depth[ids[y]]; });
       S.groups[tr][g].first = (int64_t)T.termNodes.size();
-      S.groups[tr][g].count = (int)members[tr][g].size();
-      T.termNodes.insert(T.termNodes.end(), members[tr][g].begin(), members[tr][g].end());
-      S.termNode.insert(S.termNode.end(), memberNode[tr][g].begin(), memberNode[tr][g].end());
+      S.groups[tr][g].count = (int)ids.size();
+      for (const int32_t k : ord) {
+        T.termNodes.push_back(members[tr][g][k]);
+        S.termNode.push_back(ids[k]);
+        S.termDepth.push_back((int16_t)t->depth[ids[k]]);
+      }
     }
   L.termNodes = L.add((int64_t)(T.termNodes.size() * sizeof(TermNode)));
   S.bytes = L.bytes;
@@ -1655,9 +1670,16 @@ struct ppsg_engine {
   int maxSpots = 0;
   std::vector<Spot> spots;
   // [traverser][0 forward / 1 backward per depth, 2 fold groups, 3 showdown
-  // groups, 4 wide showdown groups (one list each)]
+  // groups, 4 wide showdown groups (two lists each: [0] the entries deeper
+  // than termSplit, [1] those at depth <= termSplit)]
   std::vector<DevList> lists[2][kLists];
   int groupSize = 4;  // terminal nodes per work item
+  // Terminals at depth <= termSplit run on the second stream as soon as
+  // the forward pass has produced their reach rows, beside the deeper
+  // forward, terminal and backward launches (0: one stream)
+  int termSplit = 0;
+  hipStream_t side = nullptr;
+  hipEvent_t forkEv = nullptr, joinEv = nullptr;
   // per-depth launch shapes (combos per lane), [traverser][depth]: forward
   // decide, forward chance, backward decide
   std::vector<uint8_t> shapeFD[2], shapeFC[2], shapeB[2];
@@ -1726,9 +1748,14 @@ struct ppsg_engine {
   int prepare();
   void dropGraph();
   int launchFwd(int tr, int dpt, int fromStrat, int kindMask, hipStream_t s);
-  int launchTerm(int tr, int ps, hipStream_t s);
+  int launchTerm(int tr, int ps, int part, hipStream_t s);
   int launchBwd(int tr, int dpt, int mode, int kindMask, hipStream_t s, int dry = 0);
-  int launchHalf(int tr, int mode, hipStream_t s);
+  int launchHalf(int tr, int mode, hipStream_t s, int dry = 0);
+  // entries of group g (of spot S) at depth <= termSplit: a prefix of it
+  int shallowCount(const Spot& S, const TermGroup& g) const {
+    const auto b = S.termDepth.begin() + g.first;
+    return (int)(std::upper_bound(b, b + g.count, (int16_t)std::min(termSplit, 32767)) - b);
+  }
   int capture();
   int covers(int64_t iters);
   int64_t kernelsPerIter() const;
@@ -1761,6 +1788,9 @@ ppsg_engine::~ppsg_engine() {
   if (ctotals) (void)hipFree(ctotals);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
+  if (forkEv) (void)hipEventDestroy(forkEv);
+  if (joinEv) (void)hipEventDestroy(joinEv);
+  if (side) (void)hipStreamDestroy(side);
   if (stream) (void)hipStreamDestroy(stream);
   if (ctl) (void)hipFree(ctl);
   if (sumStage) (void)hipFree(sumStage);
@@ -1791,8 +1821,11 @@ int ppsg_engine::init(int64_t bytes, int spotsCap) {
   maxSpots = spotsCap;
   spots.assign(spotsCap, Spot{});
   HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&ev0));
   HIP_TRY(hipEventCreate(&ev1));
+  HIP_TRY(hipEventCreateWithFlags(&forkEv, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&joinEv, hipEventDisableTiming));
   return 0;
 }
 
@@ -1837,10 +1870,13 @@ std::vector<Item> ppsg_engine::spotItems(int slot, int tr, int ps, int dep) cons
     }
     return v;
   }
+  // terminal lists: dep 0 = the entries deeper than termSplit, 1 = the rest
   const uint8_t kind = ps == 2 ? kBwdFold : kBwdShowdown;
   for (const TermGroup& g : S.groups[tr]) {
     if (g.kind != kind || (ps == 4) != g.wide) continue;
-    for (int s0 = 0; s0 < g.count; s0 += groupSize) {
+    const int cut = shallowCount(S, g);
+    const int lo = dep == 0 ? cut : 0, hi = dep == 0 ? g.count : cut;
+    for (int s0 = lo; s0 < hi; s0 += groupSize) {
       Item it{};
       it.kind = g.kind;
       it.a = S.off + S.L.termNodes + (int64_t)sizeof(TermNode) * (g.first + s0);
@@ -1850,7 +1886,7 @@ std::vector<Item> ppsg_engine::spotItems(int slot, int tr, int ps, int dep) cons
       it.n = g.nT;
       it.m = g.nO;
       it.na = g.rows;
-      it.cnt = std::min(groupSize, g.count - s0);
+      it.cnt = std::min(groupSize, hi - s0);
       it.f = slot;
       v.push_back(it);
     }
@@ -1870,7 +1906,10 @@ int64_t ppsg_engine::countItems(int slot) const {
       for (size_t dep = 0; dep < S.items[tr][ps].size(); ++dep)
         for (const Item& it : S.items[tr][ps][dep])
           n += it.tile % shapeOf(tr, (int)dep, it.kind) == 0;
-    for (const TermGroup& g : S.groups[tr]) n += (g.count + groupSize - 1) / groupSize;
+    for (const TermGroup& g : S.groups[tr]) {
+      const int cut = shallowCount(S, g);
+      n += (cut + groupSize - 1) / groupSize + (g.count - cut + groupSize - 1) / groupSize;
+    }
   }
   return n;
 }
@@ -1908,7 +1947,7 @@ int ppsg_engine::appendSpot(int slot) {
   for (int tr = 0; tr < 2; ++tr)
     for (int ps = 0; ps < kLists; ++ps) {
       std::vector<DevList>& L = lists[tr][ps];
-      const int nl = ps < 2 ? D : 1;
+      const int nl = ps < 2 ? D : 2;
       if ((int)L.size() < nl) L.resize(nl);
       for (int dep = 0; dep < nl; ++dep) {
         const std::vector<Item> v = spotItems(slot, tr, ps, dep);
@@ -2045,9 +2084,11 @@ int ppsg_engine::rebuild() {
   for (int tr = 0; tr < 2; ++tr)
     for (int ps = 0; ps < kLists; ++ps) {
       std::vector<DevList>& L = lists[tr][ps];
-      const int nl = ps < 2 ? D : (any ? 1 : 0);
-      for (int dep = nl; dep < (int)L.size(); ++dep)
+      const int nl = ps < 2 ? D : (any ? 2 : 0);
+      for (int dep = nl; dep < (int)L.size(); ++dep) {
         if (L[dep].buf) (void)hipFree(L[dep].buf);
+        if (L[dep].spare) (void)hipFree(L[dep].spare);
+      }
       L.resize(nl);
       for (int dep = 0; dep < nl; ++dep) {
         std::vector<Item> v;
@@ -2231,9 +2272,9 @@ int ppsg_engine::launchFwd(int tr, int dpt, int fromStrat, int kindMask, hipStre
   return 0;
 }
 
-int ppsg_engine::launchTerm(int tr, int ps, hipStream_t s) {
-  if (lists[tr][ps].empty()) return 0;
-  const DevList& l = lists[tr][ps][0];
+int ppsg_engine::launchTerm(int tr, int ps, int part, hipStream_t s) {
+  if ((int)lists[tr][ps].size() <= part) return 0;
+  const DevList& l = lists[tr][ps][part];
   if (l.count == 0) return 0;
   if (ps == 2) foldKernel<<<(unsigned)l.count, kBlock, l.lds, s>>>(dev(), l.buf, masked());
   else if (ps == 3) showdownKernel<<<(unsigned)l.count, kBlock, l.lds, s>>>(dev(), l.buf, masked());
@@ -2251,15 +2292,37 @@ int ppsg_engine::launchBwd(int tr, int dpt, int mode, int kindMask, hipStream_t 
   return 0;
 }
 
-int ppsg_engine::launchHalf(int tr, int mode, hipStream_t s) {
+// One half-step (or walk) of traverser tr. With a terminal split k, the
+// terminals at depth <= k (read first by the backward launch of depth k - 1)
+// fork onto the second stream after the forward launch of depth k - 1 (the
+// last one their reach rows need) and join the main stream before that
+// backward launch; everything else runs in order on s. dry: backward
+// launches store the old regret and strategy values (timing only).
+int ppsg_engine::launchHalf(int tr, int mode, hipStream_t s, int dry) {
   const int fromStrat = mode == PPSG_MODE_CFR ? 0 : 1;
   const int D = depthCount();
-  for (int dpt = 0; dpt < D; ++dpt)
+  const int k = std::min(termSplit, D);
+  bool split = false;
+  for (int ps = 2; ps < kLists && k > 0; ++ps)
+    split |= (int)lists[tr][ps].size() > 1 && lists[tr][ps][1].count > 0;
+  for (int dpt = 0; dpt < D; ++dpt) {
     if (launchFwd(tr, dpt, fromStrat, -1, s) != 0) return -1;
-  for (int ps = 2; ps < kLists; ++ps)
-    if (launchTerm(tr, ps, s) != 0) return -1;
-  for (int dpt = D - 1; dpt >= 0; --dpt)
-    if (launchBwd(tr, dpt, mode, -1, s) != 0) return -1;
+    if (split && dpt == k - 1) {
+      HIP_TRY(hipEventRecord(forkEv, s));
+      HIP_TRY(hipStreamWaitEvent(side, forkEv, 0));
+      for (int ps = 2; ps < kLists; ++ps)
+        if (launchTerm(tr, ps, 1, side) != 0) return -1;
+      HIP_TRY(hipEventRecord(joinEv, side));
+    }
+  }
+  for (int ps = 2; ps < kLists; ++ps) {
+    if (launchTerm(tr, ps, 0, s) != 0) return -1;
+    if (!split && launchTerm(tr, ps, 1, s) != 0) return -1;
+  }
+  for (int dpt = D - 1; dpt >= 0; --dpt) {
+    if (split && dpt == k - 1) HIP_TRY(hipStreamWaitEvent(s, joinEv, 0));
+    if (launchBwd(tr, dpt, mode, -1, s, dry) != 0) return -1;
+  }
   return 0;
 }
 
@@ -2844,16 +2907,17 @@ int ppsg_profile(ppsg_t* h, int iters, double* out, int cap) {
                      13 + 2 * k);
       }
       double lastFwd = acc[0];
-      for (int ps = 2; ps < kLists && rc == 0; ++ps) {
-        if (h->lists[tr][ps].empty() || !h->lists[tr][ps][0].count) continue;
-        rc = timed([&] { return h->launchTerm(tr, ps, h->stream); }, 0);
-        // fold and showdown have launches of their own: always attributed
-        if (rc == 0 && cap >= 13) {
-          acc[ps == 2 ? 9 : 11] += acc[0] - lastFwd;
-          acc[ps == 2 ? 10 : 12] += 1;
+      for (int ps = 2; ps < kLists && rc == 0; ++ps)
+        for (int part = 0; part < (int)h->lists[tr][ps].size() && rc == 0; ++part) {
+          if (!h->lists[tr][ps][part].count) continue;
+          rc = timed([&] { return h->launchTerm(tr, ps, part, h->stream); }, 0);
+          // fold and showdown have launches of their own: always attributed
+          if (rc == 0 && cap >= 13) {
+            acc[ps == 2 ? 9 : 11] += acc[0] - lastFwd;
+            acc[ps == 2 ? 10 : 12] += 1;
+          }
+          lastFwd = acc[0];
         }
-        lastFwd = acc[0];
-      }
       for (int dpt = D - 1; dpt >= 0 && rc == 0; --dpt) {
         if (!h->lists[tr][1][dpt].count) continue;
         rc = timed([&] { return h->launchBwd(tr, dpt, PPSG_MODE_CFR, -1, h->stream); }, 2);
@@ -2935,9 +2999,10 @@ int ppsg_profile_depths(ppsg_t* h, int iters, double* out, int cap) {
           rc = timed([&] { return h->launchFwd(tr, dpt, 0, -1, h->stream); },
                      (size_t)(4 * dpt + 2 * tr));
       for (int ps = 2; ps < kLists && rc == 0; ++ps)
-        if (!h->lists[tr][ps].empty() && h->lists[tr][ps][0].count)
-          rc = timed([&] { return h->launchTerm(tr, ps, h->stream); },
-                     (size_t)(4 * D + (ps == 2 ? 0 : 2) + tr));
+        for (int part = 0; part < (int)h->lists[tr][ps].size() && rc == 0; ++part)
+          if (h->lists[tr][ps][part].count)
+            rc = timed([&] { return h->launchTerm(tr, ps, part, h->stream); },
+                       (size_t)(4 * D + (ps == 2 ? 0 : 2) + tr));
       for (int dpt = D - 1; dpt >= 0 && rc == 0; --dpt)
         if (h->lists[tr][1][dpt].count)
           rc = timed([&] { return h->launchBwd(tr, dpt, PPSG_MODE_CFR, -1, h->stream); },
@@ -3081,29 +3146,46 @@ int ppsg_kind_bytes(ppsg_t* h, double out[6]) {
 
 int ppsg_tune(ppsg_t* h, int param, int value) {
   if (!h) return fail("null handle");
-  if (param != PPSG_TUNE_GROUP) return fail("tune: unknown parameter");
-  if (value < 1 || value > kMaxGroup) return fail("tune: group size must be 1..64");
-  if (value == h->groupSize) return 0;
-  h->groupSize = value;
-  return h->rebuild();
+  if (param == PPSG_TUNE_GROUP) {
+    if (value < 1 || value > kMaxGroup) return fail("tune: group size must be 1..64");
+    if (value == h->groupSize) return 0;
+    h->groupSize = value;
+    return h->rebuild();
+  }
+  if (param == PPSG_TUNE_TERM_SPLIT) {
+    if (value < 0 || value > 32767) return fail("tune: terminal split depth must be 0..32767");
+    if (value == h->termSplit) return 0;
+    h->termSplit = value;
+    return h->rebuild();
+  }
+  return fail("tune: unknown parameter");
 }
 
-int ppsg_autotune(ppsg_t* h, const int* candidates, int n, int reps, int* chosen, double* ms) {
+int ppsg_autotune_param(ppsg_t* h, int param, const int* candidates, int n, int reps, int* chosen,
+                        double* ms) {
   if (!h || (n > 0 && !candidates)) return fail("null argument");
+  if (param != PPSG_TUNE_GROUP && param != PPSG_TUNE_TERM_SPLIT)
+    return fail("autotune: unknown parameter");
   if (n <= 0 || reps <= 0) return fail("autotune: nothing to try");
   if (h->prepare() != 0) return -1;
   if (h->live.empty()) return fail("no live spot");
-  int best = h->groupSize;
+  int best = param == PPSG_TUNE_GROUP ? h->groupSize : h->termSplit;
   double bestMs = 1e300;
   for (int i = 0; i < n; ++i) {
-    if (ppsg_tune(h, PPSG_TUNE_GROUP, candidates[i]) != 0) return -1;
+    if (ppsg_tune(h, param, candidates[i]) != 0) return -1;
     if (h->prepare() != 0) return -1;
-    // EV walks read the strategy sums only: timing them changes no state
-    if (h->launchHalf(0, PPSG_MODE_EV, h->stream) != 0) return -1;  // warm
+    // the group size: EV walks (they read the strategy sums only); the
+    // split: CFR half-steps with dry backward updates (the update's loads
+    // and stores, the old values stored back), the work it overlaps. Neither
+    // changes a row the solve keeps (reach and cfv rows are rewritten by
+    // every half-step before they are read).
+    const int mode = param == PPSG_TUNE_GROUP ? PPSG_MODE_EV : PPSG_MODE_CFR;
+    const int dry = param == PPSG_TUNE_GROUP ? 0 : 1;
+    if (h->launchHalf(0, mode, h->stream, dry) != 0) return -1;  // warm
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
     for (int r = 0; r < reps; ++r)
       for (int tr = 0; tr < 2; ++tr)
-        if (h->launchHalf(tr, PPSG_MODE_EV, h->stream) != 0) return -1;
+        if (h->launchHalf(tr, mode, h->stream, dry) != 0) return -1;
     HIP_TRY(hipEventRecord(h->ev1, h->stream));
     HIP_TRY(hipEventSynchronize(h->ev1));
     float t = 0.0f;
@@ -3114,9 +3196,13 @@ int ppsg_autotune(ppsg_t* h, const int* candidates, int n, int reps, int* chosen
       best = candidates[i];
     }
   }
-  if (ppsg_tune(h, PPSG_TUNE_GROUP, best) != 0) return -1;
+  if (ppsg_tune(h, param, best) != 0) return -1;
   if (chosen) *chosen = best;
   return 0;
+}
+
+int ppsg_autotune(ppsg_t* h, const int* candidates, int n, int reps, int* chosen, double* ms) {
+  return ppsg_autotune_param(h, PPSG_TUNE_GROUP, candidates, n, reps, chosen, ms);
 }
 
 int ppsg_autotune_shapes(ppsg_t* h, int reps, int* table, int cap) {

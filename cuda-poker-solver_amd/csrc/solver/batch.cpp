@@ -124,6 +124,7 @@ std::vector<SpotResult> serve(const std::vector<SpotRequest>& reqs, const ServeO
   }
   check(ppsg_tune(h, PPSG_TUNE_GROUP, group), "tune");
   rep.groupSize = group;
+  check(ppsg_tune(h, PPSG_TUNE_TERM_SPLIT, opt.termSplit), "terminal split");
 
   // ---- compile workers ----
   std::vector<Compiled> compiled(n);
@@ -411,6 +412,10 @@ BatchProfile profileBatch(const std::vector<SpotRequest>& reqs, const std::strin
   P.shapes.assign(1 + 6 * 1024, 0);
   check(ppsg_autotune_shapes(E.h, 3, P.shapes.data(), (int)P.shapes.size()), "profile: shapes");
   P.shapes.resize(1 + 6 * (size_t)P.shapes[0]);
+  P.splitMs.assign(kNumSplitCandidates, 0.0);
+  check(ppsg_autotune_param(E.h, PPSG_TUNE_TERM_SPLIT, kSplitCandidates, kNumSplitCandidates, 2,
+                            &P.termSplit, P.splitMs.data()),
+        "profile: terminal split");
   if (warmup > 0) check(ppsg_batch_step(E.h, warmup), "profile: warmup");
   check(ppsg_batch_step(E.h, iters), "profile: replay");
   double ms = 0.0;

@@ -488,6 +488,7 @@ void pps_serve_defaults(pps_serve_options* opt) {
   opt->flags = 0;
   opt->shapes = nullptr;
   opt->n_shapes = 0;
+  opt->term_split = 0;
 }
 
 int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const double* arrival_ms,
@@ -510,6 +511,7 @@ int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const do
       o.realTime = opt->real_time != 0;
       o.prefill = opt->prefill != 0;
       if (opt->shapes && opt->n_shapes > 0) o.shapes.assign(opt->shapes, opt->shapes + opt->n_shapes);
+      o.termSplit = opt->term_split;
     }
     if (algo) o.algo = algo;
     std::vector<pps::gpu::SpotRequest> reqs(n);
@@ -564,7 +566,7 @@ int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const do
 }
 
 int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, const char* algo,
-                      int iters, int warmup, int64_t arena_bytes, int detail, double out[33],
+                      int iters, int warmup, int64_t arena_bytes, int detail, double out[42],
                       int32_t* shapes, int32_t shapes_cap) {
   return guard([&] {
     if (n <= 0 || !spots || !cfgs || !out) throw std::runtime_error("null argument");
@@ -588,6 +590,8 @@ int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, 
     for (int k = 0; k < 6; ++k) out[25 + k] = P.kindBytes[k];
     out[31] = P.foldLaunches;
     out[32] = P.showdownLaunches;
+    out[33] = P.termSplit;
+    for (int i = 0; i < pps::gpu::kNumSplitCandidates; ++i) out[34 + i] = P.splitMs[i];
     if (shapes) {
       if ((size_t)shapes_cap < P.shapes.size())
         throw std::runtime_error("profile: the shape table needs " + std::to_string(P.shapes.size()) +

@@ -32,7 +32,8 @@ class ServeOptions(ctypes.Structure):
                 ("max_spots", ctypes.c_int32), ("max_live", ctypes.c_int32),
                 ("compile_threads", ctypes.c_int32), ("real_time", ctypes.c_int32),
                 ("prefill", ctypes.c_int32), ("flags", ctypes.c_int32),
-                ("shapes", ctypes.POINTER(ctypes.c_int32)), ("n_shapes", ctypes.c_int32)]
+                ("shapes", ctypes.POINTER(ctypes.c_int32)), ("n_shapes", ctypes.c_int32),
+                ("term_split", ctypes.c_int32)]
 
 
 class ServeResult(ctypes.Structure):
@@ -86,7 +87,7 @@ PPSG_SYMBOLS = [
     "ppsg_batch_read", "ppsg_batch_walk_range", "ppsg_batch_root_sums",
     "ppsg_batch_root_sums_noalloc",
     "ppsg_batch_write", "ppsg_batch_free_bytes", "ppsg_batch_compact",
-    "ppsg_device_memory", "ppsg_tune", "ppsg_autotune", "ppsg_autotune_shapes",
+    "ppsg_device_memory", "ppsg_tune", "ppsg_autotune", "ppsg_autotune_param", "ppsg_autotune_shapes",
     "ppsg_set_shapes", "ppsg_kind_bytes",
 ]
 
@@ -250,6 +251,7 @@ def copy_probe(nbytes, reps):
 
 
 GROUP_CANDIDATES = [1, 2, 4, 8, 16, 32]  # terminal group sizes the autotuner tries
+SPLIT_CANDIDATES = [0, 3, 4, 5, 6, 7, 8, 9]  # terminal split depths the autotuner tries
 
 
 class CapiError(RuntimeError):
@@ -553,7 +555,7 @@ def batch_profile(spots, cfgs, iters, algo="dcfr", arena_bytes=0, detail=False, 
     n = len(spots)
     S = (Spot * n)(*spots)
     C = (BetConfig * n)(*cfgs)
-    out = (ctypes.c_double * 33)()
+    out = (ctypes.c_double * 42)()
     cap = 1 + 6 * 1024
     shapes = (ctypes.c_int32 * cap)()
     _check(lib().pps_batch_profile(n, S, C, algo.encode(), int(iters), int(warmup),
@@ -571,6 +573,8 @@ def batch_profile(spots, cfgs, iters, algo="dcfr", arena_bytes=0, detail=False, 
     d["fold_launches"], d["showdown_launches"] = out[31], out[32]
     d["term_launches"] = out[31] + out[32]
     d["shapes"] = list(shapes[:1 + 6 * shapes[0]])
+    d["term_split"] = int(out[33])
+    d["term_split_ms"] = {k: round(out[34 + i], 4) for i, k in enumerate(SPLIT_CANDIDATES)}
     if not detail:
         for k in ("decide_ms", "chance_ms", "fwd_decide_ms", "fwd_chance_ms"):
             del d[k]

@@ -55,6 +55,8 @@ struct ServeOptions {
   // per-depth launch shapes for the engine (a ppsg_autotune_shapes table);
   // empty: the defaults
   std::vector<int> shapes;
+  // terminal split depth of the engine (PPSG_TUNE_TERM_SPLIT; 0: one stream)
+  int termSplit = 0;
 };
 
 struct ServeReport {
@@ -106,11 +108,18 @@ struct BatchProfile {
   std::vector<double> tuneMs;
   // per-depth launch shapes the autotuner chose (ppsg_autotune_shapes table)
   std::vector<int> shapes;
+  // terminal split depth the autotuner chose, and the time (ms per dry
+  // half-step pair) of each candidate in kSplitCandidates order
+  int termSplit = 0;
+  std::vector<double> splitMs;
 };
 
 // Terminal group sizes the autotuner tries.
 constexpr int kGroupCandidates[] = {1, 2, 4, 8, 16, 32};
 constexpr int kNumGroupCandidates = 6;
+// Terminal split depths the autotuner tries (0: one stream).
+constexpr int kSplitCandidates[] = {0, 3, 4, 5, 6, 7, 8, 9};
+constexpr int kNumSplitCandidates = 8;
 BatchProfile profileBatch(const std::vector<SpotRequest>& requests, const std::string& algo,
                           int iters, int64_t arenaBytes = 0, bool detail = false,
                           int warmup = 0);

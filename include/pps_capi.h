@@ -150,6 +150,8 @@ typedef struct pps_serve_options {
   const int32_t* shapes;    /* per-depth launch shapes (a ppsg_autotune_shapes
                                table, e.g. from pps_batch_profile) or NULL */
   int32_t n_shapes;         /* entries in shapes */
+  int32_t term_split;       /* terminal split depth (PPSG_TUNE_TERM_SPLIT, e.g.
+                               from pps_batch_profile); 0: one stream */
 } pps_serve_options;
 
 typedef struct pps_serve_result {
@@ -194,13 +196,16 @@ int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const do
    autotuner chose and its EV-walk ms for group sizes 1, 2, 4, 8, 16, 32,
    (detail) forward decide and forward chance ms per iteration run alone,
    the algorithmic bytes per batched iteration of forward decide, forward
-   chance, backward decide, backward chance, fold, showdown, and the fold
-   and showdown launch counts over the timed iterations]. The per-depth
-   launch shapes are autotuned first (ppsg_autotune_shapes); their table goes
-   to `shapes` (if not NULL and it holds shapes_cap >= 1 + 6 * depths
-   entries) for pps_serve_options.shapes. */
+   chance, backward decide, backward chance, fold, showdown, the fold
+   and showdown launch counts over the timed iterations, the terminal split
+   depth the autotuner chose and its ms per dry half-step pair for splits
+   0, 3, 4, 5, 6, 7, 8, 9]. The per-depth launch shapes are autotuned first
+   (ppsg_autotune_shapes); their table goes to `shapes` (if not NULL and it
+   holds shapes_cap >= 1 + 6 * depths entries) for
+   pps_serve_options.shapes; the split (after the shapes) is
+   pps_serve_options.term_split. */
 int pps_batch_profile(int n, const pps_spot* spots, const pps_bet_config* cfgs, const char* algo,
-                      int iters, int warmup, int64_t arena_bytes, int detail, double out[33],
+                      int iters, int warmup, int64_t arena_bytes, int detail, double out[42],
                       int32_t* shapes, int32_t shapes_cap);
 /* Device bytes the compiled spot takes in an arena (host only). */
 int pps_compile_footprint(const pps_spot* spot, const pps_bet_config* cfg, int64_t* bytes);

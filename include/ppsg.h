@@ -28,7 +28,7 @@ extern "C" {
 enum { PPSG_DECIDE = 0, PPSG_SHOWDOWN = 1, PPSG_FOLD = 2, PPSG_CHANCE = 3 };
 enum { PPSG_MODE_CFR = 0, PPSG_MODE_EV = 1, PPSG_MODE_BR = 2 };
 enum { PPSG_REGRET = 0, PPSG_STRAT = 1, PPSG_REACH0 = 2, PPSG_REACH1 = 3, PPSG_CFV = 4 };
-enum { PPSG_TUNE_GROUP = 0 };
+enum { PPSG_TUNE_GROUP = 0, PPSG_TUNE_TERM_SPLIT = 1 };
 
 /* Row offsets (reg_off, cfv_off, reach_off, in floats) are multiples of
    64: every f32 row starts on a 256-byte line and is padded to
@@ -258,12 +258,31 @@ int ppsg_kind_bytes(ppsg_t* h, double out[6]);
    from the spot's mean terminal nodes per board class: below 24, up to
    400, above). Larger groups load a class's gather list and per-combo
    metadata once for more nodes and launch fewer workgroups; smaller groups
-   spread a small batch over more CUs. */
+   spread a small batch over more CUs.
+   PPSG_TUNE_TERM_SPLIT: depth k (0..32767; default 0). The fold and
+   showdown nodes at depth <= k get work lists of their own, launched on a
+   second stream as soon as the forward launch of depth k - 1 has produced
+   their reach rows, beside the deeper forward, terminal and backward
+   launches (which do not read them), and joined before the backward launch
+   of depth k - 1. The terminal kernels are bound by LDS and VALU work, the
+   update kernels by HBM, so a large batch gains from running them side by
+   side; 0 keeps every launch on one stream (one spot: launch latency
+   dominates). */
 int ppsg_tune(ppsg_t* h, int param, int value);
 /* Times n candidate group sizes on the live spots (reps EV walks of both
    traversers each, which change no rows), keeps the fastest; *chosen and
-   ms[i] (per rep) may be NULL. */
+   ms[i] (per rep) may be NULL. The same as ppsg_autotune_param with
+   PPSG_TUNE_GROUP. */
 int ppsg_autotune(ppsg_t* h, const int* candidates, int n, int reps, int* chosen, double* ms);
+/* Times n candidate values of a ppsg_tune parameter on the live spots and
+   keeps the fastest: group sizes over EV walks, terminal splits over CFR
+   half-steps whose backward launches store the old regret and strategy
+   values (reach and cfv rows are rewritten by every half-step before they
+   are read, so no solve state changes). reps half-steps of both traversers
+   per candidate; *chosen and ms[i] (per rep) may be NULL. No reference
+   counterpart. */
+int ppsg_autotune_param(ppsg_t* h, int param, const int* candidates, int n, int reps, int* chosen,
+                        double* ms);
 /* Per-depth launch shapes (combos per lane of forward decide items from
    {1, 2, 4}, forward chance items from {2, 4, 8}, backward decide items from
    {1, 2}), picked for the live spots by timing each depth's launch of each

@@ -45,12 +45,15 @@ def test_solve_tables_are_pinned(name, iters, digest, ev_oop, expl):
     assert st["ev_oop"] == ev_oop and st["exploitability"] == expl
 
 
-def test_served_batch_is_pinned():
+@pytest.mark.parametrize("term_split", [0, 5])
+def test_served_batch_is_pinned(term_split):
+    """The same served batch with every launch on one stream and with the
+    terminals at depth <= 5 on a second stream beside the deeper launches."""
     spots = configs.batch_stream(4)
     pairs = [spot_cfg(s) for s in spots]
     res, _ = capi.serve([p[0] for p in pairs], [p[1] for p in pairs], "dcfr", None,
                         target_frac=0.005, check_every=32, max_iters=4096, real_time=0,
-                        prefill=1)
+                        prefill=1, term_split=term_split)
     got = [(r["iterations"], r["expl"], r["ev0"]) for r in res]
     assert got == [(128, 0.5087439806226541, 71.6570791858915),
                    (64, 0.28165960617813823, 54.26376348232555),

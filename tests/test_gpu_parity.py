@@ -297,6 +297,49 @@ def test_node_ev_matches_oracle_node_values(name, tmp_path):
             assert np.allclose(ev["sides"][side]["mass"], mass, rtol=1e-5, atol=1e-6)
 
 
+def test_terminal_split_changes_no_rows_and_no_results():
+    """PPSG_TUNE_TERM_SPLIT k puts the fold and showdown nodes at depth <= k
+    on a second stream (forked after the forward launch of depth k - 1,
+    joined before the backward launch of depth k - 1), and
+    ppsg_autotune_param times the candidates over CFR half-steps with dry
+    backward updates: the rows stay bitwise the same, and solves at every
+    split (none, shallow, mid-tree, all terminals) equal the one-stream
+    solve."""
+    import ctypes
+    L = capi.lib()
+    L.pps_solver_engine.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                    ctypes.POINTER(ctypes.c_int)]
+    L.ppsg_tune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    L.ppsg_autotune_param.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                      ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                      ctypes.POINTER(ctypes.c_double)]
+    TERM_SPLIT = 1
+    s = configs.large_turn()
+    a = gpu_solver(s)
+    a.solve(30, "dcfr")
+    r0, r1 = a.rows(0), a.rows(1)
+    eng, slot = ctypes.c_void_p(), ctypes.c_int()
+    assert L.pps_solver_engine(a.h, ctypes.byref(eng), ctypes.byref(slot)) == 0
+    cands = [0, 2, 3, 5, 7, 40]
+    arr = (ctypes.c_int * len(cands))(*cands)
+    ms = (ctypes.c_double * len(cands))()
+    chosen = ctypes.c_int(-1)
+    assert L.ppsg_autotune_param(eng, TERM_SPLIT, arr, len(cands), 2, ctypes.byref(chosen), ms) == 0
+    assert chosen.value in cands and all(x > 0.0 for x in ms)
+    assert np.array_equal(a.rows(0), r0) and np.array_equal(a.rows(1), r1)
+    a.continue_solve(20)
+    b = gpu_solver(s)
+    b.solve(50, "dcfr")
+    assert np.array_equal(a.rows(0), b.rows(0)) and np.array_equal(a.rows(1), b.rows(1))
+    for k in (1, 3, 6, 100):
+        assert L.ppsg_tune(eng, TERM_SPLIT, k) == 0
+        a.solve(50, "dcfr")
+        assert np.array_equal(a.rows(0), b.rows(0)), k
+        assert np.array_equal(a.rows(1), b.rows(1)), k
+        assert a.stats() == b.stats(), k
+    assert L.ppsg_tune(eng, TERM_SPLIT, -1) != 0
+
+
 def test_shape_autotuner_changes_no_rows_and_no_results():
     """ppsg_autotune_shapes times every depth's launches at each candidate
     shape (forward launches rewrite reach only, backward ones run a dry
