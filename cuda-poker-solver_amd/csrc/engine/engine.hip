@@ -212,18 +212,24 @@ __device__ __forceinline__ float sigmaOf(float x, float acc, float uniform, bool
 // Forward reach.
 // ---------------------------------------------------------------------------
 
+// The end of the whole 128-byte lines (32 floats) that hold a row's first n
+// values. Rows are padded to PPSG_ROW_STRIDE(n) floats (whole 64-lane
+// tiles); a tile's lanes up to lineEnd(n) load (indices clamped into the
+// row's last needed line, so no load sits behind a per-lane branch) and
+// store, the lanes past it store nothing. Every store then covers whole
+// cache lines: a store that covers part of a line costs far more than its
+// share (a launch whose nodes ended in part-filled lines ran ~15 % slower),
+// while the lanes past the last line only moved padding (~1 % of a
+// backward launch's time).
+__host__ __device__ constexpr int lineEnd(int n) { return (n + 31) & ~31; }
+
 // Decide (opponent acts): child reach a = reach * sigma_a. NA > 0: action
 // count fixed at compile time (the row values stay in registers); every
 // lane handles up to R combos kWave apart, all loads issued first.
-// Tiles are whole: a row of n combos is padded to PPSG_ROW_STRIDE(n), a
-// multiple of kWave, and the lanes past n load and store its padding (no
-// one reads it back), so every store covers whole cache lines. A store
-// that covers part of a line costs far more than its share: a launch
-// whose nodes end in part-filled tiles runs ~15 % slower.
 template <int NA, int R>
 __device__ __forceinline__ void fwdDecide(const Dev& d, const Item& it, int fromStrat) {
   const int c0 = it.tile * kWave * R + waveLane();
-  const int n = (int)PPSG_ROW_STRIDE(it.n), na = NA > 0 ? NA : it.na;
+  const int n = lineEnd(it.n), na = NA > 0 ? NA : it.na;
   if (c0 >= n) return;
   const int64_t ld = it.ld, cs = it.m;
   const float* __restrict__ reach = at<float>(d, it.b);
@@ -235,10 +241,10 @@ __device__ __forceinline__ void fwdDecide(const Dev& d, const Item& it, int from
     float x[R][NA > 0 ? NA : 1], r[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-      const int c = c0 + k * kWave;
-      r[k] = c < n ? reach[c] : 0.0f;
+      const int c = min(c0 + k * kWave, n - 1);
+      r[k] = reach[c];
 #pragma unroll
-      for (int a = 0; a < NA; ++a) x[k][a] = c < n ? src[a * ld + c] : 0.0f;
+      for (int a = 0; a < NA; ++a) x[k][a] = src[a * ld + c];
     }
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -272,10 +278,10 @@ __device__ __forceinline__ void fwdItem(const Dev& d, const Item& it, int fromSt
     const float* __restrict__ in = at<float>(d, it.b);
     const uint16_t* __restrict__ tp = at<uint16_t>(d, it.a);
     float* __restrict__ out = at<float>(d, it.c);
-    // whole tiles: zeros into the row's padding; the loads are issued for
-    // every lane (index clamped into the list) and the lanes past n drop
-    // the value, so no load sits behind a per-lane branch
-    const int n = it.n, nw = (int)PPSG_ROW_STRIDE(n);
+    // whole lines: zeros into the row's last line past n; the loads are
+    // issued for every lane (index clamped into the list) and the lanes
+    // past n drop the value, so no load sits behind a per-lane branch
+    const int n = it.n, nw = lineEnd(n);
     float x[RC];
 #pragma unroll
     for (int k = 0; k < RC; ++k) x[k] = in[tp[min(c0 + k * kWave, n - 1)]];
@@ -357,17 +363,16 @@ __device__ __forceinline__ const TermNode* stageNodes(TermNode* grp, const TermN
 // A node's results, stored once the next node's reach is staged: its
 // stores are then older than the next reach request, so the wait for that
 // reach (which waits for all vector memory operations) never waits for
-// them. The row is padded to PPSG_ROW_STRIDE(nT) floats, a multiple of 64,
-// so waveBase + q * NT < padT holds for a whole wave or for none of it:
-// waves past the padded row skip with a scalar test, and the lanes in the
-// padding store values that nothing reads, so every store covers whole
-// cache lines and none needs an exec mask of its own.
+// them. Stores run to padT = lineEnd(nT): waves past it skip with a scalar
+// test, and the lanes between nT and padT store values that nothing reads,
+// so every store covers whole cache lines.
 template <int ROWS, int NT>
 __device__ __forceinline__ void storePadded(float* out, const float (&res)[ROWS], int waveBase,
                                             int padT) {
 #pragma unroll
   for (int q = 0; q < ROWS; ++q)
-    if (waveBase + q * NT < padT) out[(int)threadIdx.x + q * NT] = res[q];
+    if (waveBase + q * NT < padT && (int)threadIdx.x + q * NT < padT)
+      out[(int)threadIdx.x + q * NT] = res[q];
 }
 
 // Fold group: mass(i) = total - cardSum[c1] - cardSum[c2] + r[same(i)].
@@ -383,7 +388,7 @@ __device__ void foldGroup(const Dev& d, const Item& it, char* lds) {
   const int tid = threadIdx.x, lane = tid & 63;
   const int waveBase = __builtin_amdgcn_readfirstlane(tid >> 6) * 64;  // uniform
   const int nO = it.m, nT = it.n, cnt = it.cnt;
-  const int padT = (int)PPSG_ROW_STRIDE(nT);
+  const int padT = lineEnd(nT);
   float* raw = reinterpret_cast<float*>(lds);
   const TermNode* nodes = stageNodes(grp, at<TermNode>(d, it.a), cnt);
   const char* base = d.arena + it.b;
@@ -539,7 +544,7 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar tests
   const int nO = it.m, nT = it.n, cnt = it.cnt;
-  const int waveBase = wave * 64, padT = (int)PPSG_ROW_STRIDE(nT);
+  const int waveBase = wave * 64, padT = lineEnd(nT);
   float* raw = reinterpret_cast<float*>(lds);
   double* Xs = reinterpret_cast<double*>(lds + sdRawBytes(nO));
   float* Y = reinterpret_cast<float*>(lds + sdRawBytes(nO) + (size_t)NT * 4 * sizeof(double));
@@ -754,15 +759,17 @@ __global__ __launch_bounds__(kBlock) void fwdKernel(Dev d, const Item* __restric
 // dry (CFR mode, timing only): the update is computed and stored, but the
 // stored values are the old ones, so the tile moves exactly the bytes of a
 // real update and changes no row.
-// Whole tiles, as in the forward pass: the lanes past the n traverser
-// combos work on the rows' padding.
+// Whole lines, as in the forward pass (lineEnd): the lanes past the n
+// traverser combos in the row's last line work on its padding.
 template <int MODE, int NA, int R>
 __device__ __forceinline__ void decideTile(const Dev& d, const Item& it, int dry) {
   const int c0 = it.tile * kWave * R + waveLane();
-  const int n = (int)PPSG_ROW_STRIDE(it.n);
+  const int n = lineEnd(it.n);
   if (c0 >= n) return;
   const int na = NA > 0 ? NA : it.na;
   const int64_t stride = it.m, ld = it.ld;
+  // loads of every lane, index clamped into the row's last line
+  auto cl = [n](int c) { return min(c, n - 1); };
   const float* __restrict__ kid = at<float>(d, it.c);
   float* __restrict__ out = at<float>(d, it.b);
   if (!it.e || MODE == PPSG_MODE_BR) {
@@ -780,7 +787,7 @@ __device__ __forceinline__ void decideTile(const Dev& d, const Item& it, int dry
       for (int k = 0; k < R; ++k)
 #pragma unroll
         for (int a = 0; a < A; ++a)
-          x[k][a] = c0 + k * kWave < n ? kid[a * stride + c0 + k * kWave] : 0.0f;
+          x[k][a] = kid[a * stride + cl(c0 + k * kWave)];
 #pragma unroll
       for (int k = 0; k < R; ++k)
 #pragma unroll
@@ -813,9 +820,9 @@ __device__ __forceinline__ void decideTile(const Dev& d, const Item& it, int dry
       const int c = c0 + k * kWave;
 #pragma unroll
       for (int a = 0; a < A; ++a) {
-        x[k][a] = c < n ? src[a * ld + c] : 0.0f;
-        v[k][a] = c < n ? kid[a * stride + c] : 0.0f;
-        sm[k][a] = plus && c < n ? st[a * ld + c] : 0.0f;
+        x[k][a] = src[a * ld + cl(c)];
+        v[k][a] = kid[a * stride + cl(c)];
+        sm[k][a] = plus ? st[a * ld + cl(c)] : 0.0f;
       }
     }
     float pos = 0.0f, neg = 0.0f, avg = 0.0f;
@@ -895,7 +902,7 @@ __device__ __forceinline__ void decideTile(const Dev& d, const Item& it, int dry
 __device__ __forceinline__ void chanceTile(const Dev& d, const Item& it) {
   const int i = it.tile * kWave + waveLane();
   const int n = it.n;
-  if (i >= (int)PPSG_ROW_STRIDE(n)) return;  // whole tiles: lanes past n store 0
+  if (i >= lineEnd(n)) return;  // whole lines: lanes past n store 0
   const bool in = i < n;
   const uint32_t ic = (uint32_t)min(i, n - 1);  // lanes past n read slot n-1 (then drop it)
   // 32-bit unsigned byte offsets from uniform bases (a branch block is at
@@ -2497,8 +2504,8 @@ int walkRange(ppsg_t* h, int slot, int tr, int mode, int64_t lo, int64_t hi, con
     it.f = slot;
     return it;
   };
-  // items and terminal entries are in node order (per depth, per group), so
-  // a node range is one run of each, found by bisection
+  // items are in node order per depth, terminal entries per group and
+  // depth, so a node range is one run of each, found by bisection
   auto run = [](const std::vector<int32_t>& ids, int64_t a, int64_t b) {
     const auto x = std::lower_bound(ids.begin(), ids.end(), (int32_t)a);
     const auto y = std::lower_bound(x, ids.end(), (int32_t)b);
@@ -2525,24 +2532,30 @@ int walkRange(ppsg_t* h, int slot, int tr, int mode, int64_t lo, int64_t hi, con
     for (size_t q = r.first; q < r.second; ++q) all.push_back(absolute(v[q]));
     if ((int64_t)all.size() > first) segs.push_back({first, (int64_t)all.size() - first, 0, 0});
   }
+  // a group's entries run by depth, then node: one bisection per depth run
   std::vector<Item> term[3];
   for (const TermGroup& g : S.groups[tr]) {
-    const std::vector<int32_t> ids(S.termNode.begin() + g.first,
-                                   S.termNode.begin() + g.first + g.count);
-    const auto r = run(ids, lo, hi);
-    for (size_t s0 = r.first; s0 < r.second; s0 += (size_t)h->groupSize) {
-      Item it{};
-      it.kind = g.kind;
-      it.a = S.off + S.L.termNodes + (int64_t)sizeof(TermNode) * (g.first + (int64_t)s0);
-      it.b = S.off;
-      it.c = S.off + g.gather;
-      it.d = S.off + g.meta;
-      it.n = g.nT;
-      it.m = g.nO;
-      it.na = g.rows;
-      it.cnt = (int32_t)std::min<size_t>((size_t)h->groupSize, r.second - s0);
-      it.f = slot;
-      term[g.kind == kBwdFold ? 0 : g.wide ? 2 : 1].push_back(it);
+    for (int64_t a = 0; a < g.count;) {
+      int64_t e = a;
+      while (e < g.count && S.termDepth[g.first + e] == S.termDepth[g.first + a]) ++e;
+      const std::vector<int32_t> ids(S.termNode.begin() + g.first + a,
+                                     S.termNode.begin() + g.first + e);
+      const auto r = run(ids, lo, hi);
+      for (size_t s0 = r.first; s0 < r.second; s0 += (size_t)h->groupSize) {
+        Item it{};
+        it.kind = g.kind;
+        it.a = S.off + S.L.termNodes + (int64_t)sizeof(TermNode) * (g.first + a + (int64_t)s0);
+        it.b = S.off;
+        it.c = S.off + g.gather;
+        it.d = S.off + g.meta;
+        it.n = g.nT;
+        it.m = g.nO;
+        it.na = g.rows;
+        it.cnt = (int32_t)std::min<size_t>((size_t)h->groupSize, r.second - s0);
+        it.f = slot;
+        term[g.kind == kBwdFold ? 0 : g.wide ? 2 : 1].push_back(it);
+      }
+      a = e;
     }
   }
   for (int k = 0; k < 3; ++k) {
