@@ -1245,6 +1245,7 @@ struct Spot {
   std::vector<int16_t> termDepth;
   int64_t nodes = 0;
   std::vector<TermGroup> groups[2];  // per traverser
+  int group = 4;  // terminal nodes per work item, set when the spot is added
   int64_t itemCount = 0;  // listed items (terminal items at the group size in use)
   double fwdBytes = 0.0, bwdBytes = 0.0;
   double kindBytes[6] = {0, 0, 0, 0, 0, 0};  // per ItemKind, one iteration
@@ -1681,7 +1682,7 @@ struct ppsg_engine {
   // groups, 4 wide showdown groups (two lists each: [0] the entries deeper
   // than termSplit, [1] those at depth <= termSplit)]
   std::vector<DevList> lists[2][kLists];
-  int groupSize = 4;  // terminal nodes per work item
+  int groupSize = 4;  // terminal nodes per work item of spots added next
   // Terminals at depth <= termSplit run on the second stream as soon as
   // the forward pass has produced their reach rows, beside the deeper
   // forward, terminal and backward launches (0: one stream)
@@ -1884,7 +1885,7 @@ std::vector<Item> ppsg_engine::spotItems(int slot, int tr, int ps, int dep) cons
     if (g.kind != kind || (ps == 4) != g.wide) continue;
     const int cut = shallowCount(S, g);
     const int lo = dep == 0 ? cut : 0, hi = dep == 0 ? g.count : cut;
-    for (int s0 = lo; s0 < hi; s0 += groupSize) {
+    for (int s0 = lo; s0 < hi; s0 += S.group) {
       Item it{};
       it.kind = g.kind;
       it.a = S.off + S.L.termNodes + (int64_t)sizeof(TermNode) * (g.first + s0);
@@ -1894,7 +1895,7 @@ std::vector<Item> ppsg_engine::spotItems(int slot, int tr, int ps, int dep) cons
       it.n = g.nT;
       it.m = g.nO;
       it.na = g.rows;
-      it.cnt = std::min(groupSize, hi - s0);
+      it.cnt = std::min(S.group, hi - s0);
       it.f = slot;
       v.push_back(it);
     }
@@ -1916,7 +1917,7 @@ int64_t ppsg_engine::countItems(int slot) const {
           n += it.tile % shapeOf(tr, (int)dep, it.kind) == 0;
     for (const TermGroup& g : S.groups[tr]) {
       const int cut = shallowCount(S, g);
-      n += (cut + groupSize - 1) / groupSize + (g.count - cut + groupSize - 1) / groupSize;
+      n += (cut + S.group - 1) / S.group + (g.count - cut + S.group - 1) / S.group;
     }
   }
   return n;
@@ -2150,6 +2151,7 @@ int ppsg_engine::add(const ppsg_tree_desc* t, int* slot) {
     return -2;
   }
   S.off = off;
+  S.group = groupSize;
   HIP_TRY(hipStreamSynchronize(stream));
   char* base = arena.base + off;
   auto put = [&](int64_t rel, const void* src, int64_t bytes) -> int {
@@ -2542,7 +2544,7 @@ int walkRange(ppsg_t* h, int slot, int tr, int mode, int64_t lo, int64_t hi, con
       const std::vector<int32_t> ids(S.termNode.begin() + g.first + a,
                                      S.termNode.begin() + g.first + e);
       const auto r = run(ids, lo, hi);
-      for (size_t s0 = r.first; s0 < r.second; s0 += (size_t)h->groupSize) {
+      for (size_t s0 = r.first; s0 < r.second; s0 += (size_t)S.group) {
         Item it{};
         it.kind = g.kind;
         it.a = S.off + S.L.termNodes + (int64_t)sizeof(TermNode) * (g.first + a + (int64_t)s0);
@@ -2552,7 +2554,7 @@ int walkRange(ppsg_t* h, int slot, int tr, int mode, int64_t lo, int64_t hi, con
         it.n = g.nT;
         it.m = g.nO;
         it.na = g.rows;
-        it.cnt = (int32_t)std::min<size_t>((size_t)h->groupSize, r.second - s0);
+        it.cnt = (int32_t)std::min<size_t>((size_t)S.group, r.second - s0);
         it.f = slot;
         term[g.kind == kBwdFold ? 0 : g.wide ? 2 : 1].push_back(it);
       }
@@ -3160,11 +3162,16 @@ int ppsg_kind_bytes(ppsg_t* h, double out[6]) {
 
 int ppsg_tune(ppsg_t* h, int param, int value) {
   if (!h) return fail("null handle");
-  if (param == PPSG_TUNE_GROUP) {
+  if (param == PPSG_TUNE_GROUP || param == PPSG_TUNE_ADD_GROUP) {
     if (value < 1 || value > kMaxGroup) return fail("tune: group size must be 1..64");
-    if (value == h->groupSize) return 0;
     h->groupSize = value;
-    return h->rebuild();
+    if (param == PPSG_TUNE_ADD_GROUP) return 0;  // spots added from now on
+    bool same = true;
+    for (Spot& S : h->spots) {
+      same &= S.state == kFree || S.group == value;
+      S.group = value;
+    }
+    return same ? 0 : h->rebuild();
   }
   if (param == PPSG_TUNE_TERM_SPLIT) {
     if (value < 0 || value > 32767) return fail("tune: terminal split depth must be 0..32767");

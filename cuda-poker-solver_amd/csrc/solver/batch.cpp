@@ -124,6 +124,7 @@ std::vector<SpotResult> serve(const std::vector<SpotRequest>& reqs, const ServeO
   }
   check(ppsg_tune(h, PPSG_TUNE_GROUP, group), "tune");
   rep.groupSize = group;
+  int addGroup = group;  // group size of the spots admitted next
   check(ppsg_tune(h, PPSG_TUNE_TERM_SPLIT, opt.termSplit), "terminal split");
 
   // ---- compile workers ----
@@ -203,6 +204,17 @@ std::vector<SpotResult> serve(const std::vector<SpotRequest>& reqs, const ServeO
       x.tree.reset();
       ++finished;
       return true;
+    }
+    if (!opt.prefill) {
+      // a stream gives each spot the group size its live count calls for
+      // when it is admitted (the same rule as a batch admitted at once);
+      // the listed spots keep theirs, so nothing is re-cut
+      int g = 4;
+      while (g < 32 && g * 8 < (int)live.size() + 1) g *= 2;
+      if (g != addGroup) {
+        check(ppsg_tune(h, PPSG_TUNE_ADD_GROUP, g), "tune");
+        addGroup = g;
+      }
     }
     const ppsg_tree_desc d = x.tree->desc();
     int slot = -1;

@@ -28,7 +28,7 @@ extern "C" {
 enum { PPSG_DECIDE = 0, PPSG_SHOWDOWN = 1, PPSG_FOLD = 2, PPSG_CHANCE = 3 };
 enum { PPSG_MODE_CFR = 0, PPSG_MODE_EV = 1, PPSG_MODE_BR = 2 };
 enum { PPSG_REGRET = 0, PPSG_STRAT = 1, PPSG_REACH0 = 2, PPSG_REACH1 = 3, PPSG_CFV = 4 };
-enum { PPSG_TUNE_GROUP = 0, PPSG_TUNE_TERM_SPLIT = 1 };
+enum { PPSG_TUNE_GROUP = 0, PPSG_TUNE_TERM_SPLIT = 1, PPSG_TUNE_ADD_GROUP = 2 };
 
 /* Row offsets (reg_off, cfv_off, reach_off, in floats) are multiples of
    64: every f32 row starts on a 256-byte line and is padded to
@@ -258,7 +258,11 @@ int ppsg_kind_bytes(ppsg_t* h, double out[6]);
    from the spot's mean terminal nodes per board class: below 24, up to
    400, above). Larger groups load a class's gather list and per-combo
    metadata once for more nodes and launch fewer workgroups; smaller groups
-   spread a small batch over more CUs.
+   spread a small batch over more CUs. Sets the size of every spot (their
+   terminal lists are re-cut) and of the spots added later.
+   PPSG_TUNE_ADD_GROUP: the group size (1..64) of the spots added from now
+   on only; the listed spots keep theirs (nothing is re-cut), so a stream
+   can follow its live count without rebuilding the lists.
    PPSG_TUNE_TERM_SPLIT: depth k (0..32767; default 0). The fold and
    showdown nodes at depth <= k get work lists of their own, launched on a
    second stream as soon as the forward launch of depth k - 1 has produced

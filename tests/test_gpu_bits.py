@@ -59,3 +59,20 @@ def test_served_batch_is_pinned(term_split):
                    (64, 0.28165960617813823, 54.26376348232555),
                    (64, 0.5142293226740549, 58.04267755971645),
                    (64, 0.5652470131910121, 56.25281028438216)]
+
+
+def test_stream_group_sizes_leave_results_unchanged():
+    """A stream gives each spot the terminal group size its live count calls
+    for at admission (PPSG_TUNE_ADD_GROUP: 4 up to 31 live spots, 8 from
+    32); the spots admitted with different sizes solve exactly as they do
+    one at a time."""
+    spots = configs.batch_stream(40)
+    pairs = [spot_cfg(s) for s in spots]
+    S, C = [p[0] for p in pairs], [p[1] for p in pairs]
+    opt = dict(target_frac=0.005, check_every=32, max_iters=4096, real_time=0)
+    streamed, rep = capi.serve(S, C, "dcfr", None, prefill=0, **opt)
+    alone, _ = capi.serve(S, C, "dcfr", None, prefill=1, max_live=1, **opt)
+    assert rep["peak_live"] > 32
+    got = [(r["iterations"], r["expl"], r["ev0"], r["ev1"]) for r in streamed]
+    want = [(r["iterations"], r["expl"], r["ev0"], r["ev1"]) for r in alone]
+    assert got == want
