@@ -67,6 +67,7 @@ constexpr int kFoldPart = 12;
 // chunks past its longest part
 constexpr int kPartChunk = 3;
 constexpr int kMaxGroup = 64;  // terminal nodes per work item at most
+constexpr int kGraphIters = 8;  // iterations per replay of a stable solve
 // Combos per lane (kWave apart) of forward decide, forward chance and
 // backward decide items: a per-depth launch shape. Spots are planned at one
 // combo per lane; a list built for a depth merges each node's tiles to the
@@ -1722,6 +1723,13 @@ struct ppsg_engine {
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // one-spot engines (ppsg_create): kGraphIters iterations in one graph,
+  // captured once the one-iteration graph has been replayed by a second
+  // iterate call without a change
+  bool multiIter = false;
+  hipGraph_t graphK = nullptr;
+  hipGraphExec_t execK = nullptr;
+  int stableCalls = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double lastMs = -1.0;
   int* flagsDev = nullptr;  // maxSpots alive flags staged for setAliveFlags
@@ -1760,6 +1768,7 @@ struct ppsg_engine {
   int launchTerm(int tr, int ps, int part, hipStream_t s);
   int launchBwd(int tr, int dpt, int mode, int kindMask, hipStream_t s, int dry = 0);
   int launchHalf(int tr, int mode, hipStream_t s, int dry = 0);
+  int captureIters(int iters, hipGraph_t* g, hipGraphExec_t* x);
   // entries of group g (of spot S) at depth <= termSplit: a prefix of it
   int shallowCount(const Spot& S, const TermGroup& g) const {
     const auto b = S.termDepth.begin() + g.first;
@@ -1775,8 +1784,13 @@ void ppsg_engine::dropGraph() {
   if (stream) (void)hipStreamSynchronize(stream);
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
+  if (execK) (void)hipGraphExecDestroy(execK);
+  if (graphK) (void)hipGraphDestroy(graphK);
   exec = nullptr;
   graph = nullptr;
+  execK = nullptr;
+  graphK = nullptr;
+  stableCalls = 0;
   graphDirty = true;
 }
 
@@ -2336,23 +2350,36 @@ int ppsg_engine::launchHalf(int tr, int mode, hipStream_t s, int dry) {
   return 0;
 }
 
-int ppsg_engine::capture() {
-  if (prepare() != 0) return -1;
-  if (exec && !graphDirty) return 0;
-  dropGraph();
+// Captures `iters` iterations (both half-steps and the counter bump each)
+// into one graph.
+int ppsg_engine::captureIters(int iters, hipGraph_t* g, hipGraphExec_t* x) {
   HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-  int rc = launchHalf(0, PPSG_MODE_CFR, stream);
-  if (rc == 0) rc = launchHalf(1, PPSG_MODE_CFR, stream);
-  if (rc == 0) advanceCounters<<<1, kBlock, 0, stream>>>(ctl, liveDev, (int)live.size());
-  hipGraph_t g = nullptr;
-  const hipError_t e = hipStreamEndCapture(stream, &g);
+  int rc = 0;
+  for (int i = 0; i < iters && rc == 0; ++i) {
+    rc = launchHalf(0, PPSG_MODE_CFR, stream);
+    if (rc == 0) rc = launchHalf(1, PPSG_MODE_CFR, stream);
+    if (rc == 0) advanceCounters<<<1, kBlock, 0, stream>>>(ctl, liveDev, (int)live.size());
+  }
+  hipGraph_t cg = nullptr;
+  const hipError_t e = hipStreamEndCapture(stream, &cg);
   if (rc != 0) {
-    if (g) (void)hipGraphDestroy(g);
+    if (cg) (void)hipGraphDestroy(cg);
     return rc;
   }
   if (e != hipSuccess) return fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-  graph = g;
-  HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  *g = cg;
+  HIP_TRY(hipGraphInstantiate(x, cg, nullptr, nullptr, 0));
+  return 0;
+}
+
+int ppsg_engine::capture() {
+  if (prepare() != 0) return -1;
+  if (exec && !graphDirty) {
+    ++stableCalls;
+    return 0;
+  }
+  dropGraph();
+  if (captureIters(1, &graph, &exec) != 0) return -1;
   graphDirty = false;
   return 0;
 }
@@ -2441,8 +2468,18 @@ int iterate(ppsg_t* h, int n) {
   if (h->live.empty()) return fail("no live spot");
   if (h->covers(n) != 0) return -1;
   if (h->capture() != 0) return -1;
+  // a one-spot solve replays the same lists call after call: kGraphIters
+  // iterations per replay save the gaps between replays (~4 % of the plain
+  // turn's iteration). Batches change their lists every few steps (spots
+  // come and go), where the extra capture cost more than it saved.
+  if (h->multiIter && !h->execK && h->stableCalls > 0 && n >= 2 * kGraphIters &&
+      h->captureIters(kGraphIters, &h->graphK, &h->execK) != 0)
+    return -1;
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
-  for (int i = 0; i < n; ++i) HIP_TRY(hipGraphLaunch(h->exec, h->stream));
+  int i = 0;
+  if (h->execK)
+    for (; i + kGraphIters <= n; i += kGraphIters) HIP_TRY(hipGraphLaunch(h->execK, h->stream));
+  for (; i < n; ++i) HIP_TRY(hipGraphLaunch(h->exec, h->stream));
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
   HIP_TRY(hipEventSynchronize(h->ev1));
   float ms = 0.0f;
@@ -2789,6 +2826,7 @@ int ppsg_create(const ppsg_tree_desc* t, ppsg_t** out) {
   int64_t need = 0;
   if (ppsg_footprint(t, &need) != 0) return -1;
   auto* E = new ppsg_engine();
+  E->multiIter = true;
   int slot = -1;
   if (E->init(need, 1) != 0 || E->add(t, &slot) != 0) {
     delete E;
