@@ -249,12 +249,12 @@ std::vector<SpotResult> serve(const std::vector<SpotRequest>& reqs, const ServeO
 
   std::vector<int> slots, which;
   std::vector<double> sums;
+  std::vector<float*> noRows;  // walk_only: no root rows copied back
   // root value per pair of every wanted live spot: one walk of all live
   // spots, then the weighted root sums on the device (one launch, one copy
   // back; the same f64 values the host loop over copied rows gave)
   auto rootValues = [&](int tr, int mode, const std::vector<char>& want, std::vector<double>& v) {
     const int k = (int)live.size();
-    check(ppsg_batch_walk(h, tr, mode, 0, nullptr, nullptr), "batch walk");
     slots.clear();
     which.clear();
     for (int i = 0; i < k; ++i)
@@ -262,6 +262,15 @@ std::vector<SpotResult> serve(const std::vector<SpotRequest>& reqs, const ServeO
         slots.push_back(live[i].slot);
         which.push_back(i);
       }
+    // every live spot: one walk of all; fewer (the spots that retire): a
+    // walk of just those (the others' items masked), the same rows for them
+    if ((int)slots.size() == k)
+      check(ppsg_batch_walk(h, tr, mode, 0, nullptr, nullptr), "batch walk");
+    else if (!slots.empty()) {
+      noRows.assign(slots.size(), nullptr);
+      check(ppsg_batch_walk_only(h, tr, mode, (int)slots.size(), slots.data(), noRows.data()),
+            "batch walk");
+    }
     sums.assign(slots.size(), 0.0);
     check(ppsg_batch_root_sums(h, tr, (int)slots.size(), slots.data(), sums.data()), "root sums");
     v.assign(k, 0.0);
