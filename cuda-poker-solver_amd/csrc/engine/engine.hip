@@ -1674,6 +1674,28 @@ int scratchAddress(char** out) {
 // Any change re-captures the iteration graph on the next step.
 // ---------------------------------------------------------------------------
 
+namespace {
+
+// Copies and fills in an engine's stream order, waited for on that stream.
+// Nothing goes through the legacy null stream: a graph capture running in
+// another thread (another engine's) makes null-stream work fail instead of
+// waiting on it.
+int copySync(void* dst, const void* src, size_t n, hipMemcpyKind kind, hipStream_t s) {
+  if (n == 0) return 0;
+  HIP_TRY(hipMemcpyAsync(dst, src, n, kind, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+int fillSync(void* dst, int v, size_t n, hipStream_t s) {
+  if (n == 0) return 0;
+  HIP_TRY(hipMemsetAsync(dst, v, n, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+}  // namespace
+
 struct ppsg_engine {
   Arena arena;
   Ctl* ctl = nullptr;
@@ -1830,6 +1852,8 @@ int ppsg_engine::init(int64_t bytes, int spotsCap) {
     HIP_TRY(hipHostMalloc(&hp, kSumStageBytes, hipHostMallocDefault));
     sumHost = static_cast<char*>(hp);
   }
+  HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
   bytes = roundUp(bytes);
   void* p = nullptr;
   HIP_TRY(hipMalloc(&p, (size_t)bytes));
@@ -1838,13 +1862,11 @@ int ppsg_engine::init(int64_t bytes, int spotsCap) {
   arena.holes[0] = bytes;
   HIP_TRY(hipMalloc(&p, sizeof(Ctl) * (size_t)spotsCap));
   ctl = static_cast<Ctl*>(p);
-  HIP_TRY(hipMemset(ctl, 0, sizeof(Ctl) * (size_t)spotsCap));
+  if (fillSync(ctl, 0, sizeof(Ctl) * (size_t)spotsCap, stream) != 0) return -1;
   HIP_TRY(hipMalloc(&p, sizeof(int) * (size_t)spotsCap));
   flagsDev = static_cast<int*>(p);
   maxSpots = spotsCap;
   spots.assign(spotsCap, Spot{});
-  HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  HIP_TRY(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&ev0));
   HIP_TRY(hipEventCreate(&ev1));
   HIP_TRY(hipEventCreateWithFlags(&forkEv, hipEventDisableTiming));
@@ -1853,14 +1875,13 @@ int ppsg_engine::init(int64_t bytes, int spotsCap) {
 }
 
 int ppsg_engine::setAlive(int slot, int v) {
-  HIP_TRY(hipStreamSynchronize(stream));
-  HIP_TRY(hipMemcpy(&ctl[slot].alive, &v, sizeof(int), hipMemcpyHostToDevice));
-  return 0;
+  return copySync(&ctl[slot].alive, &v, sizeof(int), hipMemcpyHostToDevice, stream);
 }
 
 int ppsg_engine::setAliveAll(const std::vector<int>& flags) {
-  HIP_TRY(hipStreamSynchronize(stream));
-  HIP_TRY(hipMemcpy(flagsDev, flags.data(), sizeof(int) * (size_t)maxSpots, hipMemcpyHostToDevice));
+  if (copySync(flagsDev, flags.data(), sizeof(int) * (size_t)maxSpots, hipMemcpyHostToDevice,
+               stream) != 0)
+    return -1;
   const unsigned g = (unsigned)std::min<int64_t>(1024, (maxSpots + kBlock - 1) / kBlock);
   setAliveFlags<<<g, kBlock, 0, stream>>>(ctl, flagsDev, maxSpots);
   HIP_TRY(hipGetLastError());
@@ -1939,20 +1960,22 @@ int64_t ppsg_engine::countItems(int slot) const {
 
 namespace {
 
-int uploadTo(DevList& dl, const std::vector<Item>& v, bool append, int lanes = kBlock) {
+int uploadTo(DevList& dl, const std::vector<Item>& v, bool append, hipStream_t s,
+             int lanes = kBlock) {
   const int64_t at0 = append ? dl.count : 0;
   const int64_t need = at0 + (int64_t)v.size();
   if (need > dl.cap) {
     const int64_t cap = std::max<int64_t>(need, dl.cap + dl.cap / 2);
     void* p = nullptr;
     HIP_TRY(hipMalloc(&p, (size_t)cap * sizeof(Item)));
-    if (at0) HIP_TRY(hipMemcpy(p, dl.buf, (size_t)at0 * sizeof(Item), hipMemcpyDeviceToDevice));
+    if (at0 && copySync(p, dl.buf, (size_t)at0 * sizeof(Item), hipMemcpyDeviceToDevice, s) != 0)
+      return -1;
     if (dl.buf) (void)hipFree(dl.buf);
     dl.buf = static_cast<Item*>(p);
     dl.cap = cap;
   }
-  if (!v.empty())
-    HIP_TRY(hipMemcpy(dl.buf + at0, v.data(), v.size() * sizeof(Item), hipMemcpyHostToDevice));
+  if (copySync(dl.buf + at0, v.data(), v.size() * sizeof(Item), hipMemcpyHostToDevice, s) != 0)
+    return -1;
   dl.count = need;
   for (const Item& it : v)
     if (it.kind >= kBwdFold) dl.lds = std::max(dl.lds, terminalLdsOf(it, lanes));
@@ -1974,7 +1997,8 @@ int ppsg_engine::appendSpot(int slot) {
       if ((int)L.size() < nl) L.resize(nl);
       for (int dep = 0; dep < nl; ++dep) {
         const std::vector<Item> v = spotItems(slot, tr, ps, dep);
-        if (!v.empty() && uploadTo(L[dep], v, true, ps == 4 ? kWideBlock : kBlock) != 0) return -1;
+        if (!v.empty() && uploadTo(L[dep], v, true, stream, ps == 4 ? kWideBlock : kBlock) != 0)
+          return -1;
       }
     }
   S.itemCount = countItems(slot);
@@ -2080,7 +2104,7 @@ int ppsg_engine::rebuildList(int tr, int ps, int dep) {
   }
   DevList& l = lists[tr][ps][dep];
   l.lds = 0;
-  return uploadTo(l, v, false, kBlock);
+  return uploadTo(l, v, false, stream, kBlock);
 }
 
 int ppsg_engine::rebuild() {
@@ -2125,7 +2149,7 @@ int ppsg_engine::rebuild() {
             return (int64_t)x.cnt * x.m > (int64_t)y.cnt * y.m;
           });
         L[dep].lds = 0;
-        if (uploadTo(L[dep], v, false, ps == 4 ? kWideBlock : kBlock) != 0) return -1;
+        if (uploadTo(L[dep], v, false, stream, ps == 4 ? kWideBlock : kBlock) != 0) return -1;
       }
     }
   for (int s = 0; s < maxSpots; ++s)
@@ -2169,7 +2193,8 @@ int ppsg_engine::add(const ppsg_tree_desc* t, int* slot) {
   HIP_TRY(hipStreamSynchronize(stream));
   char* base = arena.base + off;
   auto put = [&](int64_t rel, const void* src, int64_t bytes) -> int {
-    if (bytes > 0) HIP_TRY(hipMemcpy(base + rel, src, (size_t)bytes, hipMemcpyHostToDevice));
+    if (bytes > 0 && copySync(base + rel, src, (size_t)bytes, hipMemcpyHostToDevice, stream) != 0)
+      return -1;
     return 0;
   };
   const Layout& L = S.L;
@@ -2189,12 +2214,12 @@ int ppsg_engine::add(const ppsg_tree_desc* t, int* slot) {
       put(L.termNodes, T.termNodes.data(), (int64_t)(T.termNodes.size() * sizeof(TermNode))))
     rc = -1;
   // untrained rows: zero regret and strategy sums (uniform average strategy)
-  if (rc == 0 && (hipMemset(base + L.regret, 0, 4 * (size_t)t->reg_total) != hipSuccess ||
-                  hipMemset(base + L.strat, 0, 4 * (size_t)t->reg_total) != hipSuccess))
+  if (rc == 0 && (fillSync(base + L.regret, 0, 4 * (size_t)t->reg_total, stream) != 0 ||
+                  fillSync(base + L.strat, 0, 4 * (size_t)t->reg_total, stream) != 0))
     rc = fail("row init failed");
   Ctl c{};
   c.alive = 1;
-  if (rc == 0 && hipMemcpy(ctl + free, &c, sizeof(Ctl), hipMemcpyHostToDevice) != hipSuccess)
+  if (rc == 0 && copySync(ctl + free, &c, sizeof(Ctl), hipMemcpyHostToDevice, stream) != 0)
     rc = fail("control block init failed");
   if (rc != 0) {
     arena.release(off, S.bytes);
@@ -2275,7 +2300,8 @@ int ppsg_engine::prepare() {
     liveCap = maxSpots;
   }
   if (!live.empty())
-    HIP_TRY(hipMemcpy(liveDev, live.data(), live.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (copySync(liveDev, live.data(), live.size() * sizeof(int), hipMemcpyHostToDevice, stream) != 0)
+      return -1;
   return 0;
 }
 
@@ -2416,10 +2442,12 @@ int setSchedule(ppsg_t* h, int slot, const float* coef, int64_t n) {
     S.coefCap = cap;
     // kernels read the table through the spot's control block: no re-capture
     const float* dp = S.coefDev;
-    HIP_TRY(hipMemcpy(&h->ctl[slot].coef, &dp, sizeof(dp), hipMemcpyHostToDevice));
+    if (copySync(&h->ctl[slot].coef, &dp, sizeof(dp), hipMemcpyHostToDevice, h->stream) != 0)
+      return -1;
   }
-  HIP_TRY(hipMemcpy(S.coefDev, coef, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemset(S.coefDev + 3 * n, 0, 3 * sizeof(float)));
+  if (copySync(S.coefDev, coef, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice, h->stream) != 0 ||
+      fillSync(S.coefDev + 3 * n, 0, 3 * sizeof(float), h->stream) != 0)
+    return -1;
   S.coefLen = n;
   refreshCoef<<<1, 64, 0, h->stream>>>(h->ctl, slot, (long long)n);
   HIP_TRY(hipGetLastError());
@@ -2432,7 +2460,8 @@ int setCounter(ppsg_t* h, int slot, int64_t t) {
   if (!h->valid(slot)) return fail("no such spot");
   HIP_TRY(hipStreamSynchronize(h->stream));
   const long long v = t;
-  HIP_TRY(hipMemcpy(&h->ctl[slot].counter, &v, sizeof(v), hipMemcpyHostToDevice));
+  if (copySync(&h->ctl[slot].counter, &v, sizeof(v), hipMemcpyHostToDevice, h->stream) != 0)
+    return -1;
   h->spots[slot].counter = t;
   refreshCoef<<<1, 64, 0, h->stream>>>(h->ctl, slot, (long long)h->spots[slot].coefLen);
   HIP_TRY(hipGetLastError());
@@ -2445,7 +2474,8 @@ int getCounter(ppsg_t* h, int slot, int64_t* t) {
   if (!h->valid(slot)) return fail("no such spot");
   HIP_TRY(hipStreamSynchronize(h->stream));
   long long v = 0;
-  HIP_TRY(hipMemcpy(&v, &h->ctl[slot].counter, sizeof(v), hipMemcpyDeviceToHost));
+  if (copySync(&v, &h->ctl[slot].counter, sizeof(v), hipMemcpyDeviceToHost, h->stream) != 0)
+    return -1;
   *t = v;
   return 0;
 }
@@ -2456,8 +2486,9 @@ int resetSpot(ppsg_t* h, int slot) {
   const Spot& S = h->spots[slot];
   char* base = h->arena.base + S.off;
   HIP_TRY(hipStreamSynchronize(h->stream));
-  HIP_TRY(hipMemset(base + S.L.regret, 0, 4 * (size_t)S.regTotal));
-  HIP_TRY(hipMemset(base + S.L.strat, 0, 4 * (size_t)S.regTotal));
+  if (fillSync(base + S.L.regret, 0, 4 * (size_t)S.regTotal, h->stream) != 0 ||
+      fillSync(base + S.L.strat, 0, 4 * (size_t)S.regTotal, h->stream) != 0)
+    return -1;
   return setCounter(h, slot, 0);
 }
 
@@ -2502,9 +2533,9 @@ int walk(ppsg_t* h, int tr, int mode, int nSlots, const int* slots, float* const
   HIP_TRY(hipStreamSynchronize(h->stream));
   for (int i = 0; i < nSlots; ++i) {
     const Spot& S = h->spots[slots[i]];
-    if (rootCfv[i])
-      HIP_TRY(hipMemcpy(rootCfv[i], h->arena.base + S.off + S.rootCfv, sizeof(float) * S.n[tr],
-                        hipMemcpyDeviceToHost));
+    if (rootCfv[i] && copySync(rootCfv[i], h->arena.base + S.off + S.rootCfv,
+                               sizeof(float) * S.n[tr], hipMemcpyDeviceToHost, h->stream) != 0)
+      return -1;
   }
   return 0;
 }
@@ -2678,9 +2709,9 @@ int walkOnly(ppsg_t* h, int tr, int mode, int nSlots, const int* slots, float* c
   if (rc2 != 0) return rc2;
   for (int i = 0; i < nSlots; ++i) {
     const Spot& S = h->spots[slots[i]];
-    if (rootCfv[i])
-      HIP_TRY(hipMemcpy(rootCfv[i], h->arena.base + S.off + S.rootCfv, sizeof(float) * S.n[tr],
-                        hipMemcpyDeviceToHost));
+    if (rootCfv[i] && copySync(rootCfv[i], h->arena.base + S.off + S.rootCfv,
+                               sizeof(float) * S.n[tr], hipMemcpyDeviceToHost, h->stream) != 0)
+      return -1;
   }
   return 0;
 }
@@ -2707,8 +2738,8 @@ int transfer(ppsg_t* h, int slot, int buf, int64_t off, int64_t n, void* host, b
   HIP_TRY(hipStreamSynchronize(h->stream));
   char* dev = h->arena.base + S.off + rel + 4 * off;
   if (n) {
-    if (toDevice) HIP_TRY(hipMemcpy(dev, host, (size_t)n * 4, hipMemcpyHostToDevice));
-    else HIP_TRY(hipMemcpy(host, dev, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (toDevice) return copySync(dev, host, (size_t)n * 4, hipMemcpyHostToDevice, h->stream);
+    return copySync(host, dev, (size_t)n * 4, hipMemcpyDeviceToHost, h->stream);
   }
   return 0;
 }

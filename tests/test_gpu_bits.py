@@ -76,3 +76,40 @@ def test_stream_group_sizes_leave_results_unchanged():
     got = [(r["iterations"], r["expl"], r["ev0"], r["ev1"]) for r in streamed]
     want = [(r["iterations"], r["expl"], r["ev0"], r["ev1"]) for r in alone]
     assert got == want
+
+
+def test_solvers_in_two_threads():
+    """Two solvers, each with its own engine, solved at the same time from
+    two threads: one thread's graph capture must not break the other's
+    copies (the engines keep every copy and fill on their own streams), and
+    both give the pinned tables."""
+    import threading
+    cases = {"plain-turn": (configs.plain_turn, 100, "cc053de37075"),
+             "large-turn": (configs.large_turn, 60, "5e709bc7ce5a")}
+    got, errs = {}, []
+
+    def run(name):
+        make, iters, _ = cases[name]
+        spot = make()
+        try:
+            for _ in range(2):  # twice: captures and copies interleave
+                s = capi.Solver(*spot_cfg(spot))
+                try:
+                    s.solve(iters, spot["algo"])
+                    h = hashlib.sha256()
+                    for which in (0, 1):
+                        h.update(np.asarray(s.rows(which), dtype=np.float32).tobytes())
+                    got.setdefault(name, []).append(h.hexdigest()[:12])
+                finally:
+                    s.close()
+        except Exception as e:  # surfaced below
+            errs.append(f"{name}: {e}")
+
+    th = [threading.Thread(target=run, args=(n,)) for n in cases]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errs, errs
+    for name, (_, _, digest) in cases.items():
+        assert got[name] == [digest, digest], (name, got[name])
