@@ -41,6 +41,11 @@ the line reports how many converged.
                   the captured graph), with their own roofline; the large
                   flop also with the CPU oracle on the same tree (a bounded
                   sample of whole iterations, --flop-cpu-seconds)
+  reference_geometries  the solves the reference publishes a GPU rate for
+                  (its gate ranges and bench geometries: four turn spots,
+                  two practical turns, the standard and wide flops; same
+                  node counts), timed as its bench scripts time them, with
+                  the published RTX 4070 rate beside ours
 
 --workload large-turn / plain-turn / large-flop measures one spot alone
 (2000 / 500 / 1000 iterations from the captured graph) instead.
@@ -369,6 +374,8 @@ def bench_batched(args, capi, configs):
                                         ("large-flop", args.flop_iters,
                                          0.0 if args.no_cpu else args.flop_cpu_seconds))}
 
+    ref_geo = None if args.no_reference_geometries else reference_geometries(capi)
+
     n_conv = {k: (v["converged"], v["spots"]) for k, v in
               (("batched", batched), ("single", single), ("serving", serving)) if v}
     line = {
@@ -401,6 +408,7 @@ def bench_batched(args, capi, configs):
             "serving": serving,
             "http": http,
             "single_spot": single_spot,
+            "reference_geometries": ref_geo,
             "launch_shapes": shapes_summary(prof["shapes"]),
             "resident_batch": {
                 "steps": args.steps, "warmup": args.warmup,
@@ -527,6 +535,60 @@ def single_spot_record(capi, configs, workload, iters, warmup, cpu_seconds=0.0):
     return rec
 
 
+# The reference's own published solves (docs/STATUS.md:63-89: RTX 4070,
+# the gate ranges of tools/quality/bench_turn.sh:14-15, --algo dcfr):
+# label, board, pot, stack, bets, raises, published node count, iterations
+# timed here, published iters/s (low, high)
+REFERENCE_RANGES = ("22+,A9s+,KTs+,QJs,JTs,T9s", "22+,A9s+,KTs+,QJs,JTs,T9s,AQo+")
+REFERENCE_GEOMETRIES = [
+    ("turn-shortstack", "Qs9h2d7c", 500, 150, "0.5,0.75", "2.5,3.0", 537, 2000, (11240, 11240)),
+    ("turn-standard", "Qs9h2d7c", 200, 500, "0.5,0.75", "2.5,3.0", 5943, 2000, (4650, 5288)),
+    ("turn-wide", "Qs9h2d7c", 300, 1500, "0.25,0.5,0.75,1.25", "2.5,3.0", 51903, 2000, (747, 807)),
+    ("turn-deep", "AhTd9s6h", 300, 1500, "0.25,0.5,0.75,1.25", "2.5,3.0", 51903, 2000, (747, 807)),
+    ("turn-practical-500", "Qs9h2d7c", 200, 500, "0.75,a", "2.5,3", 2085, 2000, (6697, 6697)),
+    ("turn-practical-1500", "Qs9h2d7c", 300, 1500, "0.75,a", "2.5,3", 5943, 2000, (4283, 4283)),
+    ("flop-standard", "Qs9h2d", 200, 500, "0.5,0.75", "2.5,3.0", 615234, 200, (55, 55)),
+    ("flop-wide", "Qs9h2d", 300, 1500, "0.25,0.5,0.75,1.25", "2.5,3.0", 11716550, 20, (0.57, 0.57)),
+]
+
+
+def card_ids(board):
+    """Card ids (suit * 13 + rank, suits c d h s) of a board string."""
+    return ["cdhs".index(board[i + 1]) * 13 + "23456789TJQKA".index(board[i])
+            for i in range(0, len(board), 2)]
+
+
+def reference_geometries(capi):
+    """Our solve of each geometry the reference publishes a GPU rate for,
+    timed as its bench scripts time theirs: the iterations replayed after
+    the compile, no exploitability checks inside. Reports our rate beside
+    the published one (another GPU: RTX 4070) and the node count both
+    trees have."""
+    w0, w1 = (capi.parse_range(r) for r in REFERENCE_RANGES)
+    out = {}
+    for label, board, pot, stack, bets, raises, nodes, iters, (lo, hi) in REFERENCE_GEOMETRIES:
+        sizes = [float(t) for t in bets.split(",") if t != "a"]
+        cfg = capi.make_config(sizes, [float(t) for t in raises.split(",")], "a" in bets.split(","),
+                               False, 0)
+        sp = capi.make_spot(card_ids(board), w0, w1, pot, stack)
+        t0 = time.perf_counter()
+        solver = capi.Solver(sp, cfg)
+        compile_s = time.perf_counter() - t0
+        solver.solve(min(8, iters), "dcfr")  # graph capture, caches
+        t0 = time.perf_counter()
+        solver.solve(iters, "dcfr")
+        wall = time.perf_counter() - t0
+        rate = iters / wall
+        out[label] = {"nodes": solver.num_nodes, "published_nodes": nodes, "iterations": iters,
+                      "compile_upload_s": round(compile_s, 3), "iters_per_s": round(rate, 2),
+                      "ms_per_iter": round(wall * 1e3 / iters, 4),
+                      "published_iters_per_s_rtx4070": [lo, hi],
+                      "vs_published_high": round(rate / hi, 2)}
+        solver.close()
+        log(f"bench: reference geometry {label} {json.dumps(out[label])}")
+    return out
+
+
 def bench_single(args, capi, configs):
     spot = {"large-turn": configs.large_turn, "plain-turn": configs.plain_turn,
             "large-flop": configs.large_flop}[args.workload]()
@@ -586,6 +648,8 @@ def main():
                     help="the serving stream again at this arrival rate (its throughput when "
                          "the GPU is kept full) and one at a time; 0: skip")
     ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--no-reference-geometries", action="store_true",
+                    help="skip the solves of the reference's published geometries")
     ap.add_argument("--no-single-spot", action="store_true",
                     help="skip the plain turn / large turn / large flop records")
     ap.add_argument("--flop-iters", type=int, default=1000,
