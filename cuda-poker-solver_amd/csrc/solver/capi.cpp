@@ -489,6 +489,7 @@ void pps_serve_defaults(pps_serve_options* opt) {
   opt->shapes = nullptr;
   opt->n_shapes = 0;
   opt->term_split = 0;
+  opt->request_flags = nullptr;
 }
 
 int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const double* arrival_ms,
@@ -515,11 +516,17 @@ int pps_serve(int n, const pps_spot* spots, const pps_bet_config* cfgs, const do
     }
     if (algo) o.algo = algo;
     std::vector<pps::gpu::SpotRequest> reqs(n);
+    const int32_t* perReq = opt ? opt->request_flags : nullptr;
     for (int i = 0; i < n; ++i) {
       reqs[i].spot = toSpot(spots + i);
       reqs[i].cfg = toCfg(cfgs + i);
       reqs[i].arrivalMs = arrival_ms ? arrival_ms[i] : 0.0;
       reqs[i].options = spotOpt;
+      if (perReq) {
+        if (perReq[i] & ~PPS_SUIT_ISOMORPHISM)
+          throw std::runtime_error("request " + std::to_string(i) + ": unknown option flags");
+        reqs[i].options.suitIsomorphism = (perReq[i] & PPS_SUIT_ISOMORPHISM) != 0;
+      }
     }
     pps::gpu::ServeReport rep;
     const std::vector<pps::gpu::SpotResult> res = pps::gpu::serve(reqs, o, &rep);

@@ -33,7 +33,8 @@ class ServeOptions(ctypes.Structure):
                 ("compile_threads", ctypes.c_int32), ("real_time", ctypes.c_int32),
                 ("prefill", ctypes.c_int32), ("flags", ctypes.c_int32),
                 ("shapes", ctypes.POINTER(ctypes.c_int32)), ("n_shapes", ctypes.c_int32),
-                ("term_split", ctypes.c_int32)]
+                ("term_split", ctypes.c_int32),
+                ("request_flags", ctypes.POINTER(ctypes.c_int32))]
 
 
 class ServeResult(ctypes.Structure):
@@ -523,8 +524,15 @@ def serve(spots, cfgs, algo="dcfr", arrival_ms=None, **options):
         keep = (ctypes.c_int32 * len(shapes))(*shapes)
         o.shapes = ctypes.cast(keep, ctypes.POINTER(ctypes.c_int32))
         o.n_shapes = len(shapes)
+    req_flags = options.pop("request_flags", None)  # one option-flag word per request
+    keep_flags = None
+    if req_flags is not None:
+        if len(req_flags) != n:
+            raise ValueError("one request flag word per spot")
+        keep_flags = (ctypes.c_int32 * n)(*req_flags)
+        o.request_flags = ctypes.cast(keep_flags, ctypes.POINTER(ctypes.c_int32))
     for k, v in options.items():
-        if not hasattr(o, k) or k in ("shapes", "n_shapes"):
+        if not hasattr(o, k) or k in ("shapes", "n_shapes", "request_flags"):
             raise ValueError(f"unknown serve option {k}")
         setattr(o, k, v)
     S = (Spot * n)(*spots)
@@ -533,7 +541,7 @@ def serve(spots, cfgs, algo="dcfr", arrival_ms=None, **options):
     R = (ServeResult * n)()
     rep = ServeReport()
     _check(lib().pps_serve(n, S, C, A, algo.encode(), ctypes.byref(o), R, ctypes.byref(rep)))
-    del keep
+    del keep, keep_flags
     out = []
     for r in R:
         out.append({

@@ -152,6 +152,8 @@ typedef struct pps_serve_options {
   int32_t n_shapes;         /* entries in shapes */
   int32_t term_split;       /* terminal split depth (PPSG_TUNE_TERM_SPLIT, e.g.
                                from pps_batch_profile); 0: one stream */
+  const int32_t* request_flags; /* n option flags, one per request (they
+                               replace `flags` for it), or NULL */
 } pps_serve_options;
 
 typedef struct pps_serve_result {

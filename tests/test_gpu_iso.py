@@ -134,6 +134,34 @@ def test_serve_isomorphic_requests_equal_solvers():
         assert r["nodes"] < plain
 
 
+def test_serve_mixes_isomorphic_and_full_requests():
+    """pps_serve_options.request_flags: one option-flag word per request, so
+    one served batch mixes isomorphic and full-tree requests; each equals a
+    single-spot solve with its own option, run for the same iterations."""
+    from conftest import board_of
+    spots, flags = [], []
+    for k, board in enumerate(("Ks8s5s2s", "7s7h7d2c", "Qs9s2d8s", "Ks8s5s2s")):
+        sp = capi.make_spot(board_of(board), capi.parse_range(SYM_OOP), capi.parse_range(SYM_IP),
+                            60, 200)
+        spots.append((sp, capi.make_config([0.5], [2.5], True, False, 0)))
+        flags.append(capi.SUIT_ISOMORPHISM if k % 2 == 0 else 0)
+    res, _ = capi.serve([p[0] for p in spots], [p[1] for p in spots], "dcfr", None,
+                        arena_bytes=1 << 30, real_time=0, request_flags=flags)
+    for (sp, cf), f, r in zip(spots, flags, res):
+        assert r["ok"], r["error"]
+        iso = f == capi.SUIT_ISOMORPHISM
+        own = capi.Solver(sp, cf, isomorphism=iso)
+        own.solve(r["iterations"], "dcfr")
+        st = own.stats()
+        assert r["nodes"] == own.num_nodes
+        assert abs(r["ev0"] - st["ev_oop"]) <= 1e-12, (r["ev0"], st)
+        assert abs(r["expl"] - st["exploitability"]) <= 1e-12, (r["expl"], st)
+    assert res[0]["nodes"] < res[3]["nodes"]  # the same spot, isomorphic and full
+    with pytest.raises(capi.CapiError):
+        capi.serve([spots[0][0]], [spots[0][1]], "dcfr", None, arena_bytes=1 << 30,
+                   real_time=0, request_flags=[4])
+
+
 ISO_NAMES = ["iso-turn-monotone", "iso-turn-trips", "iso-flop-two-tone", "iso-flop-monotone"]
 
 
