@@ -113,3 +113,28 @@ def test_solvers_in_two_threads():
     assert not errs, errs
     for name, (_, _, digest) in cases.items():
         assert got[name] == [digest, digest], (name, got[name])
+
+
+def test_multi_iteration_replays_match_single_iteration_replays():
+    """A one-spot solver replays 8 iterations per graph once its lists are
+    stable (here in the second and third of its 128-iteration chunks); a
+    served spot, on a batch engine, replays one iteration per graph. The
+    same spot for 300 iterations both ways: the same EVs, exploitability
+    and root strategy."""
+    spot = configs.batch_turn(1001)
+    sp, cf = spot_cfg(spot)
+    r = capi.serve([sp], [cf], "dcfr", None, target_frac=0.0, check_every=300, max_iters=300,
+                   real_time=0, prefill=1)[0][0]
+    assert r["ok"] and r["iterations"] == 300
+    g = capi.Solver(sp, cf)
+    try:
+        g.solve(300, "dcfr")
+        st = g.stats()
+        # (the served figures come from the device root sums, the solver's
+        # from its own walks: the same f64 operations, held to 1e-12 as the
+        # batch tests hold them)
+        assert abs(st["ev_oop"] - r["ev0"]) <= 1e-12 and abs(st["ev_ip"] - r["ev1"]) <= 1e-12
+        assert abs(st["exploitability"] - r["expl"]) <= 1e-12
+        assert max(abs(a - b) for a, b in zip(g.root_strategy(), r["root_strategy"])) <= 1e-12
+    finally:
+        g.close()
