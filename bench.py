@@ -622,6 +622,8 @@ def bench_single(args, capi, configs):
         "roofline": roofline_of(prof),
         "cpu_baseline": None,
     }
+    line["roofline"]["traffic"] = counter_traffic(line["roofline"]["kernel"].split()[0],
+                                                  args.workload)
     solver.close()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline_single(spot, args.cpu_seconds)
