@@ -67,7 +67,8 @@ constexpr int kFoldPart = 12;
 // chunks past its longest part
 constexpr int kPartChunk = 3;
 constexpr int kMaxGroup = 64;  // terminal nodes per work item at most
-constexpr int kGraphIters = 8;  // iterations per replay of a stable solve
+constexpr int kGraphIters = 8;  // iterations per replay of a stable solve, at most
+constexpr int kGraphNodes = 256;  // kernels per multi-iteration graph, at most
 // Combos per lane (kWave apart) of forward decide, forward chance and
 // backward decide items: a per-depth launch shape. Spots are planned at one
 // combo per lane; a list built for a depth merges each node's tiles to the
@@ -1751,6 +1752,7 @@ struct ppsg_engine {
   bool multiIter = false;
   hipGraph_t graphK = nullptr;
   hipGraphExec_t execK = nullptr;
+  int itersK = 0;  // iterations in graphK
   int stableCalls = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double lastMs = -1.0;
@@ -2503,13 +2505,18 @@ int iterate(ppsg_t* h, int n) {
   // iterations per replay save the gaps between replays (~4 % of the plain
   // turn's iteration). Batches change their lists every few steps (spots
   // come and go), where the extra capture cost more than it saved.
-  if (h->multiIter && !h->execK && h->stableCalls > 0 && n >= 2 * kGraphIters &&
-      h->captureIters(kGraphIters, &h->graphK, &h->execK) != 0)
-    return -1;
+  // (as many iterations as fit kGraphNodes kernels, up to kGraphIters)
+  if (h->multiIter && !h->execK && h->stableCalls > 0) {
+    const int k = (int)std::min<int64_t>(kGraphIters, kGraphNodes / std::max<int64_t>(h->kernelsPerIter(), 1));
+    if (k >= 2 && n >= 2 * k) {
+      if (h->captureIters(k, &h->graphK, &h->execK) != 0) return -1;
+      h->itersK = k;
+    }
+  }
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
   int i = 0;
   if (h->execK)
-    for (; i + kGraphIters <= n; i += kGraphIters) HIP_TRY(hipGraphLaunch(h->execK, h->stream));
+    for (; i + h->itersK <= n; i += h->itersK) HIP_TRY(hipGraphLaunch(h->execK, h->stream));
   for (; i < n; ++i) HIP_TRY(hipGraphLaunch(h->exec, h->stream));
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
   HIP_TRY(hipEventSynchronize(h->ev1));
