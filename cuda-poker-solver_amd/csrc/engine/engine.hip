@@ -60,7 +60,10 @@ constexpr int kMaxList = 1326;
 constexpr int kRowRegs = (kMaxList + kBlock - 1) / kBlock;  // 6
 constexpr int64_t kAlign = 256;
 constexpr int kLists = 5;  // per traverser: forward, backward, fold, showdown, wide showdown
-constexpr int kGatherHeader = 256;  // bytes: 53 int32 segment bounds + pad
+// bytes: 53 int32 segment bounds, then the 52 cards in the order their
+// lane groups take them (longest segment first), then pad
+constexpr int kGatherHeader = 320;
+constexpr int kCardOrderAt = 53 * 4;  // byte offset of the card order
 // fold: entries of a card's segment per lane (4 lanes per card)
 constexpr int kFoldPart = 12;
 // a lane's part is gathered in chunks of this many entries; a wave skips the
@@ -411,8 +414,11 @@ __device__ void foldGroup(const Dev& d, const Item& it, char* lds) {
   // this lane's part of its card's segment
   uint16_t idx[kFoldPart];
   int partMax;  // the wave's longest part
+  // the card of this lane group (the groups take the cards longest segment
+  // first); lane groups 52.. load card order[51] and drop it
+  const int card = at<uint8_t>(d, it.c + kCardOrderAt)[min(tid >> 2, 51)];
   {
-    const int c = min(tid >> 2, 51), part = tid & 3;
+    const int c = card, part = tid & 3;
     const int32_t* so = at<int32_t>(d, it.c);
     const int s0 = so[c], sl = so[c + 1] - s0;
     const int lo = s0 + (sl * part) / 4;
@@ -450,7 +456,7 @@ __device__ void foldGroup(const Dev& d, const Item& it, char* lds) {
     }
     part += __shfl_xor(part, 1, 64);
     part += __shfl_xor(part, 2, 64);
-    if ((tid & 3) == 0 && (tid >> 2) < 52) cs[tid >> 2] = part;
+    if ((tid & 3) == 0 && (tid >> 2) < 52) cs[card] = part;
     __syncthreads();
     if (tid < 64) {  // total = half the sum of the card sums
       double x = tid < 52 ? cs[tid] : 0.0;
@@ -559,14 +565,15 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
   // drop it: a load under a per-lane condition is compiled to a branch with
   // its own wait, one round trip after another
   const uint2 gs = at<uint2>(d, it.c + kGatherHeader)[tid];
-  const int sc = tid >> 2, part = tid & 3;  // lanes of cards 52.. have no segment
+  const int sg = tid >> 2, part = tid & 3;  // lane groups 52.. have no segment
+  // the card of this lane group (the groups take the cards longest segment first)
+  const int sc = at<uint8_t>(d, it.c + kCardOrderAt)[min(sg, 51)];
   int segLo, segLen;
   {
     const int32_t* so = at<int32_t>(d, it.c);
-    const int cl = min(sc, 51);  // load index only
-    const int s0 = so[cl], sl = so[cl + 1] - s0;
-    segLo = sc < 52 ? s0 + (sl * part) / 4 : 0;
-    segLen = sc < 52 ? s0 + (sl * (part + 1)) / 4 - segLo : 0;
+    const int s0 = so[sc], sl = so[sc + 1] - s0;
+    segLo = sg < 52 ? s0 + (sl * part) / 4 : 0;
+    segLen = sg < 52 ? s0 + (sl * (part + 1)) / 4 - segLo : 0;
   }
   uint16_t idx[kFoldPart];
   {
@@ -646,7 +653,7 @@ __device__ void showdownGroup(const Dev& d, const Item& it, char* lds) {
     // 3. segment-relative prefixes: the quad's four parts, in order
     const double p0 = dppD<0x00>(segSum), p1 = dppD<0x55>(segSum);
     const double p2 = dppD<0xaa>(segSum), p3 = dppD<0xff>(segSum);
-    if (sc < 52) {
+    if (sg < 52) {
       double y = part == 0 ? 0.0 : part == 1 ? p0 : part == 2 ? p0 + p1 : (p0 + p1) + p2;
       float* yo = Y + segLo + sc;
 #pragma unroll
@@ -1208,7 +1215,17 @@ struct SpotTables {
     offs[cl] = (int64_t)g.size();
     const size_t at0 = g.size();
     g.resize(at0 + kGatherHeader + list.size() * 2, 0);
-    std::memcpy(g.data() + at0, t->seg_off[p] + (int64_t)cl * 53, 53 * 4);
+    const int32_t* so = t->seg_off[p] + (int64_t)cl * 53;
+    std::memcpy(g.data() + at0, so, 53 * 4);
+    // lane group k (lanes 4k..4k+3) takes card order[k]: longest segments
+    // first, so the 16 cards of a wave have similar segment lengths and a
+    // wave's longest part (which sets its gather chunks) is close to the rest
+    uint8_t order[52];
+    for (int c = 0; c < 52; ++c) order[c] = (uint8_t)c;
+    std::stable_sort(order, order + 52, [&](uint8_t a, uint8_t b) {
+      return so[a + 1] - so[a] > so[b + 1] - so[b];
+    });
+    std::memcpy(g.data() + at0 + kCardOrderAt, order, 52);
     std::memcpy(g.data() + at0 + kGatherHeader, list.data(), list.size() * 2);
   }
 };
